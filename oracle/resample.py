@@ -1,0 +1,202 @@
+"""SampleGridLut / SampleGridTransform and their gradients (FP64).
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+Shapes follow the ops (resample_grid.cc:23-172): input ``[N, D, H, W, C]``,
+lookup ``[V, D', H', W', 4]`` (x, y, z, LoD), output ``[N, V, D', H', W', C]``
+when ``separate_camera_batch`` (the reference's ``globalSampling``) is true,
+else ``[N, 1, D', H', W', C]`` with instance n using camera n
+(resample_grid.cu.cc:728-739).
+"""
+import numpy as np
+
+from . import BORDER, LINEAR, NEAREST, MIN, MAX
+from . import sampling as S
+from . import geometry as G
+
+
+def _index_grid(d, h, w):
+    z, y, x = np.meshgrid(np.arange(d), np.arange(h), np.arange(w), indexing="ij")
+    return np.stack([x, y, z], axis=-1).astype(np.float64)
+
+
+def _pairs(N, V, separate_camera_batch):
+    if separate_camera_batch:
+        return [(n, v, v) for n in range(N) for v in range(V)]
+    if V != N:
+        raise ValueError("camera batch must match data batch when not using global sampling.")
+    return [(n, n, 0) for n in range(N)]
+
+
+def lut_positions(lut_v, relative, normalized):
+    """Sampling position from one LUT slice (resample_grid.cu.cc:610-618):
+    ``normalized`` scales by the *output* dims, ``relative`` adds the integer
+    output index."""
+    d, h, w = lut_v.shape[:3]
+    pos = lut_v[..., :3].astype(np.float64).copy()
+    if normalized:
+        pos = pos * np.array([w, h, d], dtype=np.float64)
+    if relative:
+        pos = pos + _index_grid(d, h, w)
+    return pos
+
+
+def sample_grid_lut(inp, lut, filter_mode=LINEAR, boundary=BORDER, cell_center_offset=0.0,
+                    separate_camera_batch=True, relative=True, normalized=False):
+    inp = np.asarray(inp, dtype=np.float64)
+    lut = np.asarray(lut, dtype=np.float64)
+    N = inp.shape[0]
+    V = lut.shape[0]
+    Vo = V if separate_camera_batch else 1
+    out = np.zeros((N, Vo) + lut.shape[1:4] + (inp.shape[-1],))
+    for n, v, vo in _pairs(N, V, separate_camera_batch):
+        q = lut_positions(lut[v], relative, normalized) - cell_center_offset
+        out[n, vo] = S.sample(inp[n], q, filter_mode, boundary)
+    return out
+
+
+def sample_grid_lut_grad(inp, lut, out_grad, filter_mode=LINEAR, boundary=BORDER, cell_center_offset=0.0,
+                         separate_camera_batch=True, relative=True, normalized=False, semantics="exact"):
+    """Returns (input_grad, lookup_grad).
+
+    semantics="exact": the true adjoint of ``sample_grid_lut``.  The LUT
+      gradient is accumulated over the data batch and carries the chain-rule
+      factor of the ``normalized`` scaling; NEAREST / MIN / MAX route the input
+      gradient to the cell the forward read from.
+    semantics="reference": SampleGridLutGrad (resample_grid.cu.cc:1420-1665):
+      input gradient = trilinear scatter with the reference's BORDER mask for
+      every filter, divided per voxel by the number of corner hits summed over
+      the cameras of that batch entry; LUT gradient (LINEAR only) written, not
+      accumulated, per camera, so with a data batch the last batch entry wins,
+      and it ignores the ``normalized`` scaling.  The count buffer is zeroed
+      (the reference does not zero it before first use).
+    """
+    inp = np.asarray(inp, dtype=np.float64)
+    lut = np.asarray(lut, dtype=np.float64)
+    out_grad = np.asarray(out_grad, dtype=np.float64)
+    N, D, H, W, C = inp.shape
+    V = lut.shape[0]
+    in_grad = np.zeros_like(inp)
+    lut_grad = np.zeros_like(lut)
+    d, h, w = lut.shape[1:4]
+    for n in range(N):
+        count = np.zeros((D, H, W), dtype=np.int64) if semantics == "reference" else None
+        for nn, v, vo in _pairs(N, V, separate_camera_batch):
+            if nn != n:
+                continue
+            q = lut_positions(lut[v], relative, normalized) - cell_center_offset
+            g = out_grad[n, vo]
+            if semantics == "reference":
+                S.scatter_linear(inp.shape[1:], q, g, boundary, mask="reference", count=count, out=in_grad[n])
+                if filter_mode == LINEAR:
+                    dx, dy, dz = S.data_gradient(inp[n], q, boundary)
+                    lut_grad[v, ..., 0] = np.sum(g * dx, axis=-1)
+                    lut_grad[v, ..., 1] = np.sum(g * dy, axis=-1)
+                    lut_grad[v, ..., 2] = np.sum(g * dz, axis=-1)
+                    lut_grad[v, ..., 3] = 0.0
+            else:
+                if filter_mode == LINEAR:
+                    S.scatter_linear(inp.shape[1:], q, g, boundary, mask="forward", out=in_grad[n])
+                    dx, dy, dz = S.data_gradient(inp[n], q, boundary)
+                    sc = np.array([w, h, d], dtype=np.float64) if normalized else np.ones(3)
+                    lut_grad[v, ..., 0] += np.sum(g * dx, axis=-1) * sc[0]
+                    lut_grad[v, ..., 1] += np.sum(g * dy, axis=-1) * sc[1]
+                    lut_grad[v, ..., 2] += np.sum(g * dz, axis=-1) * sc[2]
+                elif filter_mode == NEAREST:
+                    S.scatter_nearest(inp.shape[1:], q, g, boundary, out=in_grad[n])
+                else:
+                    S.scatter_minmax(inp[n], q, g, boundary, filter_mode == MAX, out=in_grad[n])
+        if semantics == "reference":
+            nz = count > 0
+            in_grad[n][nz] = in_grad[n][nz] / count[nz][:, None]
+    return in_grad, lut_grad
+
+
+# --- transform sampling -----------------------------------------------------
+
+TRANSFORM_LINDEPTH, TRANSFORM_LINDEPTH_REVERSE, TRANSFORM, TRANSFORM_REVERSE = 0, 1, 3, 4
+COORDINATE_MODE = {"TRANSFORM_LINDEPTH": 0, "TRANSFORM_LINDEPTH_REVERSE": 1, "TRANSFORM": 3, "TRANSFORM_REVERSE": 4}
+
+
+def transform_positions(cx, out_dhw, in_dhw, coordinate_mode):
+    """samplePos3D<CM,false> on every output cell centre (sampling.hpp:151-228)."""
+    d, h, w = out_dhw
+    idx = _index_grid(d, h, w) + 0.5
+    out_dims = np.array([w, h, d], dtype=np.float64)
+    in_dims = np.array([in_dhw[2], in_dhw[1], in_dhw[0]], dtype=np.float64)
+    if coordinate_mode == TRANSFORM:
+        return G.idx_to_os(cx, idx, out_dims)
+    if coordinate_mode == TRANSFORM_LINDEPTH:
+        return G.idx_to_os_lindepth(cx, idx, out_dims)
+    if coordinate_mode == TRANSFORM_REVERSE:
+        return G.os_to_idx(cx, idx, in_dims)
+    if coordinate_mode == TRANSFORM_LINDEPTH_REVERSE:
+        return G.os_to_idx_lindepth(cx, idx, in_dims)
+    raise ValueError("Invalid coordinate_mode")
+
+
+def _transform_pairs(N, V, separate_camera_batch):
+    return _pairs(N, V, separate_camera_batch)
+
+
+def _sample2(grid, q, filter_mode, boundary):
+    """Sampler2 (sampling.hpp:802-826): BORDER/CLAMP only, WRAP reads 0;
+    modes other than LINEAR / QUADRATIC fall to NEAREST."""
+    from . import WRAP
+    if boundary == WRAP:
+        return np.zeros(q.shape[:-1] + (grid.shape[-1],))
+    if filter_mode == LINEAR:
+        return S.sample_linear(grid, q, boundary)
+    return S.sample_nearest(grid, q, boundary)
+
+
+def sample_grid_transform(inp, M, V, P, F, out_dhw, filter_mode=LINEAR, boundary=BORDER,
+                          coordinate_mode=TRANSFORM_LINDEPTH, separate_camera_batch=True):
+    """SampleGridTransform (resample_grid.cu.cc:255-544).  M/V/P are op arguments
+    (flat 16 floats each, per batch / per camera), F is [V, 6]."""
+    inp = np.asarray(inp, dtype=np.float64)
+    N = inp.shape[0]
+    nV = len(V)
+    Vo = nV if separate_camera_batch else 1
+    out = np.zeros((N, Vo) + tuple(out_dhw) + (inp.shape[-1],))
+    for n, v, vo in _transform_pairs(N, nV, separate_camera_batch):
+        cx = G.CameraXform.from_op_args(M[n], V[v], P[v], F[v])
+        q = transform_positions(cx, out_dhw, inp.shape[1:4], coordinate_mode) - 0.5
+        out[n, vo] = _sample2(inp[n], q, filter_mode, boundary)
+    return out
+
+
+def sample_grid_transform_grad(inp, out_grad, M, V, P, F, filter_mode=LINEAR, boundary=BORDER,
+                               coordinate_mode=TRANSFORM_LINDEPTH, separate_camera_batch=True, semantics="exact"):
+    """SampleGridTransformGrad.  semantics="reference": trilinear scatter with
+    the reference BORDER mask, divided by the per-voxel corner-hit count summed
+    over the cameras of each batch entry (resample_grid.cu.cc:1160-1387).
+    semantics="exact": true adjoint of ``sample_grid_transform``."""
+    from . import WRAP
+    inp = np.asarray(inp, dtype=np.float64)
+    out_grad = np.asarray(out_grad, dtype=np.float64)
+    N, D, H, W, C = inp.shape
+    nV = len(V)
+    out_dhw = out_grad.shape[2:5]
+    in_grad = np.zeros_like(inp)
+    for n in range(N):
+        count = np.zeros((D, H, W), dtype=np.int64)
+        for nn, v, vo in _transform_pairs(N, nV, separate_camera_batch):
+            if nn != n:
+                continue
+            cx = G.CameraXform.from_op_args(M[n], V[v], P[v], F[v])
+            q = transform_positions(cx, out_dhw, inp.shape[1:4], coordinate_mode) - 0.5
+            g = out_grad[n, vo]
+            if semantics == "reference":
+                S.scatter_linear(inp.shape[1:], q, g, boundary, mask="reference", count=count, out=in_grad[n])
+            else:
+                if boundary == WRAP:
+                    continue
+                if filter_mode == LINEAR:
+                    S.scatter_linear(inp.shape[1:], q, g, boundary, mask="forward", out=in_grad[n])
+                else:
+                    S.scatter_nearest(inp.shape[1:], q, g, boundary, out=in_grad[n])
+        if semantics == "reference":
+            nz = count > 0
+            in_grad[n][nz] = in_grad[n][nz] / count[nz][:, None]
+    return in_grad
