@@ -1,0 +1,198 @@
+/*
+ * ngt_transport.h - C ABI of the MI355X (gfx950) differentiable
+ * volume-transport library (libngt_transport.so).
+ *
+ * This is the drop-in boundary for the four TF custom-op libraries of
+ * Neural-Global-Transport (phitest/render/cuda/src/{resample_grid,
+ * raymarch_grid,reduce_blend,advect}.cc).  Each entry point replaces one
+ * OpKernel::Compute and takes the op's inputs as plain device pointers plus
+ * NDHWC shapes, the op's string attributes as the integer enums below, and
+ * the 4x4 matrices / frustum parameters as HOST pointers (the reference
+ * registers them as host-memory inputs, resample_grid.cc:1183-1219).
+ *
+ * Conventions
+ *  - Shapes are int64[5] = {N, D, H, W, C}; data is fp32, x fastest.
+ *  - Matrices are the flat 16-float arrays the reference ops receive (already
+ *    transposed by the Renderer), read column-major; `matrix_m` holds N
+ *    matrices, `matrix_v`/`matrix_p` hold V, `frustum` holds V x 6 floats
+ *    (near, far, left, right, top, bottom).
+ *  - `separate_camera_batch` = the reference attribute of that name: 1 ->
+ *    output [N, V, ...], every batch entry seen by every camera; 0 -> V == N,
+ *    output [N, 1, ...], entry n seen by camera n.
+ *  - `gradient_semantics`: NGT_GRAD_EXACT (true adjoint) or
+ *    NGT_GRAD_REFERENCE (the reference's count / weight normalised gradients).
+ *  - Every call is stream-ordered on `stream` (a hipStream_t, 0 = default
+ *    stream), never synchronises the device, never allocates device memory
+ *    (scratch comes in through `workspace`, sized by the *_workspace_size
+ *    query), keeps no global mutable state and is hipGraph-capturable.
+ *  - Return value: NGT_OK or an NGT_ERR_* code; ngt_last_error() returns the
+ *    calling thread's message for the last failure (the reference's
+ *    InvalidArgument / Unimplemented texts where one exists).
+ */
+#ifndef NGT_TRANSPORT_H
+#define NGT_TRANSPORT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NGT_OK 0
+#define NGT_ERR_INVALID_ARGUMENT 1
+#define NGT_ERR_UNIMPLEMENTED 2
+#define NGT_ERR_WORKSPACE 3
+#define NGT_ERR_HIP 4
+
+/* interpolation */
+#define NGT_FILTER_NEAREST 0
+#define NGT_FILTER_LINEAR 1
+#define NGT_FILTER_MIN 2
+#define NGT_FILTER_MAX 3
+/* boundary */
+#define NGT_BOUNDARY_BORDER 0
+#define NGT_BOUNDARY_CLAMP 1
+#define NGT_BOUNDARY_WRAP 2
+/* mipmapping */
+#define NGT_MIP_NONE 0
+#define NGT_MIP_NEAREST 1
+#define NGT_MIP_LINEAR 2
+/* coordinate_mode (sampling_settings.hpp:61-70 numbering) */
+#define NGT_COORD_TRANSFORM_LINDEPTH 0
+#define NGT_COORD_TRANSFORM_LINDEPTH_REVERSE 1
+#define NGT_COORD_TRANSFORM 3
+#define NGT_COORD_TRANSFORM_REVERSE 4
+/* blending_mode */
+#define NGT_BLEND_BEER_LAMBERT 0
+#define NGT_BLEND_ALPHA 2
+#define NGT_BLEND_ALPHA_ADDITIVE 3
+#define NGT_BLEND_ADDITIVE 4
+/* gradient semantics */
+#define NGT_GRAD_EXACT 0
+#define NGT_GRAD_REFERENCE 1
+/* advection clamp */
+#define NGT_CLAMP_NONE 0
+#define NGT_CLAMP_MC_SMOOTH 1
+#define NGT_CLAMP_NEGATIVE 2
+
+typedef void* ngt_stream_t; /* hipStream_t */
+
+const char* ngt_last_error(void);
+int ngt_version(void);
+
+/* ---- SampleGridLut (resample_grid.cc:94-120, 742-899) ----------------------
+ * input [N,D,H,W,C] (C in 1,2,4), lookup [V,D',H',W',4] -> output
+ * [N,V|1,D',H',W',C].  Position = lookup.xyz (x output dims if
+ * normalized_coords) (+ output index if relative_coords) - cell_center_offset.
+ * Only mipmapping NONE is implemented (NGT_ERR_UNIMPLEMENTED otherwise). */
+int ngt_sample_grid_lut_fwd(const float* input, const int64_t* input_shape, const float* lookup,
+                            const int64_t* lookup_shape, int interpolation, int boundary, int mipmapping,
+                            int num_mipmaps, float mip_bias, float cell_center_offset, int separate_camera_batch,
+                            int relative_coords, int normalized_coords, float* output, ngt_stream_t stream);
+
+/* ---- SampleGridLutGrad (resample_grid.cc:124-150, 925-1081) ----------------
+ * output_grad [N,V|1,D',H',W',C] -> input_grad [N,D,H,W,C],
+ * lookup_grad [V,D',H',W',4] (both fully overwritten). */
+size_t ngt_sample_grid_lut_bwd_workspace_size(const int64_t* input_shape, const int64_t* lookup_shape,
+                                              int gradient_semantics);
+int ngt_sample_grid_lut_bwd(const float* input, const int64_t* input_shape, const float* output_grad,
+                            const float* lookup, const int64_t* lookup_shape, int interpolation, int boundary,
+                            int mipmapping, int num_mipmaps, float mip_bias, float cell_center_offset,
+                            int separate_camera_batch, int relative_coords, int normalized_coords,
+                            int gradient_semantics, float* input_grad, float* lookup_grad, void* workspace,
+                            size_t workspace_bytes, ngt_stream_t stream);
+
+/* ---- SampleGridTransform (resample_grid.cc:23-65, 316-516) -----------------
+ * input [N,D,H,W,C] -> output [N,V|1,out_dhw,C].  interpolation NEAREST or
+ * LINEAR, boundary BORDER or CLAMP, cell_center_offset must be 0.5. */
+int ngt_sample_grid_transform_fwd(const float* input, const int64_t* input_shape, const float* matrix_m,
+                                  const float* matrix_v, const float* matrix_p, const float* frustum,
+                                  int num_cameras, const int32_t* output_dhw, int interpolation, int boundary,
+                                  int mipmapping, int num_mipmaps, float mip_bias, int coordinate_mode,
+                                  float cell_center_offset, int separate_camera_batch, float* output,
+                                  ngt_stream_t stream);
+
+/* ---- SampleGridTransformGrad (resample_grid.cc:68-92, 538-717) ------------ */
+size_t ngt_sample_grid_transform_bwd_workspace_size(const int64_t* input_shape, int gradient_semantics);
+int ngt_sample_grid_transform_bwd(const float* input, const int64_t* input_shape, const float* output_grad,
+                                  const float* matrix_m, const float* matrix_v, const float* matrix_p,
+                                  const float* frustum, int num_cameras, const int32_t* output_dhw,
+                                  int interpolation, int boundary, int coordinate_mode, float cell_center_offset,
+                                  int separate_camera_batch, int gradient_semantics, float* input_grad,
+                                  void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+
+/* ---- ReduceGridBlend (+Grad) (reduce_blend.cc:24-55) -----------------------
+ * input [N,D,H,W,C] -> output [N,H,W,C] or, keep_dims, [N,D,H,W,C]. */
+int ngt_reduce_grid_blend_fwd(const float* input, const int64_t* input_shape, int blending_mode, int keep_dims,
+                              float* output, ngt_stream_t stream);
+int ngt_reduce_grid_blend_bwd(const float* output_grad, const float* output, const float* input,
+                              const int64_t* input_shape, int blending_mode, int keep_dims, int gradient_semantics,
+                              float* input_grad, ngt_stream_t stream);
+
+/* ---- RaymarchGridTransform (+Grad) (raymarch_grid.cc:29-92) ----------------
+ * input [N,D,H,W,C] -> output [N,V|1,H',W',C] for output_dhw = (D',H',W')
+ * (D' = samples per ray before BORDER clipping). */
+int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_shape, const float* matrix_m,
+                                    const float* matrix_v, const float* matrix_p, const float* frustum,
+                                    int num_cameras, const int32_t* output_dhw, int interpolation, int boundary,
+                                    int blending_mode, int separate_camera_batch, float* output,
+                                    ngt_stream_t stream);
+size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape, int num_cameras,
+                                                      int gradient_semantics);
+int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_shape, const float* output,
+                                    const float* output_grad, const float* matrix_m, const float* matrix_v,
+                                    const float* matrix_p, const float* frustum, int num_cameras,
+                                    const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
+                                    int separate_camera_batch, int gradient_semantics, float* input_grad,
+                                    void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+
+/* ---- fused density render (Renderer.render_density, renderer.py:718-827) --
+ * The fused raymarch of [light, density] * depth_step with constant lights
+ * (Light.grid_lighting, lighting.py:26-34): the 4-channel cell
+ * (rho*color*ds, rho*ds) is formed inside the kernel from the 1-channel
+ * density, so neither the light grid nor the concatenation is materialised.
+ * density [N,D,H,W,1], light_rgb = sum of color*intensity over the lights,
+ * depth_step = (far-near)/D' per camera (all cameras equal); output
+ * [N,V,H',W',4] = (RGB, optical depth), exactly the raymarch of the
+ * reference's concatenated grid.  BEER_LAMBERT / ADDITIVE / ALPHA /
+ * ALPHA_ADDITIVE as the raymarch op. */
+int ngt_render_density_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
+                           float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
+                           const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
+                           int boundary, int blending_mode, float* output, ngt_stream_t stream);
+int ngt_render_density_bwd(const float* density, const int64_t* density_shape, const float* light_rgb,
+                           float depth_step, const float* output, const float* output_grad, const float* matrix_m,
+                           const float* matrix_v, const float* matrix_p, const float* frustum, int num_cameras,
+                           const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
+                           float* density_grad, ngt_stream_t stream);
+
+/* ---- AdvectGridSemiLangrange (advect.cc:21-71) with VelocityGrid.warp
+ * semantics (data_structures.py:1690-1737) --------------------------------
+ * input [N,D,H,W,C]; velocity either centered (vel_y = vel_z = NULL, vel_x =
+ * [V,D,H,W,3]) or MAC-staggered (vel_x [V,D,H,W+1,1], vel_y [V,D,H+1,W,1],
+ * vel_z [V,D+1,H,W,1], averaged to cell centres inside the kernels).
+ * Sample position = index + timestep * v (callers pass timestep = -dt),
+ * order 1 (semi-Lagrangian) or 2 (MacCormack), clamp NGT_CLAMP_*.
+ * output [N,V|1,D,H,W,C].  `sl_out` (nullable, same shape as output) receives
+ * the semi-Lagrangian field of an order-2 step for reuse by the backward. */
+size_t ngt_advect_workspace_size(const int64_t* input_shape, int num_velocities, int separate_velocity_batch,
+                                 int order, int backward);
+int ngt_advect_fwd(const float* input, const int64_t* input_shape, const float* vel_x, const float* vel_y,
+                   const float* vel_z, int num_velocities, float timestep, int order, int clamp, int boundary,
+                   int separate_velocity_batch, float* output, float* sl_out, void* workspace,
+                   size_t workspace_bytes, ngt_stream_t stream);
+/* Gradients: input_grad [N,D,H,W,C]; velocity grads in the velocity layout
+ * (vel_x_grad [V,D,H,W,3] centered, or the three face arrays).  All outputs
+ * are fully overwritten.  `sl` (nullable) = the sl_out of the forward. */
+int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* vel_x, const float* vel_y,
+                   const float* vel_z, int num_velocities, float timestep, int order, int clamp, int boundary,
+                   int separate_velocity_batch, const float* output_grad, const float* sl, int gradient_semantics,
+                   float* input_grad, float* vel_x_grad, float* vel_y_grad, float* vel_z_grad, void* workspace,
+                   size_t workspace_bytes, ngt_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGT_TRANSPORT_H */

@@ -1,0 +1,501 @@
+// Semi-Lagrangian / MacCormack advection (VelocityGrid.warp semantics,
+// data_structures.py:1690-1737) and its exact adjoint on gfx950.
+//
+// The reference composes a warp from 2-4 SampleGridLut launches (each with a
+// device sync), a 4-channel LUT materialisation and ~6 TF elementwise ops.
+// Here a MacCormack step is two kernels:
+//   A  sl(x)  = S(rho, x + ts*v(x))                        (gather, 8 taps)
+//   B  back   = S(sl, x - ts*v(x)); corr = sl + (rho - back)/2
+//      MC_SMOOTH: min/max of rho's 8 corners at x + ts*v revert corr to sl
+//      NEGATIVE : max(., 0)
+// Velocity is read centered or MAC-staggered; the face average is folded
+// into the loads (VelocityGrid.centered, data_structures.py:1634-1653), so
+// no centered copy is materialised.
+//
+// Exact backward (B1, B2, optional face-gather B3):
+//   B1  g_sl(x) += g;  if not reverted: g_rho(x) += g/2, scatter -g/2 into
+//       g_sl at x - ts*v, g_v(x) -= ts * <grad S(sl), -g/2>
+//   B2  scatter g_sl into g_rho at x + ts*v, g_v(x) += ts * <grad S(rho), g_sl>
+//   B3  staggered: g_face = (g_v(left cell) + g_v(right cell)) / 2
+// Scatters use hardware fp32 atomics.
+#include "ngt_host.hpp"
+
+namespace ngt {
+namespace {
+
+constexpr int kBX = 64, kBY = 4;
+
+struct AdvArgs {
+  Dims d;
+  int n_data, n_vel, separate;
+  float ts;
+  const float* vx;
+  const float* vy;
+  const float* vz;
+  int staggered;
+};
+
+NGT_DEVICE void pair_indices(const AdvArgs& a, int pair, int& n, int& v) {
+  if (a.separate) { n = pair / a.n_vel; v = pair % a.n_vel; }
+  else { n = pair; v = pair; }
+}
+
+NGT_DEVICE void load_velocity(const AdvArgs& a, int v, int x, int y, int z, float& ux, float& uy, float& uz) {
+  const Dims& d = a.d;
+  if (!a.staggered) {
+    const float* p = a.vx + (static_cast<int64_t>(v) * d.x * d.y * d.z + cell_offset(d, x, y, z)) * 3;
+    ux = p[0]; uy = p[1]; uz = p[2];
+  } else {
+    const int64_t sx = static_cast<int64_t>(v) * d.z * d.y * (d.x + 1);
+    const int64_t rowx = sx + (static_cast<int64_t>(z) * d.y + y) * (d.x + 1) + x;
+    ux = (a.vx[rowx + 1] + a.vx[rowx]) * 0.5f;
+    const int64_t sy = static_cast<int64_t>(v) * d.z * (d.y + 1) * d.x;
+    const int64_t rowy = sy + (static_cast<int64_t>(z) * (d.y + 1) + y) * d.x + x;
+    uy = (a.vy[rowy + d.x] + a.vy[rowy]) * 0.5f;
+    const int64_t sz = static_cast<int64_t>(v) * (d.z + 1) * d.y * d.x;
+    const int64_t rowz = sz + (static_cast<int64_t>(z) * d.y + y) * d.x + x;
+    uz = (a.vz[rowz + static_cast<int64_t>(d.y) * d.x] + a.vz[rowz]) * 0.5f;
+  }
+}
+
+struct ThreadCell {
+  int x, y, z, pair;
+  bool ok;
+};
+
+NGT_DEVICE ThreadCell thread_cell(const AdvArgs& a) {
+  ThreadCell t;
+  t.x = blockIdx.x * kBX + threadIdx.x;
+  t.y = blockIdx.y * kBY + threadIdx.y;
+  t.z = blockIdx.z % a.d.z;
+  t.pair = blockIdx.z / a.d.z;
+  t.ok = t.x < a.d.x && t.y < a.d.y;
+  return t;
+}
+
+// A: semi-Lagrangian step (also the whole order-1 warp).
+template <int C, int BM, bool NEG>
+__global__ __launch_bounds__(256) void advect_sl_kernel(AdvArgs a, const float* __restrict__ rho,
+                                                        float* __restrict__ out) {
+  const ThreadCell t = thread_cell(a);
+  if (!t.ok) return;
+  int n, v;
+  pair_indices(a, t.pair, n, v);
+  float ux, uy, uz;
+  load_velocity(a, v, t.x, t.y, t.z, ux, uy, uz);
+  const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
+  const float px = static_cast<float>(t.x) + a.ts * ux;
+  const float py = static_cast<float>(t.y) + a.ts * uy;
+  const float pz = static_cast<float>(t.z) + a.ts * uz;
+  Vec<C> r = read_linear<C, BM>(rho + n * cells * C, a.d, px, py, pz);
+  if constexpr (NEG) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) r.v[i] = fmaxf(r.v[i], 0.f);
+  }
+  vstore<C>(out + (t.pair * cells + cell_offset(a.d, t.x, t.y, t.z)) * C, r);
+}
+
+// B: MacCormack correction + clamp.
+template <int C, int BM, int CLAMPM>
+__global__ __launch_bounds__(256) void advect_mc_kernel(AdvArgs a, const float* __restrict__ rho,
+                                                        const float* __restrict__ sl, float* __restrict__ out) {
+  const ThreadCell t = thread_cell(a);
+  if (!t.ok) return;
+  int n, v;
+  pair_indices(a, t.pair, n, v);
+  float ux, uy, uz;
+  load_velocity(a, v, t.x, t.y, t.z, ux, uy, uz);
+  const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
+  const int64_t cell = cell_offset(a.d, t.x, t.y, t.z);
+  const float* src = rho + n * cells * C;
+  const float* slp = sl + t.pair * cells * C;
+  const float fx = static_cast<float>(t.x), fy = static_cast<float>(t.y), fz = static_cast<float>(t.z);
+  const float tx = a.ts * ux, ty = a.ts * uy, tz = a.ts * uz;
+  const Vec<C> fwd = vload<C>(slp + cell * C);
+  const Vec<C> back = read_linear<C, BM>(slp, a.d, fx + (-tx), fy + (-ty), fz + (-tz));
+  const Vec<C> cur = vload<C>(src + cell * C);
+  Vec<C> r;
+#pragma unroll
+  for (int i = 0; i < C; ++i) r.v[i] = fwd.v[i] + 0.5f * (cur.v[i] - back.v[i]);
+  if constexpr (CLAMPM == NGT_CLAMP_MC_SMOOTH) {
+    const Vec<C> mn = read_minmax<C, BM, false>(src, a.d, fx + tx, fy + ty, fz + tz);
+    const Vec<C> mx = read_minmax<C, BM, true>(src, a.d, fx + tx, fy + ty, fz + tz);
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+      if (r.v[i] < mn.v[i] || r.v[i] > mx.v[i]) r.v[i] = fwd.v[i];
+  }
+  if constexpr (CLAMPM == NGT_CLAMP_NEGATIVE) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) r.v[i] = fmaxf(r.v[i], 0.f);
+  }
+  vstore<C>(out + (t.pair * cells + cell) * C, r);
+}
+
+// B1 (order 2): route g to the SL field and the direct rho term; scatter the
+// back-step adjoint into g_sl.  REF: scatter into tmp with hit counts.
+template <int C, int BM, int CLAMPM, bool REF>
+__global__ __launch_bounds__(256) void advect_mc_bwd_kernel(AdvArgs a, const float* __restrict__ rho,
+                                                            const float* __restrict__ sl,
+                                                            const float* __restrict__ g_out,
+                                                            float* __restrict__ g_sl, float* __restrict__ g_rho,
+                                                            float* __restrict__ g_vc, float* __restrict__ tmp,
+                                                            uint32_t* __restrict__ hits) {
+  const ThreadCell t = thread_cell(a);
+  if (!t.ok) return;
+  int n, v;
+  pair_indices(a, t.pair, n, v);
+  float ux, uy, uz;
+  load_velocity(a, v, t.x, t.y, t.z, ux, uy, uz);
+  const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
+  const int64_t cell = cell_offset(a.d, t.x, t.y, t.z);
+  const float* src = rho + n * cells * C;
+  const float* slp = sl + t.pair * cells * C;
+  const float fx = static_cast<float>(t.x), fy = static_cast<float>(t.y), fz = static_cast<float>(t.z);
+  const float tx = a.ts * ux, ty = a.ts * uy, tz = a.ts * uz;
+  const float bx = fx + (-tx), by = fy + (-ty), bz = fz + (-tz);
+  Vec<C> g = vload<C>(g_out + (t.pair * cells + cell) * C);
+  const Vec<C> fwd = vload<C>(slp + cell * C);
+  const Vec<C> back = read_linear<C, BM>(slp, a.d, bx, by, bz);
+  const Vec<C> cur = vload<C>(src + cell * C);
+  Vec<C> corr;
+#pragma unroll
+  for (int i = 0; i < C; ++i) corr.v[i] = fwd.v[i] + 0.5f * (cur.v[i] - back.v[i]);
+  Vec<C> keep = vfill<C>(1.f);  // 1 where the corrected value is kept
+  if constexpr (CLAMPM == NGT_CLAMP_MC_SMOOTH) {
+    const Vec<C> mn = read_minmax<C, BM, false>(src, a.d, fx + tx, fy + ty, fz + tz);
+    const Vec<C> mx = read_minmax<C, BM, true>(src, a.d, fx + tx, fy + ty, fz + tz);
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+      if (corr.v[i] < mn.v[i] || corr.v[i] > mx.v[i]) keep.v[i] = 0.f;
+  }
+  if constexpr (CLAMPM == NGT_CLAMP_NEGATIVE) {
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+      if (!(corr.v[i] >= 0.f)) g.v[i] = 0.f;  // tf.maximum passes the gradient where x >= 0
+  }
+  float* gsl = g_sl + (t.pair * cells) * C;
+  Vec<C> gback;
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    atomic_add(gsl + cell * C + i, g.v[i]);
+    atomic_add(g_rho + (n * cells + cell) * C + i, 0.5f * keep.v[i] * g.v[i]);
+    gback.v[i] = -0.5f * keep.v[i] * g.v[i];
+  }
+  if constexpr (REF) {
+    scatter_linear<C, BM, true>(tmp + (t.pair * cells) * C, a.d, bx, by, bz, gback, hits + t.pair * cells);
+  } else {
+    scatter_linear<C, BM, false>(gsl, a.d, bx, by, bz, gback);
+  }
+  Vec<C> dx, dy, dz;
+  read_gradient<C, BM>(slp, a.d, bx, by, bz, dx, dy, dz);
+  float* gv = g_vc + (static_cast<int64_t>(v) * cells + cell) * 3;
+  atomic_add(gv + 0, -a.ts * vdot(gback, dx));
+  atomic_add(gv + 1, -a.ts * vdot(gback, dy));
+  atomic_add(gv + 2, -a.ts * vdot(gback, dz));
+}
+
+// B2: adjoint of the semi-Lagrangian read.  For order 1, g_sl is g_out with
+// the NEGATIVE mask applied on the fly (NEG).
+template <int C, int BM, bool NEG, bool REF>
+__global__ __launch_bounds__(256) void advect_sl_bwd_kernel(AdvArgs a, const float* __restrict__ rho,
+                                                            const float* __restrict__ g_sl,
+                                                            const float* __restrict__ sl_out,
+                                                            float* __restrict__ g_rho, float* __restrict__ g_vc,
+                                                            float* __restrict__ tmp, uint32_t* __restrict__ hits) {
+  const ThreadCell t = thread_cell(a);
+  if (!t.ok) return;
+  int n, v;
+  pair_indices(a, t.pair, n, v);
+  float ux, uy, uz;
+  load_velocity(a, v, t.x, t.y, t.z, ux, uy, uz);
+  const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
+  const int64_t cell = cell_offset(a.d, t.x, t.y, t.z);
+  const float* src = rho + n * cells * C;
+  const float px = static_cast<float>(t.x) + a.ts * ux;
+  const float py = static_cast<float>(t.y) + a.ts * uy;
+  const float pz = static_cast<float>(t.z) + a.ts * uz;
+  Vec<C> g = vload<C>(g_sl + (t.pair * cells + cell) * C);
+  if constexpr (NEG) {
+    const Vec<C> r = read_linear<C, BM>(src, a.d, px, py, pz);
+#pragma unroll
+    for (int i = 0; i < C; ++i)
+      if (!(r.v[i] >= 0.f)) g.v[i] = 0.f;
+  }
+  (void)sl_out;
+  if constexpr (REF) {
+    scatter_linear<C, BM, true>(tmp + (t.pair * cells) * C, a.d, px, py, pz, g, hits + t.pair * cells);
+  } else {
+    scatter_linear<C, BM, false>(g_rho + n * cells * C, a.d, px, py, pz, g);
+  }
+  Vec<C> dx, dy, dz;
+  read_gradient<C, BM>(src, a.d, px, py, pz, dx, dy, dz);
+  float* gv = g_vc + (static_cast<int64_t>(v) * cells + cell) * 3;
+  atomic_add(gv + 0, a.ts * vdot(g, dx));
+  atomic_add(gv + 1, a.ts * vdot(g, dy));
+  atomic_add(gv + 2, a.ts * vdot(g, dz));
+}
+
+// dst(pair or n) += tmp / hits (reference count normalisation), one cell per
+// work-item; dst_index maps the pair slot to the destination slot.
+template <int C>
+__global__ __launch_bounds__(256) void add_normalized_kernel(AdvArgs a, const float* __restrict__ tmp,
+                                                             const uint32_t* __restrict__ hits,
+                                                             float* __restrict__ dst, int to_data) {
+  const ThreadCell t = thread_cell(a);
+  if (!t.ok) return;
+  int n, v;
+  pair_indices(a, t.pair, n, v);
+  const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
+  const int64_t cell = cell_offset(a.d, t.x, t.y, t.z);
+  const uint32_t h = hits[t.pair * cells + cell];
+  if (h == 0u) return;
+  const float w = 1.0f / static_cast<float>(h);
+  const int slot = to_data ? n : t.pair;
+#pragma unroll
+  for (int i = 0; i < C; ++i)
+    atomic_add(dst + (slot * cells + cell) * C + i, tmp[(t.pair * cells + cell) * C + i] * w);
+}
+
+// B3: centered-velocity gradient -> MAC face gradients (adjoint of the face
+// average).  One work-item per face of each component.
+__global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, int n_vel, const float* __restrict__ g_vc,
+                                                                      float* __restrict__ gx, float* __restrict__ gy,
+                                                                      float* __restrict__ gz) {
+  const int64_t nx = static_cast<int64_t>(d.z) * d.y * (d.x + 1);
+  const int64_t ny = static_cast<int64_t>(d.z) * (d.y + 1) * d.x;
+  const int64_t nz = static_cast<int64_t>(d.z + 1) * d.y * d.x;
+  const int64_t per_v = nx + ny + nz;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= per_v * n_vel) return;
+  const int v = static_cast<int>(i / per_v);
+  int64_t j = i % per_v;
+  const int64_t cells = static_cast<int64_t>(d.x) * d.y * d.z;
+  const float* g = g_vc + static_cast<int64_t>(v) * cells * 3;
+  if (j < nx) {
+    const int x = static_cast<int>(j % (d.x + 1));
+    const int64_t r = j / (d.x + 1);
+    const int y = static_cast<int>(r % d.y), z = static_cast<int>(r / d.y);
+    float s = 0.f;
+    if (x > 0) s += g[cell_offset(d, x - 1, y, z) * 3 + 0];
+    if (x < d.x) s += g[cell_offset(d, x, y, z) * 3 + 0];
+    gx[static_cast<int64_t>(v) * nx + j] = 0.5f * s;
+    return;
+  }
+  j -= nx;
+  if (j < ny) {
+    const int x = static_cast<int>(j % d.x);
+    const int64_t r = j / d.x;
+    const int y = static_cast<int>(r % (d.y + 1)), z = static_cast<int>(r / (d.y + 1));
+    float s = 0.f;
+    if (y > 0) s += g[cell_offset(d, x, y - 1, z) * 3 + 1];
+    if (y < d.y) s += g[cell_offset(d, x, y, z) * 3 + 1];
+    gy[static_cast<int64_t>(v) * ny + j] = 0.5f * s;
+    return;
+  }
+  j -= ny;
+  {
+    const int x = static_cast<int>(j % d.x);
+    const int64_t r = j / d.x;
+    const int y = static_cast<int>(r % d.y), z = static_cast<int>(r / d.y);
+    float s = 0.f;
+    if (z > 0) s += g[cell_offset(d, x, y, z - 1) * 3 + 2];
+    if (z < d.z) s += g[cell_offset(d, x, y, z) * 3 + 2];
+    gz[static_cast<int64_t>(v) * nz + j] = 0.5f * s;
+  }
+}
+
+#define NGT_SWITCH_C(C_, ...)                              \
+  switch (C_) {                                            \
+    case 1: { constexpr int CC = 1; __VA_ARGS__; } break;  \
+    case 2: { constexpr int CC = 2; __VA_ARGS__; } break;  \
+    case 3: { constexpr int CC = 3; __VA_ARGS__; } break;  \
+    default: { constexpr int CC = 4; __VA_ARGS__; } break; \
+  }
+#define NGT_SWITCH_BM(B_, ...)                                      \
+  switch (B_) {                                                     \
+    case BORDER: { constexpr int BB = BORDER; __VA_ARGS__; } break; \
+    case CLAMP: { constexpr int BB = CLAMP; __VA_ARGS__; } break;   \
+    default: { constexpr int BB = WRAP; __VA_ARGS__; } break;       \
+  }
+#define NGT_SWITCH_CLAMP(K_, ...)                                                          \
+  switch (K_) {                                                                            \
+    case NGT_CLAMP_MC_SMOOTH: { constexpr int KK = NGT_CLAMP_MC_SMOOTH; __VA_ARGS__; } break; \
+    case NGT_CLAMP_NEGATIVE: { constexpr int KK = NGT_CLAMP_NEGATIVE; __VA_ARGS__; } break;   \
+    default: { constexpr int KK = NGT_CLAMP_NONE; __VA_ARGS__; } break;                      \
+  }
+
+struct AdvPlan {
+  AdvArgs a;
+  int pairs;
+  int64_t cells;
+  int C;
+};
+
+int plan_advect(const int64_t* input_shape, const float* vx, const float* vy, const float* vz, int num_velocities,
+                float timestep, int order, int clamp, int boundary, int separate, AdvPlan& p) {
+  int rc = check_grid_shape(input_shape, "input", true);
+  if (rc) return rc;
+  NGT_REQUIRE(vx != nullptr, NGT_ERR_INVALID_ARGUMENT, "velocity must be given");
+  NGT_REQUIRE((vy == nullptr) == (vz == nullptr), NGT_ERR_INVALID_ARGUMENT,
+              "staggered velocity needs all three components");
+  NGT_REQUIRE(num_velocities > 0, NGT_ERR_INVALID_ARGUMENT, "velocity batch must not be empty");
+  NGT_REQUIRE(order == 1 || order == 2, NGT_ERR_INVALID_ARGUMENT, "Unsupported warp order");
+  NGT_REQUIRE(clamp >= 0 && clamp <= 2, NGT_ERR_INVALID_ARGUMENT, "invalid clamp mode");
+  NGT_REQUIRE(boundary >= 0 && boundary <= 2, NGT_ERR_INVALID_ARGUMENT, "invalid boundary");
+  if (!separate)
+    NGT_REQUIRE(num_velocities == input_shape[0], NGT_ERR_INVALID_ARGUMENT,
+                "velocity batch must match data batch when not using separate velocity batches.");
+  const Shape5 s5 = shape5(input_shape);
+  p.a.d = s5.dims();
+  p.a.n_data = static_cast<int>(s5.n);
+  p.a.n_vel = num_velocities;
+  p.a.separate = separate ? 1 : 0;
+  p.a.ts = timestep;
+  p.a.vx = vx; p.a.vy = vy; p.a.vz = vz;
+  p.a.staggered = vy != nullptr ? 1 : 0;
+  p.pairs = separate ? p.a.n_data * num_velocities : p.a.n_data;
+  p.cells = s5.cells();
+  p.C = static_cast<int>(s5.c);
+  (void)timestep;
+  return NGT_OK;
+}
+
+dim3 adv_grid(const AdvPlan& p) {
+  return dim3(blocks_for(p.a.d.x, kBX), blocks_for(p.a.d.y, kBY), static_cast<unsigned>(p.a.d.z * p.pairs));
+}
+
+}  // namespace
+}  // namespace ngt
+
+using namespace ngt;
+
+extern "C" {
+
+size_t ngt_advect_workspace_size(const int64_t* input_shape, int num_velocities, int separate_velocity_batch,
+                                 int order, int backward) {
+  if (input_shape == nullptr || num_velocities <= 0) return 0;
+  const int64_t cells = input_shape[1] * input_shape[2] * input_shape[3];
+  const int64_t C = input_shape[4];
+  const int64_t pairs = separate_velocity_batch ? input_shape[0] * num_velocities : input_shape[0];
+  int64_t floats = 0;
+  if (order == 2) floats += pairs * cells * C;          // SL field (when the caller keeps none)
+  if (backward) {
+    floats += pairs * cells * C;                         // g_sl
+    floats += static_cast<int64_t>(num_velocities) * cells * 3;  // centered velocity gradient
+    floats += pairs * cells * C;                         // reference: scatter staging
+    floats += pairs * cells;                             // reference: hit counts (uint32)
+  }
+  return static_cast<size_t>(floats) * sizeof(float) + 256;
+}
+
+int ngt_advect_fwd(const float* input, const int64_t* input_shape, const float* vel_x, const float* vel_y,
+                   const float* vel_z, int num_velocities, float timestep, int order, int clamp, int boundary,
+                   int separate_velocity_batch, float* output, float* sl_out, void* workspace, size_t workspace_bytes,
+                   ngt_stream_t stream) {
+  AdvPlan p;
+  int rc = plan_advect(input_shape, vel_x, vel_y, vel_z, num_velocities, timestep, order, clamp, boundary,
+                       separate_velocity_batch, p);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  const dim3 grid = adv_grid(p), block(kBX, kBY);
+  if (order == 1) {
+    const bool neg = clamp == NGT_CLAMP_NEGATIVE;
+    NGT_SWITCH_C(p.C, NGT_SWITCH_BM(boundary,
+        if (neg) hipLaunchKernelGGL((advect_sl_kernel<CC, BB, true>), grid, block, 0, s, p.a, input, output);
+        else hipLaunchKernelGGL((advect_sl_kernel<CC, BB, false>), grid, block, 0, s, p.a, input, output)));
+    return check_launch("ngt_advect_fwd");
+  }
+  float* sl = sl_out;
+  if (sl == nullptr) {
+    const size_t need = static_cast<size_t>(p.pairs * p.cells * p.C) * sizeof(float);
+    NGT_REQUIRE(workspace && workspace_bytes >= need, NGT_ERR_WORKSPACE, "workspace too small for ngt_advect_fwd");
+    sl = static_cast<float*>(workspace);
+  }
+  NGT_SWITCH_C(p.C, NGT_SWITCH_BM(boundary,
+      hipLaunchKernelGGL((advect_sl_kernel<CC, BB, false>), grid, block, 0, s, p.a, input, sl);
+      NGT_SWITCH_CLAMP(clamp,
+          hipLaunchKernelGGL((advect_mc_kernel<CC, BB, KK>), grid, block, 0, s, p.a, input, sl, output))));
+  return check_launch("ngt_advect_fwd");
+}
+
+int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* vel_x, const float* vel_y,
+                   const float* vel_z, int num_velocities, float timestep, int order, int clamp, int boundary,
+                   int separate_velocity_batch, const float* output_grad, const float* sl, int gradient_semantics,
+                   float* input_grad, float* vel_x_grad, float* vel_y_grad, float* vel_z_grad, void* workspace,
+                   size_t workspace_bytes, ngt_stream_t stream) {
+  AdvPlan p;
+  int rc = plan_advect(input_shape, vel_x, vel_y, vel_z, num_velocities, timestep, order, clamp, boundary,
+                       separate_velocity_batch, p);
+  if (rc) return rc;
+  NGT_REQUIRE(vel_x_grad != nullptr && (!p.a.staggered || (vel_y_grad && vel_z_grad)), NGT_ERR_INVALID_ARGUMENT,
+              "velocity gradient outputs must be given");
+  const size_t need = ngt_advect_workspace_size(input_shape, num_velocities, separate_velocity_batch, order, 1);
+  NGT_REQUIRE(workspace && workspace_bytes >= need, NGT_ERR_WORKSPACE, "workspace too small for ngt_advect_bwd");
+  const bool ref = gradient_semantics == NGT_GRAD_REFERENCE;
+  hipStream_t s = as_stream(stream);
+  const dim3 grid = adv_grid(p), block(kBX, kBY);
+  const int64_t field = p.pairs * p.cells * p.C;
+  float* ws = static_cast<float*>(workspace);
+  float* sl_buf = ws;
+  float* g_sl = sl_buf + (order == 2 ? field : 0);
+  float* g_vc = g_sl + field;
+  float* tmp = g_vc + static_cast<int64_t>(p.a.n_vel) * p.cells * 3;
+  uint32_t* hits = reinterpret_cast<uint32_t*>(tmp + field);
+  if (!p.a.staggered) g_vc = vel_x_grad;  // centered: accumulate in place
+
+  fill_zero(input_grad, p.a.n_data * p.cells * p.C, s);
+  fill_zero(g_vc, static_cast<int64_t>(p.a.n_vel) * p.cells * 3, s);
+  const float* slp = sl;
+  if (order == 2 && slp == nullptr) {
+    NGT_SWITCH_C(p.C, NGT_SWITCH_BM(boundary,
+        hipLaunchKernelGGL((advect_sl_kernel<CC, BB, false>), grid, block, 0, s, p.a, input, sl_buf)));
+    slp = sl_buf;
+  }
+  const float* gsl_in = output_grad;
+  if (order == 2) {
+    fill_zero(g_sl, field, s);
+    if (ref) {
+      fill_zero(tmp, field, s);
+      fill_zero_u32(hits, p.pairs * p.cells, s);
+    }
+    NGT_SWITCH_C(p.C, NGT_SWITCH_BM(boundary, NGT_SWITCH_CLAMP(clamp,
+        if (ref) hipLaunchKernelGGL((advect_mc_bwd_kernel<CC, BB, KK, true>), grid, block, 0, s, p.a, input, slp,
+                                    output_grad, g_sl, input_grad, g_vc, tmp, hits);
+        else hipLaunchKernelGGL((advect_mc_bwd_kernel<CC, BB, KK, false>), grid, block, 0, s, p.a, input, slp,
+                                output_grad, g_sl, input_grad, g_vc, tmp, hits))));
+    if (ref) {
+      NGT_SWITCH_C(p.C, hipLaunchKernelGGL((add_normalized_kernel<CC>), grid, block, 0, s, p.a, tmp, hits, g_sl, 0));
+    }
+    gsl_in = g_sl;
+  }
+  if (ref) {
+    fill_zero(tmp, field, s);
+    fill_zero_u32(hits, p.pairs * p.cells, s);
+  }
+  const bool neg = order == 1 && clamp == NGT_CLAMP_NEGATIVE;
+  NGT_SWITCH_C(p.C, NGT_SWITCH_BM(boundary,
+      if (neg) {
+        if (ref) hipLaunchKernelGGL((advect_sl_bwd_kernel<CC, BB, true, true>), grid, block, 0, s, p.a, input, gsl_in,
+                                    nullptr, input_grad, g_vc, tmp, hits);
+        else hipLaunchKernelGGL((advect_sl_bwd_kernel<CC, BB, true, false>), grid, block, 0, s, p.a, input, gsl_in,
+                                nullptr, input_grad, g_vc, tmp, hits);
+      } else {
+        if (ref) hipLaunchKernelGGL((advect_sl_bwd_kernel<CC, BB, false, true>), grid, block, 0, s, p.a, input,
+                                    gsl_in, nullptr, input_grad, g_vc, tmp, hits);
+        else hipLaunchKernelGGL((advect_sl_bwd_kernel<CC, BB, false, false>), grid, block, 0, s, p.a, input, gsl_in,
+                                nullptr, input_grad, g_vc, tmp, hits);
+      }));
+  if (ref) {
+    NGT_SWITCH_C(p.C, hipLaunchKernelGGL((add_normalized_kernel<CC>), grid, block, 0, s, p.a, tmp, hits, input_grad, 1));
+  }
+  if (p.a.staggered) {
+    const Dims d = p.a.d;
+    const int64_t faces = (static_cast<int64_t>(d.z) * d.y * (d.x + 1) + static_cast<int64_t>(d.z) * (d.y + 1) * d.x +
+                           static_cast<int64_t>(d.z + 1) * d.y * d.x) * p.a.n_vel;
+    hipLaunchKernelGGL(faces_from_centered_grad_kernel, dim3(blocks_for(faces, 256)), dim3(256), 0, s, d, p.a.n_vel,
+                       g_vc, vel_x_grad, vel_y_grad, vel_z_grad);
+  }
+  return check_launch("ngt_advect_bwd");
+}
+
+}  // extern "C"

@@ -1,0 +1,516 @@
+// Device-side building blocks shared by every NGT kernel (gfx950 / CDNA4).
+//
+//  * Grids are NDHWC fp32, x fastest (flat = ((z*H + y)*W + x)*C + c), the
+//    layout of the reference ops (vector_io.inl:21-37, resample_grid.cc:25).
+//  * Sampling positions are in integer-centred cell units (cell i has its
+//    centre at q = i) after the caller removed the cell-centre offset
+//    (sampling.hpp:802-826).  Corners are floor(q) / ceil(q) and weights
+//    fract(q) = q - floor(q); interpolation runs x, then y, then z with
+//    lerp(a,b,t) = a + t*(b-a) (sampling.hpp:371-465).
+//  * Boundary modes BORDER (0 outside [-0.5, dims-0.5), out-of-grid corners 0),
+//    CLAMP, WRAP (sampling.hpp:381-440).
+//
+// Everything here is header-only and force-inlined into the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NGT_DEVICE __device__ __forceinline__
+
+namespace ngt {
+
+enum Boundary : int { BORDER = 0, CLAMP = 1, WRAP = 2 };
+enum Filter : int { NEAREST = 0, LINEAR = 1, FMIN = 2, FMAX = 3 };
+enum BlendMode : int { BEER_LAMBERT = 0, ALPHA = 2, ALPHA_ADDITIVE = 3, ADDITIVE = 4 };
+
+constexpr float kFloatMax = 3.402823466e+38f;
+
+// ---------------------------------------------------------------- vectors ---
+
+template <int C>
+struct Vec {
+  float v[C];
+  NGT_DEVICE float& operator[](int i) { return v[i]; }
+  NGT_DEVICE float operator[](int i) const { return v[i]; }
+};
+
+template <int C>
+NGT_DEVICE Vec<C> vzero() {
+  Vec<C> r;
+#pragma unroll
+  for (int i = 0; i < C; ++i) r.v[i] = 0.f;
+  return r;
+}
+template <int C>
+NGT_DEVICE Vec<C> vfill(float s) {
+  Vec<C> r;
+#pragma unroll
+  for (int i = 0; i < C; ++i) r.v[i] = s;
+  return r;
+}
+
+// Vector loads/stores: one dwordx{1,2,4} access per cell for C in {1,2,4}.
+template <int C>
+NGT_DEVICE Vec<C> vload(const float* __restrict__ p) {
+  Vec<C> r;
+  if constexpr (C == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+  } else if constexpr (C == 2) {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    r.v[0] = t.x; r.v[1] = t.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < C; ++i) r.v[i] = p[i];
+  }
+  return r;
+}
+template <int C>
+NGT_DEVICE void vstore(float* __restrict__ p, const Vec<C>& r) {
+  if constexpr (C == 4) {
+    *reinterpret_cast<float4*>(p) = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+  } else if constexpr (C == 2) {
+    *reinterpret_cast<float2*>(p) = make_float2(r.v[0], r.v[1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < C; ++i) p[i] = r.v[i];
+  }
+}
+
+template <int C>
+NGT_DEVICE Vec<C> vlerp(const Vec<C>& a, const Vec<C>& b, float t) {
+  Vec<C> r;
+#pragma unroll
+  for (int i = 0; i < C; ++i) r.v[i] = a.v[i] + t * (b.v[i] - a.v[i]);
+  return r;
+}
+template <int C>
+NGT_DEVICE Vec<C> vsub(const Vec<C>& a, const Vec<C>& b) {
+  Vec<C> r;
+#pragma unroll
+  for (int i = 0; i < C; ++i) r.v[i] = a.v[i] - b.v[i];
+  return r;
+}
+template <int C>
+NGT_DEVICE Vec<C> vscale(const Vec<C>& a, float s) {
+  Vec<C> r;
+#pragma unroll
+  for (int i = 0; i < C; ++i) r.v[i] = a.v[i] * s;
+  return r;
+}
+template <int C>
+NGT_DEVICE float vdot(const Vec<C>& a, const Vec<C>& b) {
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < C; ++i) s += a.v[i] * b.v[i];
+  return s;
+}
+
+// Hardware fp32 atomic add (global_atomic_add_f32, no CAS loop).
+NGT_DEVICE void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// ------------------------------------------------------------------- grid ---
+
+struct Dims {
+  int x, y, z;  // W, H, D
+};
+
+NGT_DEVICE int wrap_index(int i, int m) {
+  // floor-mod; the reference's (i%m)+m*(i<0) differs only at i = -k*m where it
+  // yields m (an out-of-range read), reachable only for shifts >= a grid extent
+  int r = i % m;
+  return r < 0 ? r + m : r;
+}
+
+NGT_DEVICE int64_t cell_offset(const Dims& d, int x, int y, int z) {
+  return (static_cast<int64_t>(z) * d.y + y) * d.x + x;
+}
+
+struct Corners {
+  int x0, y0, z0, x1, y1, z1;   // floor / ceil indices (wrapped or clamped)
+  float wx, wy, wz;             // ceil weights = fract(q)
+  bool vx0, vy0, vz0, vx1, vy1, vz1;  // per-axis in-grid flags (BORDER)
+  bool inside;                  // BORDER forward validity of the position
+};
+
+template <int BM>
+NGT_DEVICE Corners make_corners(const Dims& d, float qx, float qy, float qz) {
+  Corners c;
+  const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+  c.wx = qx - fx; c.wy = qy - fy; c.wz = qz - fz;
+  c.x0 = static_cast<int>(fx); c.y0 = static_cast<int>(fy); c.z0 = static_cast<int>(fz);
+  c.x1 = static_cast<int>(ceilf(qx)); c.y1 = static_cast<int>(ceilf(qy)); c.z1 = static_cast<int>(ceilf(qz));
+  if constexpr (BM == BORDER) {
+    c.inside = (qx + 0.5f >= 0.f) && (qx + 0.5f < static_cast<float>(d.x)) &&
+               (qy + 0.5f >= 0.f) && (qy + 0.5f < static_cast<float>(d.y)) &&
+               (qz + 0.5f >= 0.f) && (qz + 0.5f < static_cast<float>(d.z));
+    c.vx0 = c.x0 > -1 && c.x0 < d.x; c.vx1 = c.x1 > -1 && c.x1 < d.x;
+    c.vy0 = c.y0 > -1 && c.y0 < d.y; c.vy1 = c.y1 > -1 && c.y1 < d.y;
+    c.vz0 = c.z0 > -1 && c.z0 < d.z; c.vz1 = c.z1 > -1 && c.z1 < d.z;
+  } else {
+    c.inside = true;
+    c.vx0 = c.vx1 = c.vy0 = c.vy1 = c.vz0 = c.vz1 = true;
+    if constexpr (BM == CLAMP) {
+      c.x0 = min(max(c.x0, 0), d.x - 1); c.x1 = min(max(c.x1, 0), d.x - 1);
+      c.y0 = min(max(c.y0, 0), d.y - 1); c.y1 = min(max(c.y1, 0), d.y - 1);
+      c.z0 = min(max(c.z0, 0), d.z - 1); c.z1 = min(max(c.z1, 0), d.z - 1);
+    } else {
+      c.x0 = wrap_index(c.x0, d.x); c.x1 = wrap_index(c.x1, d.x);
+      c.y0 = wrap_index(c.y0, d.y); c.y1 = wrap_index(c.y1, d.y);
+      c.z0 = wrap_index(c.z0, d.z); c.z1 = wrap_index(c.z1, d.z);
+    }
+  }
+  return c;
+}
+
+// The 8 corner values, index bit0 = x-ceil, bit1 = y-ceil, bit2 = z-ceil.
+template <int C>
+struct Corner8 {
+  Vec<C> v[8];
+  bool ok[8];
+};
+
+template <int C, int BM>
+NGT_DEVICE Corner8<C> fetch8(const float* __restrict__ g, const Dims& d, const Corners& c) {
+  Corner8<C> r;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int ix = (k & 1) ? c.x1 : c.x0;
+    const int iy = (k & 2) ? c.y1 : c.y0;
+    const int iz = (k & 4) ? c.z1 : c.z0;
+    bool ok = true;
+    if constexpr (BM == BORDER) {
+      ok = ((k & 1) ? c.vx1 : c.vx0) && ((k & 2) ? c.vy1 : c.vy0) && ((k & 4) ? c.vz1 : c.vz0);
+    }
+    r.ok[k] = ok;
+    r.v[k] = ok ? vload<C>(g + cell_offset(d, ix, iy, iz) * C) : vzero<C>();
+  }
+  return r;
+}
+
+template <int C, int BM>
+NGT_DEVICE Vec<C> read_linear(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz) {
+  const Corners c = make_corners<BM>(d, qx, qy, qz);
+  if constexpr (BM == BORDER) {
+    if (!c.inside) return vzero<C>();
+  }
+  const Corner8<C> k = fetch8<C, BM>(g, d, c);
+  const Vec<C> v0 = vlerp(vlerp(k.v[0], k.v[1], c.wx), vlerp(k.v[2], k.v[3], c.wx), c.wy);
+  const Vec<C> v1 = vlerp(vlerp(k.v[4], k.v[5], c.wx), vlerp(k.v[6], k.v[7], c.wx), c.wy);
+  return vlerp(v0, v1, c.wz);
+}
+
+// Nearest cell: int(q + 0.5) truncates toward zero (sampling.hpp:570-584).
+template <int BM>
+NGT_DEVICE bool nearest_cell(const Dims& d, float qx, float qy, float qz, int& ix, int& iy, int& iz) {
+  ix = static_cast<int>(qx + 0.5f); iy = static_cast<int>(qy + 0.5f); iz = static_cast<int>(qz + 0.5f);
+  if constexpr (BM == BORDER) {
+    return ix >= 0 && ix < d.x && iy >= 0 && iy < d.y && iz >= 0 && iz < d.z;
+  } else if constexpr (BM == CLAMP) {
+    ix = min(max(ix, 0), d.x - 1); iy = min(max(iy, 0), d.y - 1); iz = min(max(iz, 0), d.z - 1);
+  } else {
+    ix = wrap_index(ix, d.x); iy = wrap_index(iy, d.y); iz = wrap_index(iz, d.z);
+  }
+  return true;
+}
+
+template <int C, int BM>
+NGT_DEVICE Vec<C> read_nearest(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz) {
+  int ix, iy, iz;
+  if (!nearest_cell<BM>(d, qx, qy, qz, ix, iy, iz)) return vzero<C>();
+  return vload<C>(g + cell_offset(d, ix, iy, iz) * C);
+}
+
+// Min / max over the 8 corners (sampling.hpp:585-704).  ``sel`` receives, per
+// channel, the corner index (0..7) whose value was selected, or -1 when the
+// BORDER start value 0 won.
+template <int C, int BM, bool IS_MAX>
+NGT_DEVICE Vec<C> read_minmax(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz,
+                              int* sel = nullptr, Corners* cout = nullptr) {
+  const Corners c = make_corners<BM>(d, qx, qy, qz);
+  if (cout) *cout = c;
+  Vec<C> acc = vfill<C>(IS_MAX ? -kFloatMax : kFloatMax);
+  if (sel) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) sel[i] = -1;
+  }
+  if constexpr (BM == BORDER) {
+    if (!c.inside) return vzero<C>();
+    if (!(c.vx0 && c.vx1 && c.vy0 && c.vy1 && c.vz0 && c.vz1)) acc = vzero<C>();
+  }
+  const Corner8<C> k = fetch8<C, BM>(g, d, c);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (!k.ok[j]) continue;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      const bool better = IS_MAX ? (k.v[j].v[i] > acc.v[i]) : (k.v[j].v[i] < acc.v[i]);
+      if (better) {
+        acc.v[i] = k.v[j].v[i];
+        if (sel) sel[i] = j;
+      }
+    }
+  }
+  return acc;
+}
+
+template <int C, int FM, int BM>
+NGT_DEVICE Vec<C> read_filtered(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz) {
+  if constexpr (FM == LINEAR) return read_linear<C, BM>(g, d, qx, qy, qz);
+  else if constexpr (FM == NEAREST) return read_nearest<C, BM>(g, d, qx, qy, qz);
+  else if constexpr (FM == FMIN) return read_minmax<C, BM, false>(g, d, qx, qy, qz);
+  else return read_minmax<C, BM, true>(g, d, qx, qy, qz);
+}
+
+// d read_linear / d q per channel (sampling.hpp:470-565).  Coinciding
+// floor/ceil corners make the derivative along that axis 0 at integer q.
+template <int C, int BM>
+NGT_DEVICE void read_gradient(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz,
+                              Vec<C>& dx, Vec<C>& dy, Vec<C>& dz) {
+  const Corners c = make_corners<BM>(d, qx, qy, qz);
+  if constexpr (BM == BORDER) {
+    if (!c.inside) {
+      dx = dy = dz = vzero<C>();
+      return;
+    }
+  }
+  const Corner8<C> k = fetch8<C, BM>(g, d, c);
+  dx = vlerp(vlerp(vsub(k.v[1], k.v[0]), vsub(k.v[3], k.v[2]), c.wy),
+             vlerp(vsub(k.v[5], k.v[4]), vsub(k.v[7], k.v[6]), c.wy), c.wz);
+  dy = vlerp(vlerp(vsub(k.v[2], k.v[0]), vsub(k.v[3], k.v[1]), c.wx),
+             vlerp(vsub(k.v[6], k.v[4]), vsub(k.v[7], k.v[5]), c.wx), c.wz);
+  dz = vlerp(vlerp(vsub(k.v[4], k.v[0]), vsub(k.v[5], k.v[1]), c.wx),
+             vlerp(vsub(k.v[6], k.v[2]), vsub(k.v[7], k.v[3]), c.wx), c.wy);
+}
+
+// Scatter ``val`` with trilinear weights to the 8 corners.
+//  REF_MASK = false: exact adjoint (BORDER forward validity).
+//  REF_MASK = true : the reference's acceptance -1 < q < dims
+//                    (resample_grid.cu.cc:1082, sampling_v2.hpp:475).
+//  hits  (nullable): +1 per in-grid corner (count normalisation).
+//  wsum  (nullable): +weight per in-grid corner (weight normalisation).
+template <int C, int BM, bool REF_MASK>
+NGT_DEVICE void scatter_linear(float* __restrict__ g, const Dims& d, float qx, float qy, float qz, const Vec<C>& val,
+                               uint32_t* __restrict__ hits = nullptr, float* __restrict__ wsum = nullptr) {
+  const Corners c = make_corners<BM>(d, qx, qy, qz);
+  if constexpr (BM == BORDER) {
+    if constexpr (REF_MASK) {
+      if (!(qx > -1.f && qx < static_cast<float>(d.x) && qy > -1.f && qy < static_cast<float>(d.y) &&
+            qz > -1.f && qz < static_cast<float>(d.z)))
+        return;
+    } else {
+      if (!c.inside) return;
+    }
+  }
+  const float fw[2][3] = {{1.f - c.wx, 1.f - c.wy, 1.f - c.wz}, {c.wx, c.wy, c.wz}};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
+    if constexpr (BM == BORDER) {
+      const bool ok = (bx ? c.vx1 : c.vx0) && (by ? c.vy1 : c.vy0) && (bz ? c.vz1 : c.vz0);
+      if (!ok) continue;
+    }
+    const int ix = bx ? c.x1 : c.x0, iy = by ? c.y1 : c.y0, iz = bz ? c.z1 : c.z0;
+    const float w = fw[bx][0] * fw[by][1] * fw[bz][2];
+    const int64_t o = cell_offset(d, ix, iy, iz);
+#pragma unroll
+    for (int i = 0; i < C; ++i) atomic_add(g + o * C + i, val.v[i] * w);
+    if (hits) atomicAdd(hits + o, 1u);
+    if (wsum) atomic_add(wsum + o, w);
+  }
+}
+
+template <int C, int BM>
+NGT_DEVICE void scatter_nearest(float* __restrict__ g, const Dims& d, float qx, float qy, float qz, const Vec<C>& val) {
+  int ix, iy, iz;
+  if (!nearest_cell<BM>(d, qx, qy, qz, ix, iy, iz)) return;
+  const int64_t o = cell_offset(d, ix, iy, iz);
+#pragma unroll
+  for (int i = 0; i < C; ++i) atomic_add(g + o * C + i, val.v[i]);
+}
+
+template <int C, int BM, bool IS_MAX>
+NGT_DEVICE void scatter_minmax(const float* __restrict__ src, float* __restrict__ g, const Dims& d, float qx, float qy,
+                               float qz, const Vec<C>& val) {
+  int sel[C];
+  Corners c;
+  read_minmax<C, BM, IS_MAX>(src, d, qx, qy, qz, sel, &c);
+  if constexpr (BM == BORDER) {
+    if (!c.inside) return;
+  }
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const int k = sel[i];
+    if (k < 0) continue;
+    const int ix = (k & 1) ? c.x1 : c.x0, iy = (k & 2) ? c.y1 : c.y0, iz = (k & 4) ? c.z1 : c.z0;
+    atomic_add(g + cell_offset(d, ix, iy, iz) * C + i, val.v[i]);
+  }
+}
+
+// Exact adjoint of read_filtered<FM>.
+template <int C, int FM, int BM>
+NGT_DEVICE void scatter_filtered(const float* __restrict__ src, float* __restrict__ g, const Dims& d, float qx,
+                                 float qy, float qz, const Vec<C>& val) {
+  if constexpr (FM == LINEAR) scatter_linear<C, BM, false>(g, d, qx, qy, qz, val);
+  else if constexpr (FM == NEAREST) scatter_nearest<C, BM>(g, d, qx, qy, qz, val);
+  else if constexpr (FM == FMIN) scatter_minmax<C, BM, false>(src, g, d, qx, qy, qz, val);
+  else scatter_minmax<C, BM, true>(src, g, d, qx, qy, qz, val);
+}
+
+// ------------------------------------------------------------- cameras ------
+
+// One (model, camera) pair, matrices row-major in math order, prepared on the
+// host in FP64 and rounded to FP32 (the reference inverts in FP32 on the host,
+// transformations.hpp:48-68).
+struct CamParams {
+  float mv[16];
+  float mv_inv[16];
+  float p[16];
+  float p_inv[16];
+  float near_, far_;
+  int data_index;    // batch entry n
+  int out_index;     // output slot n*V + v (or n)
+};
+
+constexpr int kMaxCamsPerLaunch = 8;
+
+struct CamBatch {
+  CamParams cam[kMaxCamsPerLaunch];
+  int count;
+};
+
+NGT_DEVICE void mat4_apply(const float* __restrict__ m, float x, float y, float z, float w, float& ox, float& oy,
+                           float& oz, float& ow) {
+  ox = m[0] * x + m[1] * y + m[2] * z + m[3] * w;
+  oy = m[4] * x + m[5] * y + m[6] * z + m[7] * w;
+  oz = m[8] * x + m[9] * y + m[10] * z + m[11] * w;
+  ow = m[12] * x + m[13] * y + m[14] * z + m[15] * w;
+}
+
+// NDC -> VS (divide by w) -> OS (transformations.hpp:124-138).
+NGT_DEVICE void ndc_to_os(const CamParams& cp, float nx, float ny, float nz, float& ox, float& oy, float& oz) {
+  float vx, vy, vz, vw;
+  mat4_apply(cp.p_inv, nx, ny, nz, 1.f, vx, vy, vz, vw);
+  const float iw = 1.f / vw;
+  vx *= iw; vy *= iw; vz *= iw;
+  float ow;
+  mat4_apply(cp.mv_inv, vx, vy, vz, 1.f, ox, oy, oz, ow);
+}
+
+NGT_DEVICE void idx_to_os(const CamParams& cp, float ix, float iy, float iz, float dx_inv, float dy_inv, float dz_inv,
+                          float& ox, float& oy, float& oz) {
+  ndc_to_os(cp, ix * dx_inv * 2.f - 1.f, iy * dy_inv * 2.f - 1.f, iz * dz_inv * 2.f - 1.f, ox, oy, oz);
+}
+
+// Linear view-space depth slices (transformations.hpp:139-149).
+NGT_DEVICE void idx_to_os_lindepth(const CamParams& cp, float ix, float iy, float iz, float dx_inv, float dy_inv,
+                                   float dz_inv, float& ox, float& oy, float& oz) {
+  const float t = iz * dz_inv;
+  const float zv = -fmaf(cp.far_ - cp.near_, t, cp.near_);
+  const float nz = -cp.p[10] - cp.p[11] / zv;
+  ndc_to_os(cp, ix * dx_inv * 2.f - 1.f, iy * dy_inv * 2.f - 1.f, nz, ox, oy, oz);
+}
+
+NGT_DEVICE void os_to_ndc(const CamParams& cp, float x, float y, float z, float& nx, float& ny, float& nz) {
+  float vx, vy, vz, vw;
+  mat4_apply(cp.mv, x, y, z, 1.f, vx, vy, vz, vw);
+  float cx, cy, cz, cw;
+  mat4_apply(cp.p, vx, vy, vz, vw, cx, cy, cz, cw);
+  const float iw = 1.f / cw;
+  nx = cx * iw; ny = cy * iw; nz = cz * iw;
+}
+
+// OStoIDX (transformations.hpp:100-108).
+NGT_DEVICE void os_to_idx(const CamParams& cp, float x, float y, float z, float dx, float dy, float dz, float& ix,
+                          float& iy, float& iz) {
+  float nx, ny, nz;
+  os_to_ndc(cp, x, y, z, nx, ny, nz);
+  ix = (nx * 0.5f + 0.5f) * dx; iy = (ny * 0.5f + 0.5f) * dy; iz = (nz * 0.5f + 0.5f) * dz;
+}
+
+// OStoIDXlinearDepth (transformations.hpp:110-121).
+NGT_DEVICE void os_to_idx_lindepth(const CamParams& cp, float x, float y, float z, float dx, float dy, float dz,
+                                   float& ix, float& iy, float& iz) {
+  float nx, ny, nz;
+  os_to_ndc(cp, x, y, z, nx, ny, nz);
+  ix = (nx * 0.5f + 0.5f) * dx; iy = (ny * 0.5f + 0.5f) * dy;
+  const float zv = -cp.p[11] / (nz + cp.p[10]);
+  const float t = -(zv + cp.near_) / (cp.far_ - cp.near_);
+  iz = t * dz;
+}
+
+// ------------------------------------------------------------- blending -----
+
+// Forward blend functors (blending.hpp:9-243).  C == 1 and ADDITIVE are plain
+// sums; for C in {2,4} the last channel is density / alpha.
+template <int C, int MODE>
+NGT_DEVICE void blend_step(Vec<C>& acc, const Vec<C>& cell) {
+  if constexpr (C == 1 || MODE == ADDITIVE) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) acc.v[i] += cell.v[i];
+  } else if constexpr (MODE == BEER_LAMBERT) {
+    const float dd = fmaxf(acc.v[C - 1] + cell.v[C - 1], 0.f);
+    const float t = expf(-dd);
+#pragma unroll
+    for (int i = 0; i < C - 1; ++i) acc.v[i] += t * cell.v[i];
+    acc.v[C - 1] = dd;
+  } else if constexpr (MODE == ALPHA) {
+    const float t = 1.f - acc.v[C - 1];
+#pragma unroll
+    for (int i = 0; i < C - 1; ++i) acc.v[i] += t * cell.v[i];
+    acc.v[C - 1] += t * cell.v[C - 1];
+  } else {  // ALPHA_ADDITIVE
+    const float t = 1.f - cell.v[C - 1];
+#pragma unroll
+    for (int i = 0; i < C - 1; ++i) acc.v[i] += t * cell.v[i];
+    acc.v[C - 1] += cell.v[C - 1];
+  }
+}
+
+// Reference backward functor verbatim in behaviour (blending.hpp:40-54,
+// 139-153, 188-202): updates ``grads`` to the gradient w.r.t. the previous
+// accumulator, reconstructs ``cell_out`` to the previous accumulator and
+// returns the gradient w.r.t. the cell.
+template <int C, int MODE>
+NGT_DEVICE Vec<C> blend_grad_reference(Vec<C>& grads, const Vec<C>& cell, Vec<C>& cell_out) {
+  if constexpr (C == 1 || MODE == ADDITIVE) {
+    return grads;
+  } else {
+    float lg = 0.f;
+#pragma unroll
+    for (int i = 0; i < C - 1; ++i) lg += cell.v[i] * grads.v[i];
+    Vec<C> gc;
+    if constexpr (MODE == BEER_LAMBERT) {
+      const float e = expf(-cell_out.v[C - 1]);
+      cell_out.v[C - 1] = fmaxf(cell_out.v[C - 1] - cell.v[C - 1], 0.f);
+      const float dd = grads.v[C - 1] - e * lg;
+#pragma unroll
+      for (int i = 0; i < C - 1; ++i) gc.v[i] = e * grads.v[i];
+      gc.v[C - 1] = dd;
+      grads.v[C - 1] = dd;
+    } else if constexpr (MODE == ALPHA) {
+      cell_out.v[C - 1] = (1.f - cell_out.v[C - 1]) / (1.f - cell.v[C - 1]);
+      const float t = 1.f - cell_out.v[C - 1];
+      const float dacc = (1.f - cell.v[C - 1]) * grads.v[C - 1] - lg;
+#pragma unroll
+      for (int i = 0; i < C - 1; ++i) gc.v[i] = t * grads.v[i];
+      gc.v[C - 1] = t * grads.v[C - 1];
+      grads.v[C - 1] = dacc;
+    } else {  // ALPHA_ADDITIVE
+      cell_out.v[C - 1] = cell_out.v[C - 1] - cell.v[C - 1];
+      const float t = 1.f - cell.v[C - 1];
+#pragma unroll
+      for (int i = 0; i < C - 1; ++i) gc.v[i] = t * grads.v[i];
+      gc.v[C - 1] = grads.v[C - 1] - lg;
+    }
+    return gc;
+  }
+}
+
+// ------------------------------------------------------- misc kernels -------
+
+__global__ void fill_kernel(float* __restrict__ p, int64_t n, float value);
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t value);
+
+}  // namespace ngt

@@ -1,0 +1,72 @@
+// Host-side helpers shared by the C-ABI entry points: error state, argument
+// checks, FP64 camera preparation, launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ngt_common.hpp"
+#include "ngt_transport.h"
+
+namespace ngt {
+
+// Thread-local message of the last failure (returned by ngt_last_error).
+void set_error(const std::string& msg);
+void clear_error();
+
+struct Status {
+  int code = NGT_OK;
+  static Status ok() { return Status(); }
+};
+
+// Return an error code from an entry point with a message.
+#define NGT_FAIL(code_, msg_)            \
+  do {                                   \
+    ::ngt::set_error(msg_);              \
+    return (code_);                      \
+  } while (0)
+
+#define NGT_REQUIRE(cond_, code_, msg_) \
+  do {                                  \
+    if (!(cond_)) NGT_FAIL(code_, msg_); \
+  } while (0)
+
+// Checks hipGetLastError after launches; turns a HIP error into NGT_ERR_HIP.
+int check_launch(const char* where);
+
+inline hipStream_t as_stream(ngt_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+struct Shape5 {
+  int64_t n, d, h, w, c;
+  int64_t cells() const { return d * h * w; }
+  int64_t elems() const { return n * d * h * w * c; }
+  Dims dims() const { return Dims{static_cast<int>(w), static_cast<int>(h), static_cast<int>(d)}; }
+};
+
+inline Shape5 shape5(const int64_t* s) { return Shape5{s[0], s[1], s[2], s[3], s[4]}; }
+
+// Validates an NDHWC input shape like the reference ops
+// (resample_grid.cc:371-377): 5-D, C <= 4, C not 3, non-empty.
+int check_grid_shape(const int64_t* s, const char* what, bool allow_c3 = false);
+
+// (data entry, camera, output slot) triples of one op call.
+struct Pair {
+  int n, v, out;
+};
+int make_pairs(int N, int V, bool separate_camera_batch, std::vector<Pair>& pairs);
+
+// FP64 camera parameters of one pair: MV = V*M, inverses in double, rounded
+// to float (transformations.hpp:48-68 does the same in float).
+CamParams make_cam(const float* m16, const float* v16, const float* p16, const float* f6, int data_index,
+                   int out_index);
+
+// Zero-fill with a kernel (not hipMemsetAsync, so graph capture stays clean).
+void fill_zero(float* p, int64_t n, hipStream_t s);
+void fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s);
+
+inline unsigned blocks_for(int64_t n, int threads) { return static_cast<unsigned>((n + threads - 1) / threads); }
+
+}  // namespace ngt

@@ -1,0 +1,468 @@
+// RaymarchGridTransform (+Grad) and the fused density renderer on gfx950.
+//
+// One work-item per pixel; a 16x16 work-group is remapped so each wavefront
+// owns an 8x8 pixel tile, which keeps the 64 rays of a wavefront spatially
+// coherent and their 8-tap gathers inside a few cache lines per step.  All
+// (batch, camera) pairs of a call run in one launch (grid.z), with camera
+// matrices passed by value as kernel arguments instead of __constant__
+// uploads per camera (raymarch_grid.cu.cc:399-430).
+//
+// Ray lattice (raymarch_grid.cu.cc:173-200): start = IDXtoOS(x+.5, y+.5, .5),
+// end = IDXtoOS(x+.5, y+.5, zMax+.5), step = (end-start)/zMax, D samples;
+// BORDER clips to the cell-centre AABB widened by one cell.  Samples are
+// evaluated as start + k*step (the reference accumulates samplePos += step).
+//
+// Exact backward: a forward walk over the same lattice recomputing the
+// samples; the per-sample cell gradient uses
+//   d sigma_k = g_d - (<L_D, g_L> - sum_{j<k} e_j <l_j, g_L>)
+// (BEER_LAMBERT), so no accumulator is rebuilt by subtraction; ALPHA walks
+// backward.  The volume gradient is scattered with hardware fp32 atomics.
+// Reference semantics (gradient_semantics = REFERENCE) reproduce the
+// reference: reverse walk from `end`, subtraction reconstruction, corner
+// weights accumulated and the gradient divided by them per camera.
+#include "ngt_host.hpp"
+
+namespace ngt {
+namespace {
+
+struct RayArgs {
+  const float* input;
+  Dims din;
+  int out_w, out_h, out_d;
+  float inv_w, inv_h, inv_d;
+  float light_r, light_g, light_b, depth_step;  // fused density renderer only
+};
+
+NGT_DEVICE int kMaxRaySteps(const RayArgs& a) { return 16 * (a.din.x + a.din.y + a.din.z + a.out_d); }
+
+struct Ray {
+  float sx, sy, sz;  // start
+  float ex, ey, ez;  // end
+  float dx, dy, dz;  // step
+  int steps;
+  bool hit;
+};
+
+// AABB entry / exit distances (raymarch_grid.cu.cc:109-156).  fmaxf / fminf
+// drop NaNs produced by axis-parallel rays (0 * inf).
+NGT_DEVICE void ray_box(const Dims& din, float ox, float oy, float oz, float rx, float ry, float rz, float border,
+                        float& tmin, float& tmax) {
+  const float bx = static_cast<float>(din.x), by = static_cast<float>(din.y), bz = static_cast<float>(din.z);
+  const float hx = bx * 0.5f + border, hy = by * 0.5f + border, hz = bz * 0.5f + border;
+  const float cx = ox - bx * 0.5f, cy = oy - by * 0.5f, cz = oz - bz * 0.5f;
+  const float ix = 1.0f / rx, iy = 1.0f / ry, iz = 1.0f / rz;
+  const float nx = ix * cx, ny = iy * cy, nz = iz * cz;
+  const float kx = fabsf(ix) * hx, ky = fabsf(iy) * hy, kz = fabsf(iz) * hz;
+  tmin = fmaxf(fmaxf(0.0f - nx - kx, 0.0f - ny - ky), 0.0f - nz - kz);
+  tmax = fminf(fminf(kx - nx, ky - ny), kz - nz);
+}
+
+template <int BM>
+NGT_DEVICE Ray make_ray(const RayArgs& a, const CamParams& cp, int x, int y) {
+  Ray r;
+  const float zmax = static_cast<float>(max(a.out_d - 1, 1));
+  const float px = static_cast<float>(x) + 0.5f, py = static_cast<float>(y) + 0.5f;
+  idx_to_os(cp, px, py, 0.5f, a.inv_w, a.inv_h, a.inv_d, r.sx, r.sy, r.sz);
+  idx_to_os(cp, px, py, zmax + 0.5f, a.inv_w, a.inv_h, a.inv_d, r.ex, r.ey, r.ez);
+  r.dx = (r.ex - r.sx) / zmax; r.dy = (r.ey - r.sy) / zmax; r.dz = (r.ez - r.sz) / zmax;
+  r.steps = a.out_d;
+  r.hit = true;
+  if constexpr (BM == BORDER) {
+    float ux = r.ex - r.sx, uy = r.ey - r.sy, uz = r.ez - r.sz;
+    const float inv_len = rsqrtf(ux * ux + uy * uy + uz * uz);
+    ux *= inv_len; uy *= inv_len; uz *= inv_len;
+    float tmin, tmax;
+    ray_box(a.din, r.sx, r.sy, r.sz, ux, uy, uz, -0.5f, tmin, tmax);
+    if (tmin > tmax || tmax < 0.f) {
+      r.hit = false;
+      r.steps = 0;
+      return r;
+    }
+    r.ex = r.sx + ux * (tmax + 1.f); r.ey = r.sy + uy * (tmax + 1.f); r.ez = r.sz + uz * (tmax + 1.f);
+    const float t0 = fmaxf(tmin - 1.f, 0.f);
+    r.sx = r.sx + ux * t0; r.sy = r.sy + uy * t0; r.sz = r.sz + uz * t0;
+    const float step_len = sqrtf(r.dx * r.dx + r.dy * r.dy + r.dz * r.dz);
+    const float lx = r.sx - r.ex, ly = r.sy - r.ey, lz = r.sz - r.ez;
+    const float ray_len = sqrtf(lx * lx + ly * ly + lz * lz);
+    r.steps = static_cast<int>(ray_len / step_len) + 2;
+  }
+  // Bound the walk for degenerate cameras (step length ~ 0): no sane frustum
+  // needs more samples than a few times the grid diagonal plus the depth.
+  r.steps = min(max(r.steps, 0), kMaxRaySteps(a));
+  return r;
+}
+
+// Work-group 16x16 -> four 8x8 wavefront tiles.
+NGT_DEVICE void pixel_of_thread(int& x, int& y) {
+  const int tid = threadIdx.x;  // 0..255
+  const int wave = tid >> 6, lane = tid & 63;
+  x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+  y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+}
+
+// Cell value read by a sample: the C-channel input (LIGHT = false) or the
+// fused renderer's (rho*rgb*ds, rho*ds) from the 1-channel density.
+template <int C, int FM, int BM, bool LIGHT>
+NGT_DEVICE Vec<C> sample_cell(const RayArgs& a, const float* __restrict__ src, float qx, float qy, float qz) {
+  if constexpr (LIGHT) {
+    const Vec<1> rho = read_filtered<1, FM, BM>(src, a.din, qx, qy, qz);
+    const float s = rho.v[0] * a.depth_step;
+    Vec<C> c;
+    c.v[0] = a.light_r * s; c.v[1] = a.light_g * s; c.v[2] = a.light_b * s; c.v[3] = s;
+    return c;
+  } else {
+    return read_filtered<C, FM, BM>(src, a.din, qx, qy, qz);
+  }
+}
+
+// Scatter a cell gradient at a sample position (exact adjoint of sample_cell).
+template <int C, int FM, int BM, bool LIGHT>
+NGT_DEVICE void scatter_cell(const RayArgs& a, const float* __restrict__ src, float* __restrict__ g, float qx, float qy,
+                             float qz, const Vec<C>& gc) {
+  if constexpr (LIGHT) {
+    Vec<1> gr;
+    gr.v[0] = a.depth_step * (a.light_r * gc.v[0] + a.light_g * gc.v[1] + a.light_b * gc.v[2] + gc.v[3]);
+    scatter_filtered<1, FM, BM>(src, g, a.din, qx, qy, qz, gr);
+  } else {
+    scatter_filtered<C, FM, BM>(src, g, a.din, qx, qy, qz, gc);
+  }
+}
+
+template <int C, int FM, int BM, int MODE, bool LIGHT>
+__global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch cb, float* __restrict__ out) {
+  int x, y;
+  pixel_of_thread(x, y);
+  if (x >= a.out_w || y >= a.out_h) return;
+  const CamParams& cp = cb.cam[blockIdx.z];
+  constexpr int CIN = LIGHT ? 1 : C;
+  const float* src = a.input + static_cast<int64_t>(cp.data_index) * a.din.x * a.din.y * a.din.z * CIN;
+  const Ray r = make_ray<BM>(a, cp, x, y);
+  Vec<C> acc = vzero<C>();
+  for (int k = 0; k < r.steps; ++k) {
+    const float kf = static_cast<float>(k);
+    const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
+    blend_step<C, MODE>(acc, sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz));
+  }
+  vstore<C>(out + ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C, acc);
+}
+
+template <int C, int FM, int BM, int MODE, bool LIGHT>
+__global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamBatch cb, const float* __restrict__ out,
+                                                                 const float* __restrict__ out_grad,
+                                                                 float* __restrict__ in_grad) {
+  int x, y;
+  pixel_of_thread(x, y);
+  if (x >= a.out_w || y >= a.out_h) return;
+  const CamParams& cp = cb.cam[blockIdx.z];
+  constexpr int CIN = LIGHT ? 1 : C;
+  const int64_t slice = static_cast<int64_t>(a.din.x) * a.din.y * a.din.z * CIN;
+  const float* src = a.input + cp.data_index * slice;
+  float* gdst = in_grad + cp.data_index * slice;
+  const Ray r = make_ray<BM>(a, cp, x, y);
+  if (r.steps <= 0) return;
+  const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
+  Vec<C> g = vload<C>(out_grad + pix);
+  if constexpr (C > 1 && MODE == ALPHA) {
+    Vec<C> acc = vload<C>(out + pix);
+    for (int k = r.steps - 1; k >= 0; --k) {
+      const float kf = static_cast<float>(k);
+      const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
+      const Vec<C> cell = sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz);
+      const float aprev = 1.f - (1.f - acc.v[C - 1]) / (1.f - cell.v[C - 1]);
+      const float t = 1.f - aprev;
+      float lg = 0.f;
+#pragma unroll
+      for (int i = 0; i < C - 1; ++i) lg += cell.v[i] * g.v[i];
+      Vec<C> gc;
+#pragma unroll
+      for (int i = 0; i < C - 1; ++i) gc.v[i] = t * g.v[i];
+      gc.v[C - 1] = t * g.v[C - 1];
+      g.v[C - 1] = (1.f - cell.v[C - 1]) * g.v[C - 1] - lg;
+      acc.v[C - 1] = aprev;
+      scatter_cell<C, FM, BM, LIGHT>(a, src, gdst, qx, qy, qz, gc);
+    }
+  } else {
+    float total = 0.f;
+    if constexpr (C > 1 && MODE == BEER_LAMBERT) {
+      const Vec<C> L = vload<C>(out + pix);
+#pragma unroll
+      for (int i = 0; i < C - 1; ++i) total += L.v[i] * g.v[i];
+    }
+    float prefix = 0.f, dacc = 0.f;
+    for (int k = 0; k < r.steps; ++k) {
+      const float kf = static_cast<float>(k);
+      const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
+      Vec<C> gc;
+      if constexpr (C == 1 || MODE == ADDITIVE) {
+        gc = g;
+      } else {
+        const Vec<C> cell = sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz);
+        float lg = 0.f;
+#pragma unroll
+        for (int i = 0; i < C - 1; ++i) lg += cell.v[i] * g.v[i];
+        if constexpr (MODE == BEER_LAMBERT) {
+          dacc = fmaxf(dacc + cell.v[C - 1], 0.f);
+          const float e = expf(-dacc);
+#pragma unroll
+          for (int i = 0; i < C - 1; ++i) gc.v[i] = e * g.v[i];
+          gc.v[C - 1] = g.v[C - 1] - (total - prefix);
+          prefix += e * lg;
+        } else {  // ALPHA_ADDITIVE
+          const float t = 1.f - cell.v[C - 1];
+#pragma unroll
+          for (int i = 0; i < C - 1; ++i) gc.v[i] = t * g.v[i];
+          gc.v[C - 1] = g.v[C - 1] - lg;
+        }
+      }
+      scatter_cell<C, FM, BM, LIGHT>(a, src, gdst, qx, qy, qz, gc);
+    }
+  }
+}
+
+// Reference gradient of one camera: reverse walk from `end` with the
+// reference's reconstruction, scatter with corner weights into a per-camera
+// buffer (raymarch_grid.cu.cc:482-521).
+template <int C, int BM, int MODE>
+__global__ __launch_bounds__(256) void raymarch_bwd_reference_kernel(RayArgs a, CamBatch cb,
+                                                                     const float* __restrict__ out,
+                                                                     const float* __restrict__ out_grad,
+                                                                     float* __restrict__ gbuf,
+                                                                     float* __restrict__ wsum) {
+  int x, y;
+  pixel_of_thread(x, y);
+  if (x >= a.out_w || y >= a.out_h) return;
+  const CamParams& cp = cb.cam[blockIdx.z];
+  const float* src = a.input + static_cast<int64_t>(cp.data_index) * a.din.x * a.din.y * a.din.z * C;
+  const Ray r = make_ray<BM>(a, cp, x, y);
+  if (!r.hit) return;
+  const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
+  Vec<C> acc = vload<C>(out + pix);
+  Vec<C> g = vload<C>(out_grad + pix);
+  for (int j = 0; j < r.steps; ++j) {
+    const float jf = static_cast<float>(j);
+    const float px = r.ex - jf * r.dx, py = r.ey - jf * r.dy, pz = r.ez - jf * r.dz;
+    const Vec<C> cell = read_linear<C, BM>(src, a.din, px - 0.5f, py - 0.5f, pz - 0.5f);
+    const Vec<C> gc = blend_grad_reference<C, MODE>(g, cell, acc);
+    scatter_linear<C, BM, true>(gbuf, a.din, px - 0.5f, py - 0.5f, pz - 0.5f, gc, nullptr, wsum);
+  }
+}
+
+// in_grad += gbuf / wsum where wsum > 0; clears both buffers for the next
+// camera (raymarch_grid.cu.cc:458-480, 700-718).
+template <int C>
+__global__ __launch_bounds__(256) void normalize_by_weight_kernel(float* __restrict__ gbuf, float* __restrict__ wsum,
+                                                                  float* __restrict__ in_grad, int64_t cells) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cells) return;
+  const float w = wsum[i];
+  if (w > 0.f) {
+    const float inv = 1.0f / w;
+    const Vec<C> gb = vload<C>(gbuf + i * C);
+    Vec<C> gi = vload<C>(in_grad + i * C);
+#pragma unroll
+    for (int c = 0; c < C; ++c) gi.v[c] += gb.v[c] * inv;
+    vstore<C>(in_grad + i * C, gi);
+    vstore<C>(gbuf + i * C, vzero<C>());
+    wsum[i] = 0.f;
+  }
+}
+
+#define NGT_SWITCH_C(C_, ...)                              \
+  switch (C_) {                                            \
+    case 1: { constexpr int CC = 1; __VA_ARGS__; } break;  \
+    case 2: { constexpr int CC = 2; __VA_ARGS__; } break;  \
+    default: { constexpr int CC = 4; __VA_ARGS__; } break; \
+  }
+#define NGT_SWITCH_BM(B_, ...)                                      \
+  switch (B_) {                                                     \
+    case BORDER: { constexpr int BB = BORDER; __VA_ARGS__; } break; \
+    case CLAMP: { constexpr int BB = CLAMP; __VA_ARGS__; } break;   \
+    default: { constexpr int BB = WRAP; __VA_ARGS__; } break;       \
+  }
+#define NGT_SWITCH_FM2(F_, ...)                                        \
+  switch (F_) {                                                        \
+    case LINEAR: { constexpr int FF = LINEAR; __VA_ARGS__; } break;    \
+    default: { constexpr int FF = NEAREST; __VA_ARGS__; } break;       \
+  }
+#define NGT_SWITCH_MODE(M_, ...)                                                   \
+  switch (M_) {                                                                    \
+    case BEER_LAMBERT: { constexpr int MM = BEER_LAMBERT; __VA_ARGS__; } break;     \
+    case ALPHA: { constexpr int MM = ALPHA; __VA_ARGS__; } break;                   \
+    case ALPHA_ADDITIVE: { constexpr int MM = ALPHA_ADDITIVE; __VA_ARGS__; } break; \
+    default: { constexpr int MM = ADDITIVE; __VA_ARGS__; } break;                   \
+  }
+
+int check_ray_args(const int64_t* shape, const float* M, const float* V, const float* P, const float* F,
+                   int num_cameras, const int32_t* out_dhw, int interpolation, int boundary, int mode,
+                   int separate, std::vector<Pair>& pairs, bool light) {
+  int rc = check_grid_shape(shape, light ? "density" : "input");
+  if (rc) return rc;
+  if (light) NGT_REQUIRE(shape[4] == 1, NGT_ERR_INVALID_ARGUMENT, "density must have 1 channel");
+  NGT_REQUIRE(M && V && P && F, NGT_ERR_INVALID_ARGUMENT, "matrices and frustum must be given");
+  NGT_REQUIRE(num_cameras > 0, NGT_ERR_INVALID_ARGUMENT, "camera batch size mismatch");
+  NGT_REQUIRE(out_dhw && out_dhw[0] > 0 && out_dhw[1] > 0 && out_dhw[2] > 0, NGT_ERR_INVALID_ARGUMENT,
+              "Invalid output_shape");
+  NGT_REQUIRE(boundary >= 0 && boundary <= 2, NGT_ERR_INVALID_ARGUMENT, "invalid boundary mode");
+  NGT_REQUIRE(mode == BEER_LAMBERT || mode == ALPHA || mode == ALPHA_ADDITIVE || mode == ADDITIVE,
+              NGT_ERR_INVALID_ARGUMENT, "invalid blend mode");
+  NGT_REQUIRE(interpolation >= 0 && interpolation <= 3, NGT_ERR_INVALID_ARGUMENT, "invalid filter mode");
+  // MIN / MAX raymarching reads through the reference's lerp-based extrema
+  // tracker (sampling_v2.hpp:149-161), whose first update computes
+  // FLOAT_MAX + (v - FLOAT_MAX) = 0; not reproduced.
+  NGT_REQUIRE(interpolation == LINEAR || interpolation == NEAREST, NGT_ERR_UNIMPLEMENTED,
+              "raymarching supports LINEAR and NEAREST interpolation");
+  return make_pairs(static_cast<int>(shape[0]), num_cameras, separate != 0, pairs);
+}
+
+RayArgs make_ray_args(const float* input, const int64_t* shape, const int32_t* out_dhw) {
+  RayArgs a;
+  a.input = input;
+  a.din = shape5(shape).dims();
+  a.out_d = out_dhw[0]; a.out_h = out_dhw[1]; a.out_w = out_dhw[2];
+  a.inv_w = 1.f / a.out_w; a.inv_h = 1.f / a.out_h; a.inv_d = 1.f / a.out_d;
+  a.light_r = a.light_g = a.light_b = 0.f;
+  a.depth_step = 1.f;
+  return a;
+}
+
+template <typename F>
+void for_cam_batches(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* Fr,
+                     F&& fn) {
+  for (size_t b = 0; b < pairs.size(); b += kMaxCamsPerLaunch) {
+    CamBatch cb;
+    cb.count = static_cast<int>(std::min<size_t>(kMaxCamsPerLaunch, pairs.size() - b));
+    for (int i = 0; i < cb.count; ++i) {
+      const Pair& p = pairs[b + i];
+      cb.cam[i] = make_cam(M + 16 * p.n, V + 16 * p.v, P + 16 * p.v, Fr + 6 * p.v, p.n, p.out);
+    }
+    fn(cb);
+  }
+}
+
+}  // namespace
+}  // namespace ngt
+
+using namespace ngt;
+
+extern "C" {
+
+int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_shape, const float* matrix_m,
+                                    const float* matrix_v, const float* matrix_p, const float* frustum,
+                                    int num_cameras, const int32_t* output_dhw, int interpolation, int boundary,
+                                    int blending_mode, int separate_camera_batch, float* output,
+                                    ngt_stream_t stream) {
+  std::vector<Pair> pairs;
+  int rc = check_ray_args(input_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
+                          boundary, blending_mode, separate_camera_batch, pairs, false);
+  if (rc) return rc;
+  const RayArgs a = make_ray_args(input, input_shape, output_dhw);
+  hipStream_t s = as_stream(stream);
+  const int C = static_cast<int>(input_shape[4]);
+  for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
+    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    NGT_SWITCH_C(C, NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+        hipLaunchKernelGGL((raymarch_fwd_kernel<CC, FF, BB, MM, false>), grid, dim3(256), 0, s, a, cb, output)))));
+  });
+  return check_launch("ngt_raymarch_grid_transform_fwd");
+}
+
+size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape, int num_cameras,
+                                                      int gradient_semantics) {
+  (void)num_cameras;
+  if (gradient_semantics != NGT_GRAD_REFERENCE || input_shape == nullptr) return 0;
+  const int64_t cells = input_shape[1] * input_shape[2] * input_shape[3];
+  return static_cast<size_t>(cells * (input_shape[4] + 1)) * sizeof(float);
+}
+
+int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_shape, const float* output,
+                                    const float* output_grad, const float* matrix_m, const float* matrix_v,
+                                    const float* matrix_p, const float* frustum, int num_cameras,
+                                    const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
+                                    int separate_camera_batch, int gradient_semantics, float* input_grad,
+                                    void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
+  std::vector<Pair> pairs;
+  int rc = check_ray_args(input_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
+                          boundary, blending_mode, separate_camera_batch, pairs, false);
+  if (rc) return rc;
+  const RayArgs a = make_ray_args(input, input_shape, output_dhw);
+  hipStream_t s = as_stream(stream);
+  const Shape5 si = shape5(input_shape);
+  const int C = static_cast<int>(si.c);
+  fill_zero(input_grad, si.elems(), s);
+  if (gradient_semantics == NGT_GRAD_REFERENCE) {
+    const size_t need = ngt_raymarch_grid_transform_bwd_workspace_size(input_shape, num_cameras, gradient_semantics);
+    NGT_REQUIRE(workspace && workspace_bytes >= need, NGT_ERR_WORKSPACE,
+                "workspace too small for ngt_raymarch_grid_transform_bwd");
+    if (interpolation != LINEAR) return check_launch("ngt_raymarch_grid_transform_bwd");  // reference scatters LINEAR only
+    float* gbuf = static_cast<float*>(workspace);
+    float* wsum = gbuf + si.cells() * C;
+    fill_zero(gbuf, si.cells() * (C + 1), s);
+    for (const Pair& p : pairs) {
+      CamBatch cb;
+      cb.count = 1;
+      cb.cam[0] = make_cam(matrix_m + 16 * p.n, matrix_v + 16 * p.v, matrix_p + 16 * p.v, frustum + 6 * p.v, p.n, p.out);
+      const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), 1);
+      NGT_SWITCH_C(C, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_bwd_reference_kernel<CC, BB, MM>), grid, dim3(256), 0, s, a, cb, output,
+                             output_grad, gbuf, wsum))));
+      NGT_SWITCH_C(C,
+          hipLaunchKernelGGL((normalize_by_weight_kernel<CC>), dim3(blocks_for(si.cells(), 256)), dim3(256), 0, s, gbuf,
+                             wsum, input_grad + p.n * si.cells() * C, si.cells()));
+    }
+  } else {
+    for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
+      const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+      NGT_SWITCH_C(C, NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_bwd_exact_kernel<CC, FF, BB, MM, false>), grid, dim3(256), 0, s, a, cb, output,
+                             output_grad, input_grad)))));
+    });
+  }
+  return check_launch("ngt_raymarch_grid_transform_bwd");
+}
+
+int ngt_render_density_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
+                           float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
+                           const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
+                           int boundary, int blending_mode, float* output, ngt_stream_t stream) {
+  std::vector<Pair> pairs;
+  int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
+                          boundary, blending_mode, 1, pairs, true);
+  if (rc) return rc;
+  NGT_REQUIRE(light_rgb != nullptr, NGT_ERR_INVALID_ARGUMENT, "light_rgb must be given");
+  RayArgs a = make_ray_args(density, density_shape, output_dhw);
+  a.light_r = light_rgb[0]; a.light_g = light_rgb[1]; a.light_b = light_rgb[2];
+  a.depth_step = depth_step;
+  hipStream_t s = as_stream(stream);
+  for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
+    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+        hipLaunchKernelGGL((raymarch_fwd_kernel<4, FF, BB, MM, true>), grid, dim3(256), 0, s, a, cb, output))));
+  });
+  return check_launch("ngt_render_density_fwd");
+}
+
+int ngt_render_density_bwd(const float* density, const int64_t* density_shape, const float* light_rgb,
+                           float depth_step, const float* output, const float* output_grad, const float* matrix_m,
+                           const float* matrix_v, const float* matrix_p, const float* frustum, int num_cameras,
+                           const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
+                           float* density_grad, ngt_stream_t stream) {
+  std::vector<Pair> pairs;
+  int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
+                          boundary, blending_mode, 1, pairs, true);
+  if (rc) return rc;
+  NGT_REQUIRE(light_rgb != nullptr, NGT_ERR_INVALID_ARGUMENT, "light_rgb must be given");
+  RayArgs a = make_ray_args(density, density_shape, output_dhw);
+  a.light_r = light_rgb[0]; a.light_g = light_rgb[1]; a.light_b = light_rgb[2];
+  a.depth_step = depth_step;
+  hipStream_t s = as_stream(stream);
+  fill_zero(density_grad, shape5(density_shape).elems(), s);
+  for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
+    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+        hipLaunchKernelGGL((raymarch_bwd_exact_kernel<4, FF, BB, MM, true>), grid, dim3(256), 0, s, a, cb, output,
+                           output_grad, density_grad))));
+  });
+  return check_launch("ngt_render_density_bwd");
+}
+
+}  // extern "C"
