@@ -1,0 +1,299 @@
+#!/usr/bin/env python3
+"""Benchmark: differentiable single step at 128^3 on one MI355X.
+
+Workload (one "step"): density 128^3 fp32 (16 seeded Gaussian blobs) and a
+MAC-staggered curl-noise velocity (129x128x128 / 128x129x128 / 128x128x129,
+max |v| = 2 voxels/step); one MacCormack step (dt=1, trilinear, MC_SMOOTH
+revert clamp, BORDER) followed by the 5-view render (ring of 5 pinhole
+cameras, 256x144 pixels, 128 depth slices, BEER_LAMBERT, constant light);
+L2 loss against target images rendered from a perturbed density; full
+backward to the density and the three face arrays.  Inputs are resident in
+HBM before timing starts.
+
+Prints one JSON line.  value = voxel-updates/s (128^3 cells advected per
+fwd+bwd step / time); ray_samples_per_s = 5*256*144*128 / time.  vs_plain =
+speed-up over the per-op "plain" path (the reference's composition: an explicit
+4-channel LUT, four SampleGridLut launches and elementwise ops for the warp,
+a materialised light grid, channel concatenation and one raymarch launch per
+camera for the render).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "neural-global-transport_amd"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+GRID = (128, 128, 128)
+IMAGE_HW = (144, 256)
+DEPTH = 128
+VIEWS = 5
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--graph", action="store_true", help="replay the fused step as a captured HIP graph")
+    return p.parse_args()
+
+
+def build_problem(torch, dev):
+    from ngt import synth, ops
+    from ngt.lighting import Light
+    rho = synth.blobs(GRID, 16, seed=2)
+    target_rho = synth.blobs(GRID, 16, seed=3)
+    vx, vy, vz = synth.curl_noise_staggered(GRID, seed=4, vmax=2.0)
+    grid_t, cams = synth.ring_setup(GRID, IMAGE_HW, DEPTH, VIEWS)
+    from ngt.renderer import Renderer
+    r = Renderer(filter_mode="LINEAR", boundary_mode="BORDER", mipmapping="NONE", blend_mode="BEER_LAMBERT",
+                 fused=True)
+    M, V, P, F = r._transform_params([grid_t], cams)
+    light = Light(color=[1.0, 1.0, 1.0], intensity=1.0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    prob = dict(rho=t(rho[None]), vx=t(vx[None]), vy=t(vy[None]), vz=t(vz[None]), M=M, V=V, P=P, F=F,
+                ds=float(cams[0].depth_step), light=tuple(light.rgb()), cams=cams, grid_t=grid_t, renderer=r)
+    with torch.no_grad():
+        timg = ops.render_density_fused(t(target_rho[None]), prob["light"], prob["ds"], M, V, P, F,
+                                        (DEPTH,) + IMAGE_HW)
+    prob["target"] = timg[..., :3].contiguous()
+    return prob
+
+
+def fused_step(torch, ops, prob, leaves):
+    rho, vx, vy, vz = leaves
+    adv = ops.advect_grid_semi_langrange(rho, (vx, vy, vz), timestep=-1.0, order=2, clamp_extrema="MC_SMOOTH",
+                                         boundary="BORDER", separate_velocity_batch=False)[:, 0]
+    img = ops.render_density_fused(adv, prob["light"], prob["ds"], prob["M"], prob["V"], prob["P"], prob["F"],
+                                   (DEPTH,) + IMAGE_HW)
+    loss = ((img[..., :3] - prob["target"]) ** 2).sum()
+    loss.backward()
+    return loss
+
+
+def plain_step(torch, ops, prob, leaves):
+    """The reference's composition on the same kernels' per-op entry points."""
+    rho, vx, vy, vz = leaves
+    r = prob["renderer"]
+    from ngt.data_structures import VelocityGrid
+    vg = VelocityGrid(list(GRID), x=vx, y=vy, z=vz, warp_renderer=r)
+    adv = vg.warp(rho, order=2, dt=1.0, clamp="MC_SMOOTH", fused=False)
+    light = adv * torch.tensor(prob["light"], dtype=torch.float32, device=adv.device)
+    data = torch.cat([light, adv], dim=-1) * prob["ds"]
+    imgs = []
+    for v in range(VIEWS):
+        imgs.append(ops.raymarch_grid_transform(data, prob["M"], prob["V"][v:v + 1], prob["P"][v:v + 1],
+                                                prob["F"][v:v + 1], (DEPTH,) + IMAGE_HW))
+    img = torch.cat(imgs, dim=1)
+    loss = ((img[..., :3] - prob["target"]) ** 2).sum()
+    loss.backward()
+    return loss
+
+
+def make_leaves(torch, prob):
+    return [prob[k].clone().requires_grad_() for k in ("rho", "vx", "vy", "vz")]
+
+
+def time_steps(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps  # ms
+
+
+def time_render_bwd_kernel(torch, prob, adv, img, g, iters):
+    """Average duration of ngt_render_density_bwd (the dominant launch) with
+    HIP events on the stream it is launched on.  The call is one zero-fill of
+    the 8 MiB gradient plus the backward raymarch kernel."""
+    from ngt import _lib, ops
+    lib = _lib.load()
+    cams = ops._Cams(prob["M"], prob["V"], prob["P"], prob["F"])
+    lp, _, keep = _lib.host_floats(prob["light"], 3)
+    out_g = torch.empty_like(adv)
+    s = torch.cuda.current_stream()
+
+    def call():
+        _lib.check(lib.ngt_render_density_bwd(
+            _lib.ptr(adv), _lib.shape5(adv), lp, prob["ds"], _lib.ptr(img), _lib.ptr(g), cams.m, cams.v, cams.p,
+            cams.f, cams.nv, _lib.dhw((DEPTH,) + IMAGE_HW), 1, 0, 0, _lib.ptr(out_g), ctypes.c_void_p(s.cuda_stream)))
+
+    for _ in range(3):
+        call()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(iters):
+        call()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / iters
+
+
+def cpu_baseline():
+    """FP64 NumPy oracle (single thread) on a bounded sample of the step: the
+    MacCormack step fwd+bwd at 48^3 and the fused render fwd+bwd of one view at
+    64x36 pixels (128 slices) on the 128^3 grid; scaled per voxel-update and
+    per ray-sample to the full step."""
+    from oracle import advect as A, raymarch as RM, geometry as G, BORDER, LINEAR
+    from oracle import blend as B
+    from ngt import synth
+    n = 48
+    rho = synth.blobs((n, n, n), 8, seed=1).astype(np.float64)
+    v = synth.curl_noise_centered((n, n, n), seed=5, vmax=2.0).astype(np.float64)
+    t0 = time.perf_counter()
+    out = A.warp_forward(rho, v, 1.0, 2, "MC_SMOOTH", BORDER)
+    A.warp_backward(rho, v, out, 1.0, 2, "MC_SMOOTH", BORDER)
+    t_adv = (time.perf_counter() - t0) / n ** 3
+    grid_t, cams = synth.ring_setup(GRID, (36, 64), DEPTH, 1)
+    M = G.op_matrix(grid_t.get_transform_matrix())
+    V = G.op_matrix(cams[0].view_matrix())
+    P = G.op_matrix(cams[0].projection_matrix())
+    F = cams[0].frustum().astype(np.float64)
+    dens = synth.blobs(GRID, 16, seed=2).astype(np.float64)
+    cat = np.concatenate([dens, dens, dens, dens], -1)[None] * cams[0].depth_step
+    t0 = time.perf_counter()
+    img = RM.raymarch_grid_transform(cat, [M], [V], [P], [F], (DEPTH, 36, 64), LINEAR, BORDER, B.BEER_LAMBERT)
+    RM.raymarch_grid_transform_grad(cat, img, img, [M], [V], [P], [F], (DEPTH, 36, 64), LINEAR, BORDER, B.BEER_LAMBERT)
+    t_ren = (time.perf_counter() - t0) / (36 * 64 * DEPTH)
+    t_step = t_adv * np.prod(GRID) + t_ren * VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * DEPTH
+    return {"value": float(np.prod(GRID) / t_step), "unit": "voxel-updates/s", "cores": 1, "kind": "port",
+            "sample": "FP64 NumPy oracle, 1 thread: MacCormack+MC_SMOOTH fwd+bwd at 48^3 and 1-view 64x36x128 "
+                      "fused render fwd+bwd on the 128^3 grid, per-voxel / per-ray-sample times scaled to the "
+                      "full 128^3 + 5x256x144x128 step",
+            "seconds_per_step_est": float(t_step)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from ngt import ops, _lib
+    _lib.load()
+    dev = torch.device("cuda:0")
+    print("bench: building inputs", file=sys.stderr, flush=True)
+    prob = build_problem(torch, dev)
+    leaves = make_leaves(torch, prob)
+
+    def zero():
+        for l in leaves:
+            l.grad = None
+
+    def step():
+        zero()
+        fused_step(torch, ops, prob, leaves)
+
+    print("bench: fused step", file=sys.stderr, flush=True)
+    ms = time_steps(torch, step, args.steps, args.warmup)
+    ms_graph = None
+    if args.graph:
+        g = torch.cuda.CUDAGraph()
+        static = make_leaves(torch, prob)
+
+        def gstep():
+            for l in static:
+                l.grad = None
+            fused_step(torch, ops, prob, static)
+
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            for _ in range(3):
+                gstep()
+        torch.cuda.current_stream().wait_stream(s)
+        with torch.cuda.graph(g):
+            gstep()
+        ms_graph = time_steps(torch, g.replay, args.steps, args.warmup)
+
+    ms_plain = None
+    if not args.no_plain:
+        print("bench: plain step", file=sys.stderr, flush=True)
+        pleaves = make_leaves(torch, prob)
+
+        def pstep():
+            for l in pleaves:
+                l.grad = None
+            plain_step(torch, ops, prob, pleaves)
+
+        ms_plain = time_steps(torch, pstep, max(3, args.steps // 4), 2)
+
+    # dominant kernel: backward raymarch of the fused renderer
+    with torch.no_grad():
+        adv = ops.advect_grid_semi_langrange(prob["rho"], (prob["vx"], prob["vy"], prob["vz"]), -1.0, 2,
+                                             "MC_SMOOTH", "BORDER", False)[:, 0].contiguous()
+        img = ops.render_density_fused(adv, prob["light"], prob["ds"], prob["M"], prob["V"], prob["P"], prob["F"],
+                                       (DEPTH,) + IMAGE_HW)
+        gimg = torch.zeros_like(img)
+        gimg[..., :3] = 2.0 * (img[..., :3] - prob["target"])
+    ms_kernel = time_render_bwd_kernel(torch, prob, adv, img, gimg, max(5, args.steps))
+    cells = int(np.prod(GRID))
+    pixels = VIEWS * IMAGE_HW[0] * IMAGE_HW[1]
+    # algorithmic bytes of the render backward: img + g_img (16 B each per pixel)
+    # + density read + density gradient write (4 B each per voxel)
+    kbytes = pixels * 32 + cells * 8
+    achieved = kbytes / (ms_kernel * 1e-3) / 1e9
+    # whole-step compulsory bytes: advect fwd 20 + bwd 36 per voxel, render fwd
+    # 4/voxel + 16/pixel, render bwd 8/voxel + 32/pixel
+    step_bytes = cells * (20 + 36 + 4 + 8) + pixels * (16 + 32)
+    best_ms = ms_graph if ms_graph is not None else ms
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "render_density_bwd_128_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    rec = {
+        "metric": "fwd+bwd voxel-updates/s + ray-samples/s at 128^3 (1 GPU), % HBM peak, vs plain",
+        "value": cells / (best_ms * 1e-3),
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": best_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded blobs + curl noise)",
+        "config": {"workload": "MacCormack step (MC_SMOOTH, BORDER) of 128^3 density with MAC curl-noise velocity "
+                               "+ 5-view 256x144x128 fused render, L2 loss, full backward",
+                   "grid": list(GRID), "views": VIEWS, "image_hw": list(IMAGE_HW), "depth_slices": DEPTH,
+                   "graph": bool(args.graph)},
+        "ray_samples_per_s": pixels * DEPTH / (best_ms * 1e-3),
+        "hbm_frac_step": step_bytes / (best_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "ms_per_step_eager": ms,
+        "ms_per_step_graph": ms_graph,
+        "ms_per_step_plain": ms_plain,
+        "vs_plain": (ms_plain / best_ms) if ms_plain else None,
+        "roofline": {"kernel": "ngt_render_density_bwd (raymarch_bwd_exact_kernel<4,LINEAR,BORDER,BEER_LAMBERT,light>)",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "ms_per_launch": ms_kernel,
+                     "algorithmic_bytes_per_launch": kbytes},
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        rec["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,257 @@
+"""Grid data model for the transport path: DensityGrid, VelocityGrid, State.
+
+Mirror of the warp / render parts of phitest/render/data_structures.py:
+``VelocityGrid.component_shapes`` (:765-773), ``centered`` (:1634-1653),
+``centered_lut_grid`` (:1684-1688), ``warp`` (:1690-1737),
+``_sampled_to_component_shape`` (:1655-1682), ``_warp_vel_component`` /
+``warped`` / ``copy_warped`` (:1081-1177), ``DensityGrid.warped`` (:614-617),
+``State.render_density`` / ``density_advected`` / ``velocity_advected``
+(:2260-2330).  Tensors are torch NDHWC on the HIP device.
+
+``VelocityGrid.warp`` runs the fused MacCormack kernels (one SL kernel + one
+correction/clamp kernel, velocity centred on the fly from the MAC faces).
+``fused=False`` reproduces the reference's composition instead: 2-4
+SampleGridLut launches on an explicit 4-channel LUT plus elementwise torch
+ops - the per-op "plain" path.
+"""
+import copy
+
+import numpy as np
+import torch
+
+from . import ops
+from .renderer import Renderer
+
+
+def _default_warp_renderer(boundary="BORDER"):
+    return Renderer(filter_mode="LINEAR", boundary_mode=boundary, mipmapping="NONE", blend_mode="ADDITIVE",
+                    sample_gradients=False)
+
+
+class DensityGrid:
+    def __init__(self, shape, d=None, scale_renderer=None, hull=None, inflow=None, device=None, **kwargs):
+        self.shape = [int(v) for v in shape]
+        if d is None:
+            d = torch.zeros([1] + self.shape + [1], dtype=torch.float32, device=device or "cuda")
+        self._d = d
+        self.hull = hull
+        self.inflow = inflow
+        self.scale_renderer = scale_renderer
+
+    @property
+    def d(self):
+        return self._d
+
+    def with_inflow(self):
+        return self._d if self.inflow is None else self._d + self.inflow
+
+    def assign(self, d):
+        self._d = d
+
+    def warped(self, vel_grid, order=1, dt=1.0, clamp="NONE"):
+        if not isinstance(vel_grid, VelocityGrid):
+            raise ValueError("Invalid velocity grid")
+        return vel_grid.warp(self.with_inflow(), order=order, dt=dt, clamp=clamp)
+
+    def copy_warped(self, vel_grid, order=1, dt=1.0, clamp="NONE", **kwargs):
+        return DensityGrid(self.shape, d=self.warped(vel_grid, order, dt, clamp), scale_renderer=self.scale_renderer)
+
+
+class VelocityGrid:
+    @staticmethod
+    def component_shapes(centered_shape):
+        assert len(centered_shape) == 3
+        x_shape = list(centered_shape)
+        x_shape[2] += 1
+        y_shape = list(centered_shape)
+        y_shape[1] += 1
+        z_shape = list(centered_shape)
+        z_shape[0] += 1
+        return x_shape, y_shape, z_shape
+
+    def __init__(self, centered_shape, std=0.1, x=None, y=None, z=None, boundary=None, scale_renderer=None,
+                 warp_renderer=None, device=None, **kwargs):
+        self.centered_shape = [int(v) for v in centered_shape]
+        self.x_shape, self.y_shape, self.z_shape = self.component_shapes(self.centered_shape)
+        dev = device or "cuda"
+        self.x = x if x is not None else torch.randn([1] + self.x_shape + [1], device=dev) * std
+        self.y = y if y is not None else torch.randn([1] + self.y_shape + [1], device=dev) * std
+        self.z = z if z is not None else torch.randn([1] + self.z_shape + [1], device=dev) * std
+        self.boundary = boundary
+        self.scale_renderer = scale_renderer
+        self.warp_renderer = warp_renderer if warp_renderer is not None else _default_warp_renderer()
+
+    def assign(self, x, y, z):
+        self.x, self.y, self.z = x, y, z
+
+    def centered(self, pad_lod=False, concat=True):
+        h = 0.5
+        c = [(self.x[:, :, :, 1:] + self.x[:, :, :, :-1]) * h,
+             (self.y[:, :, 1:] + self.y[:, :, :-1]) * h,
+             (self.z[:, 1:] + self.z[:, :-1]) * h]
+        if pad_lod:
+            c.append(torch.zeros_like(c[0]))
+        return torch.cat(c, dim=-1) if concat else c
+
+    def centered_lut_grid(self, dt=1.0, centered_velocity=None):
+        vel_centered = self.centered() if centered_velocity is None else centered_velocity
+        return vel_centered * (-dt)
+
+    def _can_fuse(self):
+        return self.warp_renderer.filter_mode == "LINEAR" and self.warp_renderer.mip_mode in ("NONE", None)
+
+    def warp(self, data, order=1, dt=1.0, clamp="NONE", centered_velocity=None, fused=True):
+        """Advect ``data`` [N,D,H,W,C] with this velocity (semi-Lagrangian, or
+        MacCormack for order 2, with the MC_SMOOTH revert clamp)."""
+        clamp = clamp.upper()
+        if list(data.shape[1:4]) != self.centered_shape:
+            raise ValueError("Shape mismatch in centered warp: data {}, velocity {}".format(
+                list(data.shape[1:4]), self.centered_shape))
+        if order not in (1, 2):
+            raise ValueError("Unsupported warp order '{}'".format(order))
+        if clamp == "MC":
+            raise NotImplementedError("MIM and MAX warp sampling have wrong gradients.")
+        if fused and self._can_fuse():
+            vel = centered_velocity if centered_velocity is not None else (self.x, self.y, self.z)
+            out = ops.advect_grid_semi_langrange(data, vel, timestep=-dt, order=order,
+                                                 clamp_extrema=clamp if clamp in ("MC_SMOOTH", "NEGATIVE") else "NONE",
+                                                 boundary=self.warp_renderer.boundary_mode,
+                                                 separate_velocity_batch=False)
+            return out[:, 0]
+        return self._warp_plain(data, order, dt, clamp, centered_velocity)
+
+    def _warp_plain(self, data, order, dt, clamp, centered_velocity=None):
+        r = self.warp_renderer
+        v = self.centered_lut_grid(dt=dt, centered_velocity=centered_velocity)
+        warped = r._sample_LuT(data, v, True, relative=True)
+        if order == 2:
+            back = r._sample_LuT(warped, -v, True, relative=True)
+            corr = warped + 0.5 * (data - back)
+            if clamp == "MC_SMOOTH":
+                fm = r.filter_mode
+                r.filter_mode = "MIN"
+                dmin = r._sample_LuT(data, v, True, relative=True)
+                r.filter_mode = "MAX"
+                dmax = r._sample_LuT(data, v, True, relative=True)
+                r.filter_mode = fm
+                oob = torch.logical_or(corr < dmin, corr > dmax)
+                corr = torch.where(oob, warped, corr)
+            warped = corr
+        if clamp == "NEGATIVE":
+            warped = torch.clamp(warped, min=0)
+        return warped
+
+    # --- velocity self-advection --------------------------------------------------
+
+    def _sampled_to_component_shape(self, component, pad_lod=False, concat=True):
+        component = component.upper()
+        off_from, off_to = 0.5, -0.5
+        r = self.warp_renderer
+        out = []
+        out.append(self.x if component == "X" else r.resample_grid3D_offset(
+            self.x, [[off_from, off_to, 0.0] if component == "Y" else [off_from, 0.0, off_to]],
+            self.y_shape if component == "Y" else self.z_shape).squeeze(1))
+        out.append(self.y if component == "Y" else r.resample_grid3D_offset(
+            self.y, [[off_to, off_from, 0.0] if component == "X" else [0.0, off_from, off_to]],
+            self.x_shape if component == "X" else self.z_shape).squeeze(1))
+        out.append(self.z if component == "Z" else r.resample_grid3D_offset(
+            self.z, [[off_to, 0.0, off_from] if component == "X" else [0.0, off_to, off_from]],
+            self.x_shape if component == "X" else self.y_shape).squeeze(1))
+        if pad_lod:
+            out.append(torch.zeros_like(out[0]))
+        return torch.cat(out, dim=-1) if concat else out
+
+    def _warp_vel_component(self, data, lut, order=1, dt=1.0, clamp="NONE"):
+        """Warp one MAC component with a component-shaped relative LUT."""
+        if order < 1 or order > 2:
+            raise ValueError("Unsupported warp order '{}'".format(order))
+        clamp = clamp.upper()
+        if clamp == "MC":
+            raise NotImplementedError("MIM and MAX warp sampling have wrong gradients.")
+        if self._can_fuse():
+            # sample position = index + 1 * lut, exactly the relative LUT read
+            out = ops.advect_grid_semi_langrange(data, lut.contiguous(), timestep=1.0, order=order,
+                                                 clamp_extrema=clamp if clamp in ("MC_SMOOTH", "NEGATIVE") else "NONE",
+                                                 boundary=self.warp_renderer.boundary_mode,
+                                                 separate_velocity_batch=False)
+            return out[:, 0]
+        r = self.warp_renderer
+        warped = r._sample_LuT(data, lut, True, relative=True)
+        if order == 2:
+            back = r._sample_LuT(warped, -lut, True, relative=True)
+            corr = warped + 0.5 * (data - back)
+            if clamp == "MC_SMOOTH":
+                fm = r.filter_mode
+                r.filter_mode = "MIN"
+                dmin = r._sample_LuT(data, lut, True, relative=True)
+                r.filter_mode = "MAX"
+                dmax = r._sample_LuT(data, lut, True, relative=True)
+                r.filter_mode = fm
+                corr = torch.where(torch.logical_or(corr < dmin, corr > dmax), warped, corr)
+            warped = corr
+        return warped
+
+    def warped(self, vel_grid=None, order=1, dt=1.0, clamp="NONE"):
+        """Self-advect the three MAC components (vel_grid None) or advect them
+        with another velocity grid of the same centered shape."""
+        res = []
+        for comp, data in (("X", self.x), ("Y", self.y), ("Z", self.z)):
+            if vel_grid is None:
+                lut = self._sampled_to_component_shape(comp) * (-dt)
+            else:
+                lut = vel_grid._sampled_to_component_shape_of(self, comp) * (-dt)
+            res.append(self._warp_vel_component(data, lut, order=order, dt=dt, clamp=clamp))
+        return tuple(res)
+
+    def _sampled_to_component_shape_of(self, other, comp):
+        if other.centered_shape != self.centered_shape:
+            raise NotImplementedError("advecting a velocity grid of another resolution")
+        return self._sampled_to_component_shape(comp)
+
+    def copy_warped(self, vel_grid=None, order=1, dt=1.0, clamp="NONE", **kwargs):
+        x, y, z = self.warped(vel_grid, order, dt, clamp=clamp)
+        return VelocityGrid(self.centered_shape, x=x, y=y, z=z, boundary=self.boundary,
+                            scale_renderer=self.scale_renderer, warp_renderer=self.warp_renderer)
+
+
+class RenderingContext:
+    """Cameras + lights + renderer (renderer.py:35-59, the fields the render
+    path uses)."""
+
+    def __init__(self, cameras, lights, dens_renderer, vel_renderer=None, monochrome=False, render_SDF=False):
+        self.cameras = cameras
+        self.lights = lights
+        self.dens_renderer = dens_renderer
+        self.vel_renderer = vel_renderer
+        self.monochrome = monochrome
+        self.render_SDF = render_SDF
+
+
+class State:
+    def __init__(self, density, velocity, frame=0, transform=None, target_cameras=None, **kwargs):
+        self.density = density
+        self.velocity = velocity
+        self.frame = frame
+        self.transform = transform
+        self.target_cameras = target_cameras
+        self.images = None
+        self.t = None
+
+    def get_density_transform(self):
+        return self.transform.copy_new_data(self.density.d)
+
+    def render_density(self, render_ctx, custom_ops=None, **kwargs):
+        cams = self.target_cameras if self.target_cameras is not None else render_ctx.cameras
+        imgs = torch.stack(render_ctx.dens_renderer.render_density(
+            self.get_density_transform(), light_list=render_ctx.lights, camera_list=cams, cut_alpha=False,
+            monochrome=render_ctx.monochrome, custom_ops=custom_ops), dim=1)
+        imgs, d = imgs[..., :3], imgs[..., 3:]
+        self.images = imgs
+        self.t = torch.exp(-d)
+        return imgs, self.t
+
+    def density_advected(self, dt=1.0, order=1, clamp="NONE"):
+        return self.density.warped(self.velocity, order=order, dt=dt, clamp=clamp)
+
+    def velocity_advected(self, dt=1.0, order=1, clamp="NONE"):
+        return self.velocity.copy_warped(order=order, dt=dt, clamp=clamp)

@@ -118,8 +118,13 @@ def test_lut_autograd_matches_grad_op(ngt_ops):
 
 # --- SampleGridTransform ----------------------------------------------------------
 
-def ring_args(dims, out_dhw, views=3, dist=3.0):
-    M = G.grid_transform(dims, center=True, scale_=[1.0 / dims[2]] * 3)
+def ring_args(dims, out_dhw, views=3, dist=3.0, generic=False):
+    """Ring cameras around a centred unit grid.  generic=True shifts the grid
+    by a fraction of a cell so no sample lands exactly on a cell centre (the
+    count-normalised reference gradient is discontinuous there: fp32 and fp64
+    positions can fall on different sides of an integer)."""
+    t = (0.0123, -0.0217, 0.0311) if generic else (0, 0, 0)
+    M = G.grid_transform(dims, translation_=t, center=True, scale_=[1.0 / dims[2]] * 3)
     cams = G.look_at_ring_cameras(views, dist, 40.0, out_dhw[2] / out_dhw[1], dist - 1.0, dist + 1.0)
     Ms = [G.op_matrix(M).astype(np.float32)]
     Vs = [G.op_matrix(c[0]).astype(np.float32) for c in cams]
@@ -153,7 +158,7 @@ def test_transform_forward(ngt_ops, mode, C, bound):
 def test_transform_backward(ngt_ops, semantics):
     rng = np.random.default_rng(4)
     dims = (9, 9, 9)
-    Ms, Vs, Ps, Fs = ring_args(dims, (7, 6, 8))
+    Ms, Vs, Ps, Fs = ring_args(dims, (7, 6, 8), generic=True)
     inp = f32(rng.standard_normal((1,) + dims + (2,)))
     g = f32(rng.standard_normal((1, 3, 7, 6, 8, 2)))
     exp = R.sample_grid_transform_grad(inp, g, Ms, Vs, Ps, Fs, semantics=semantics)
@@ -212,7 +217,7 @@ def test_raymarch_reference_gradient(ngt_ops, mode, bound):
     rng = np.random.default_rng(600 + mode + bound)
     dims = (10, 10, 10)
     out_dhw = (12, 8, 9)
-    Ms, Vs, Ps, Fs = ring_args(dims, out_dhw, views=2)
+    Ms, Vs, Ps, Fs = ring_args(dims, out_dhw, views=2, generic=True)
     x = f32(rng.uniform(0.0, 0.25, (1,) + dims + (4,)))
     exp = RM.raymarch_grid_transform(x, Ms, Vs, Ps, Fs, out_dhw, LINEAR, bound, mode)
     g = f32(rng.standard_normal(exp.shape))
