@@ -118,22 +118,10 @@ def time_steps(torch, fn, steps, warmup):
     return start.elapsed_time(end) / steps  # ms
 
 
-def time_render_bwd_kernel(torch, prob, adv, img, g, iters):
-    """Average duration of ngt_render_density_bwd (the dominant launch) with
-    HIP events on the stream it is launched on.  The call is one zero-fill of
-    the 8 MiB gradient plus the backward raymarch kernel."""
-    from ngt import _lib, ops
-    lib = _lib.load()
-    cams = ops._Cams(prob["M"], prob["V"], prob["P"], prob["F"])
-    lp, _, keep = _lib.host_floats(prob["light"], 3)
-    out_g = torch.empty_like(adv)
+def time_call(torch, call, iters):
+    """Average duration of one library call (HIP events on the stream the
+    kernels are launched on)."""
     s = torch.cuda.current_stream()
-
-    def call():
-        _lib.check(lib.ngt_render_density_bwd(
-            _lib.ptr(adv), _lib.shape5(adv), lp, prob["ds"], _lib.ptr(img), _lib.ptr(g), cams.m, cams.v, cams.p,
-            cams.f, cams.nv, _lib.dhw((DEPTH,) + IMAGE_HW), 1, 0, 0, _lib.ptr(out_g), ctypes.c_void_p(s.cuda_stream)))
-
     for _ in range(3):
         call()
     torch.cuda.synchronize()
@@ -145,6 +133,43 @@ def time_render_bwd_kernel(torch, prob, adv, img, g, iters):
     e1.record(s)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / iters
+
+
+def op_breakdown(torch, prob, iters):
+    """Per-op call durations of the fused step (direct C-ABI calls)."""
+    from ngt import _lib, ops
+    lib = _lib.load()
+    st = lambda: ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    cams = ops._Cams(prob["M"], prob["V"], prob["P"], prob["F"])
+    lp, _, keep = _lib.host_floats(prob["light"], 3)
+    out_dhw = _lib.dhw((DEPTH,) + IMAGE_HW)
+    rho, vx, vy, vz = prob["rho"], prob["vx"], prob["vy"], prob["vz"]
+    s_rho = _lib.shape5(rho)
+    adv = torch.empty((1, 1) + tuple(rho.shape[1:]), device=rho.device)
+    sl = torch.empty_like(adv)
+    g_in, gx, gy, gz = (torch.empty_like(t) for t in (rho, vx, vy, vz))
+    wsb = lib.ngt_advect_workspace_size(s_rho, 1, 0, 2, 1)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=rho.device)
+    img = torch.empty((1, VIEWS) + IMAGE_HW + (4,), device=rho.device)
+    dimg = torch.randn_like(img)
+    g_adv = torch.empty_like(rho)
+    rws_b = lib.ngt_render_density_bwd_workspace_size(s_rho, cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0)
+    rws = torch.empty(max(rws_b, 1), dtype=torch.uint8, device=rho.device)
+    P = _lib.ptr
+    calls = {
+        "advect_fwd": lambda: _lib.check(lib.ngt_advect_fwd(
+            P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(adv), P(sl), None, 0, st())),
+        "advect_bwd": lambda: _lib.check(lib.ngt_advect_bwd(
+            P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(dimg), P(sl), 0, P(g_in), P(gx), P(gy),
+            P(gz), P(ws), wsb, st())),
+        "render_fwd": lambda: _lib.check(lib.ngt_render_density_fwd(
+            P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(img), st())),
+        "render_bwd": lambda: _lib.check(lib.ngt_render_density_bwd(
+            P(rho), s_rho, lp, prob["ds"], P(img), P(dimg), cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0,
+            0, P(g_adv), P(rws), rws_b, st())),
+    }
+    calls["render_fwd"]()
+    return {k: time_call(torch, c, iters) for k, c in calls.items()}
 
 
 def cpu_baseline():
@@ -237,27 +262,24 @@ def main():
 
         ms_plain = time_steps(torch, pstep, max(3, args.steps // 4), 2)
 
-    # dominant kernel: backward raymarch of the fused renderer
-    with torch.no_grad():
-        adv = ops.advect_grid_semi_langrange(prob["rho"], (prob["vx"], prob["vy"], prob["vz"]), -1.0, 2,
-                                             "MC_SMOOTH", "BORDER", False)[:, 0].contiguous()
-        img = ops.render_density_fused(adv, prob["light"], prob["ds"], prob["M"], prob["V"], prob["P"], prob["F"],
-                                       (DEPTH,) + IMAGE_HW)
-        gimg = torch.zeros_like(img)
-        gimg[..., :3] = 2.0 * (img[..., :3] - prob["target"])
-    ms_kernel = time_render_bwd_kernel(torch, prob, adv, img, gimg, max(5, args.steps))
+    ops_ms = op_breakdown(torch, prob, max(5, args.steps))
     cells = int(np.prod(GRID))
     pixels = VIEWS * IMAGE_HW[0] * IMAGE_HW[1]
-    # algorithmic bytes of the render backward: img + g_img (16 B each per pixel)
-    # + density read + density gradient write (4 B each per voxel)
-    kbytes = pixels * 32 + cells * 8
+    # algorithmic bytes per call (compulsory interface traffic):
+    #   advect fwd 20 B/voxel, advect bwd 36 B/voxel (staggered faces ~12 B/cell),
+    #   render fwd 4 B/voxel + 16 B/pixel, render bwd 8 B/voxel + 32 B/pixel
+    algo = {"advect_fwd": cells * 20, "advect_bwd": cells * 36, "render_fwd": cells * 4 + pixels * 16,
+            "render_bwd": cells * 8 + pixels * 32}
+    dominant = max(ops_ms, key=ops_ms.get)
+    ms_kernel = ops_ms[dominant]
+    kbytes = algo[dominant]
     achieved = kbytes / (ms_kernel * 1e-3) / 1e9
     # whole-step compulsory bytes: advect fwd 20 + bwd 36 per voxel, render fwd
     # 4/voxel + 16/pixel, render bwd 8/voxel + 32/pixel
     step_bytes = cells * (20 + 36 + 4 + 8) + pixels * (16 + 32)
     best_ms = ms_graph if ms_graph is not None else ms
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "render_density_bwd_128_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "%s_128_traffic.json" % dominant)
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -284,10 +306,12 @@ def main():
         "ms_per_step_graph": ms_graph,
         "ms_per_step_plain": ms_plain,
         "vs_plain": (ms_plain / best_ms) if ms_plain else None,
-        "roofline": {"kernel": "ngt_render_density_bwd (raymarch_bwd_exact_kernel<4,LINEAR,BORDER,BEER_LAMBERT,light>)",
+        "roofline": {"kernel": "ngt_" + dominant.replace("render", "render_density").replace("advect_", "advect_"),
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "ms_per_launch": ms_kernel,
                      "algorithmic_bytes_per_launch": kbytes},
+        "op_ms": ops_ms,
+        "op_hbm_frac": {k: algo[k] / (ops_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS for k in ops_ms},
     }
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)

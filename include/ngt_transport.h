@@ -138,8 +138,15 @@ int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_sha
                                     int num_cameras, const int32_t* output_dhw, int interpolation, int boundary,
                                     int blending_mode, int separate_camera_batch, float* output,
                                     ngt_stream_t stream);
-size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape, int num_cameras,
-                                                      int gradient_semantics);
+/* Exact gradients with BORDER + LINEAR run atomics-free (a per-sample
+ * gradient pass, then one gather per voxel); they need this much workspace
+ * (0 otherwise).  With less workspace the call falls back to the fp32-atomic
+ * scatter.  Reference semantics need C+1 floats per voxel. */
+size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape, const float* matrix_m,
+                                                      const float* matrix_v, const float* matrix_p,
+                                                      const float* frustum, int num_cameras,
+                                                      const int32_t* output_dhw, int interpolation, int boundary,
+                                                      int separate_camera_batch, int gradient_semantics);
 int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_shape, const float* output,
                                     const float* output_grad, const float* matrix_m, const float* matrix_v,
                                     const float* matrix_p, const float* frustum, int num_cameras,
@@ -161,11 +168,15 @@ int ngt_render_density_fwd(const float* density, const int64_t* density_shape, c
                            float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
                            const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
                            int boundary, int blending_mode, float* output, ngt_stream_t stream);
+size_t ngt_render_density_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
+                                             const float* matrix_v, const float* matrix_p, const float* frustum,
+                                             int num_cameras, const int32_t* output_dhw, int interpolation,
+                                             int boundary);
 int ngt_render_density_bwd(const float* density, const int64_t* density_shape, const float* light_rgb,
                            float depth_step, const float* output, const float* output_grad, const float* matrix_m,
                            const float* matrix_v, const float* matrix_p, const float* frustum, int num_cameras,
                            const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
-                           float* density_grad, ngt_stream_t stream);
+                           float* density_grad, void* workspace, size_t workspace_bytes, ngt_stream_t stream);
 
 /* ---- AdvectGridSemiLangrange (advect.cc:21-71) with VelocityGrid.warp
  * semantics (data_structures.py:1690-1737) --------------------------------

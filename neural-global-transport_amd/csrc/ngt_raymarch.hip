@@ -22,6 +22,9 @@
 // weights accumulated and the gradient divided by them per camera.
 #include "ngt_host.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 namespace ngt {
 namespace {
 
@@ -146,24 +149,13 @@ __global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch c
   vstore<C>(out + ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C, acc);
 }
 
-template <int C, int FM, int BM, int MODE, bool LIGHT>
-__global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamBatch cb, const float* __restrict__ out,
-                                                                 const float* __restrict__ out_grad,
-                                                                 float* __restrict__ in_grad) {
-  int x, y;
-  pixel_of_thread(x, y);
-  if (x >= a.out_w || y >= a.out_h) return;
-  const CamParams& cp = cb.cam[blockIdx.z];
-  constexpr int CIN = LIGHT ? 1 : C;
-  const int64_t slice = static_cast<int64_t>(a.din.x) * a.din.y * a.din.z * CIN;
-  const float* src = a.input + cp.data_index * slice;
-  float* gdst = in_grad + cp.data_index * slice;
-  const Ray r = make_ray<BM>(a, cp, x, y);
-  if (r.steps <= 0) return;
-  const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
-  Vec<C> g = vload<C>(out_grad + pix);
+// Per-sample cell gradients of one ray (exact adjoint of the forward walk),
+// handed to ``sink(k, qx, qy, qz, gc)``.
+template <int C, int FM, int BM, int MODE, bool LIGHT, typename Sink>
+NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ src, const Ray& r, Vec<C> g,
+                                  const float* __restrict__ out_pix, Sink& sink) {
   if constexpr (C > 1 && MODE == ALPHA) {
-    Vec<C> acc = vload<C>(out + pix);
+    Vec<C> acc = vload<C>(out_pix);
     for (int k = r.steps - 1; k >= 0; --k) {
       const float kf = static_cast<float>(k);
       const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
@@ -179,12 +171,12 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
       gc.v[C - 1] = t * g.v[C - 1];
       g.v[C - 1] = (1.f - cell.v[C - 1]) * g.v[C - 1] - lg;
       acc.v[C - 1] = aprev;
-      scatter_cell<C, FM, BM, LIGHT>(a, src, gdst, qx, qy, qz, gc);
+      sink(k, qx, qy, qz, gc);
     }
   } else {
     float total = 0.f;
     if constexpr (C > 1 && MODE == BEER_LAMBERT) {
-      const Vec<C> L = vload<C>(out + pix);
+      const Vec<C> L = vload<C>(out_pix);
 #pragma unroll
       for (int i = 0; i < C - 1; ++i) total += L.v[i] * g.v[i];
     }
@@ -214,9 +206,205 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
           gc.v[C - 1] = g.v[C - 1] - lg;
         }
       }
-      scatter_cell<C, FM, BM, LIGHT>(a, src, gdst, qx, qy, qz, gc);
+      sink(k, qx, qy, qz, gc);
     }
   }
+}
+
+// Cell gradient -> gradient of the stored channels (1 for the fused density
+// renderer: d cell / d rho = ds * (rgb, 1)).
+template <int C, bool LIGHT>
+struct GradChannels {
+  static constexpr int value = LIGHT ? 1 : C;
+};
+
+template <int C, bool LIGHT>
+NGT_DEVICE Vec<GradChannels<C, LIGHT>::value> fold_cell_grad(const RayArgs& a, const Vec<C>& gc) {
+  if constexpr (LIGHT) {
+    Vec<1> r;
+    r.v[0] = a.depth_step * (a.light_r * gc.v[0] + a.light_g * gc.v[1] + a.light_b * gc.v[2] + gc.v[3]);
+    return r;
+  } else {
+    return gc;
+  }
+}
+
+// Atomic path: scatter each sample's gradient into the volume.
+template <int C, int FM, int BM, int MODE, bool LIGHT>
+__global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamBatch cb, const float* __restrict__ out,
+                                                                 const float* __restrict__ out_grad,
+                                                                 float* __restrict__ in_grad) {
+  int x, y;
+  pixel_of_thread(x, y);
+  if (x >= a.out_w || y >= a.out_h) return;
+  const CamParams& cp = cb.cam[blockIdx.z];
+  constexpr int CIN = LIGHT ? 1 : C;
+  const int64_t slice = static_cast<int64_t>(a.din.x) * a.din.y * a.din.z * CIN;
+  const float* src = a.input + cp.data_index * slice;
+  float* gdst = in_grad + cp.data_index * slice;
+  const Ray r = make_ray<BM>(a, cp, x, y);
+  if (r.steps <= 0) return;
+  const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
+  auto sink = [&](int, float qx, float qy, float qz, const Vec<C>& gc) {
+    scatter_filtered<CIN, FM, BM>(src, gdst, a.din, qx, qy, qz, fold_cell_grad<C, LIGHT>(a, gc));
+  };
+  walk_ray_gradient<C, FM, BM, MODE, LIGHT>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
+}
+
+// ---------------------------------------------------------------------------
+// Gather-form adjoint (BORDER, LINEAR).  Pass 1 stores every sample's
+// gradient G[pair][k][pixel] and the ray parameters; pass 2 runs one
+// work-item per voxel that collects, from the rays whose pixel centres fall in
+// the projection of the voxel's trilinear support, exactly the samples whose
+// corners include the voxel - the same terms the scatter would add, with no
+// atomics and a single write per voxel.
+
+struct RayRec {
+  float sx, sy, sz, dx, dy, dz;
+  int steps;
+  int pad;
+};
+
+template <int C, int BM, int MODE, bool LIGHT>
+__global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, CamBatch cb, int pair_base,
+                                                                   const float* __restrict__ out,
+                                                                   const float* __restrict__ out_grad,
+                                                                   float* __restrict__ G, RayRec* __restrict__ rays,
+                                                                   int kcap) {
+  int x, y;
+  pixel_of_thread(x, y);
+  if (x >= a.out_w || y >= a.out_h) return;
+  const CamParams& cp = cb.cam[blockIdx.z];
+  constexpr int CIN = LIGHT ? 1 : C;
+  constexpr int CG = GradChannels<C, LIGHT>::value;
+  const int64_t slice = static_cast<int64_t>(a.din.x) * a.din.y * a.din.z * CIN;
+  const float* src = a.input + cp.data_index * slice;
+  Ray r = make_ray<BM>(a, cp, x, y);
+  r.steps = min(r.steps, kcap);
+  const int pair = pair_base + static_cast<int>(blockIdx.z);
+  const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
+  const int64_t pixel = static_cast<int64_t>(y) * a.out_w + x;
+  RayRec rec;
+  rec.sx = r.sx; rec.sy = r.sy; rec.sz = r.sz; rec.dx = r.dx; rec.dy = r.dy; rec.dz = r.dz;
+  rec.steps = r.steps;
+  rec.pad = 0;
+  rays[pair * npix + pixel] = rec;
+  if (r.steps <= 0) return;
+  const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
+  float* gp = G + static_cast<int64_t>(pair) * kcap * npix * CG;
+  auto sink = [&](int k, float, float, float, const Vec<C>& gc) {
+    vstore<CG>(gp + (static_cast<int64_t>(k) * npix + pixel) * CG, fold_cell_grad<C, LIGHT>(a, gc));
+  };
+  walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
+}
+
+// Trilinear weight of voxel coordinate v for a sample coordinate q along one
+// axis, summed over the floor and ceil corners (both when they coincide), as
+// scatter_linear assigns it; false if v is neither corner.
+NGT_DEVICE bool axis_weight(float q, int v, float& w) {
+  const float f = floorf(q);
+  const float c = ceilf(q);
+  const float t = q - f;
+  bool hit = false;
+  w = 0.f;
+  if (static_cast<int>(f) == v) { w += 1.f - t; hit = true; }
+  if (static_cast<int>(c) == v) { w += t; hit = true; }
+  return hit;
+}
+
+template <int CG>
+__global__ __launch_bounds__(256) void raymarch_bwd_gather_kernel(RayArgs a, CamBatch cb, int pair_base,
+                                                                  const RayRec* __restrict__ rays,
+                                                                  const float* __restrict__ G, int kcap,
+                                                                  float* __restrict__ in_grad, int accumulate) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z % a.din.z;
+  const int n = blockIdx.z / a.din.z;
+  if (x >= a.din.x || y >= a.din.y) return;
+  const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
+  const float fdx = static_cast<float>(a.din.x), fdy = static_cast<float>(a.din.y), fdz = static_cast<float>(a.din.z);
+  Vec<CG> acc = vzero<CG>();
+  for (int ci = 0; ci < cb.count; ++ci) {
+    const CamParams& cp = cb.cam[ci];
+    if (cp.data_index != n) continue;
+    // pixel range: pixel centres inside the projection of p in [v-0.5, v+1.5]^3
+    float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
+    bool behind = false;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const float px = static_cast<float>(x) + ((c & 1) ? 1.5f : -0.5f);
+      const float py = static_cast<float>(y) + ((c & 2) ? 1.5f : -0.5f);
+      const float pz = static_cast<float>(z) + ((c & 4) ? 1.5f : -0.5f);
+      float vx, vy, vz, vw;
+      mat4_apply(cp.mv, px, py, pz, 1.f, vx, vy, vz, vw);
+      float cx, cy, cz, cw;
+      mat4_apply(cp.p, vx, vy, vz, vw, cx, cy, cz, cw);
+      if (!(cw > 0.f)) { behind = true; continue; }
+      const float iw = 1.f / cw;
+      const float u = (cx * iw * 0.5f + 0.5f) * static_cast<float>(a.out_w) - 0.5f;
+      const float w = (cy * iw * 0.5f + 0.5f) * static_cast<float>(a.out_h) - 0.5f;
+      umin = fminf(umin, u); umax = fmaxf(umax, u);
+      vmin = fminf(vmin, w); vmax = fmaxf(vmax, w);
+    }
+    int x0 = 0, x1 = a.out_w - 1, y0 = 0, y1 = a.out_h - 1;
+    if (!behind) {
+      x0 = max(x0, static_cast<int>(ceilf(umin - 1e-3f))); x1 = min(x1, static_cast<int>(floorf(umax + 1e-3f)));
+      y0 = max(y0, static_cast<int>(ceilf(vmin - 1e-3f))); y1 = min(y1, static_cast<int>(floorf(vmax + 1e-3f)));
+    }
+    const int pair = pair_base + ci;
+    const RayRec* rp = rays + pair * npix;
+    const float* gp = G + static_cast<int64_t>(pair) * kcap * npix * CG;
+    for (int py = y0; py <= y1; ++py) {
+      for (int px = x0; px <= x1; ++px) {
+        const RayRec rr = rp[static_cast<int64_t>(py) * a.out_w + px];
+        if (rr.steps <= 0) continue;
+        // k interval where s + k d lies in (v-0.5, v+1.5) on every axis
+        float lo = -1.0e30f, hi = 1.0e30f;
+        bool miss = false;
+        const float sv[3] = {rr.sx, rr.sy, rr.sz};
+        const float dv[3] = {rr.dx, rr.dy, rr.dz};
+        const int vv[3] = {x, y, z};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+          const float b0 = static_cast<float>(vv[ax]) - 0.5f, b1 = static_cast<float>(vv[ax]) + 1.5f;
+          if (fabsf(dv[ax]) > 1e-30f) {
+            const float t0 = (b0 - sv[ax]) / dv[ax], t1 = (b1 - sv[ax]) / dv[ax];
+            lo = fmaxf(lo, fminf(t0, t1));
+            hi = fminf(hi, fmaxf(t0, t1));
+          } else if (!(sv[ax] > b0 - 1e-3f && sv[ax] < b1 + 1e-3f)) {
+            miss = true;
+          }
+        }
+        if (miss || lo > hi + 2.f) continue;
+        const int k0 = max(0, static_cast<int>(floorf(fmaxf(lo, -1.0f))) - 1);
+        const int k1 = min(rr.steps - 1, static_cast<int>(floorf(fminf(hi, static_cast<float>(rr.steps)))) + 1);
+        for (int k = k0; k <= k1; ++k) {
+          const float kf = static_cast<float>(k);
+          const float qx = fmaf(kf, rr.dx, rr.sx) - 0.5f;
+          const float qy = fmaf(kf, rr.dy, rr.sy) - 0.5f;
+          const float qz = fmaf(kf, rr.dz, rr.sz) - 0.5f;
+          // forward BORDER validity of the sample position
+          if (!((qx + 0.5f >= 0.f) && (qx + 0.5f < fdx) && (qy + 0.5f >= 0.f) && (qy + 0.5f < fdy) &&
+                (qz + 0.5f >= 0.f) && (qz + 0.5f < fdz)))
+            continue;
+          float wx, wy, wz;
+          if (!axis_weight(qx, x, wx) || !axis_weight(qy, y, wy) || !axis_weight(qz, z, wz)) continue;
+          const float w = wx * wy * wz;
+          const Vec<CG> gs = vload<CG>(gp + (static_cast<int64_t>(k) * npix + static_cast<int64_t>(py) * a.out_w + px) * CG);
+#pragma unroll
+          for (int i = 0; i < CG; ++i) acc.v[i] += gs.v[i] * w;
+        }
+      }
+    }
+  }
+  float* dst = in_grad + (static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z + cell_offset(a.din, x, y, z)) * CG;
+  if (accumulate) {
+    Vec<CG> prev = vload<CG>(dst);
+#pragma unroll
+    for (int i = 0; i < CG; ++i) acc.v[i] += prev.v[i];
+  }
+  vstore<CG>(dst, acc);
 }
 
 // Reference gradient of one camera: reverse walk from `end` with the
@@ -339,6 +527,126 @@ void for_cam_batches(const std::vector<Pair>& pairs, const float* M, const float
   }
 }
 
+// ------------------------------------------------------ gather planning ----
+
+// Upper bound of the samples per BORDER ray: steps = int(len / |step|) + 2
+// with len <= (cell-centre AABB diagonal + 2); |step| is minimised over a set
+// of pixels (it grows away from the principal point) and taken 3% smaller.
+int ray_step_cap(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* Fr,
+                 const RayArgs& a) {
+  double min_step = 1e300;
+  const double zmax = static_cast<double>(std::max(a.out_d - 1, 1));
+  std::vector<int> xs, ys;
+  auto pick = [](int n, std::vector<int>& v) {
+    for (int i = 0; i < 9; ++i) v.push_back(static_cast<int>((static_cast<long>(n - 1) * i) / 8));
+    v.push_back((n - 1) / 2);
+    v.push_back(n / 2);
+  };
+  pick(a.out_w, xs);
+  pick(a.out_h, ys);
+  for (const Pair& p : pairs) {
+    const CamParams cp = make_cam(M + 16 * p.n, V + 16 * p.v, P + 16 * p.v, Fr + 6 * p.v, p.n, p.out);
+    auto to_os = [&](double ix, double iy, double iz, double o[3]) {
+      const double n[4] = {ix / a.out_w * 2.0 - 1.0, iy / a.out_h * 2.0 - 1.0, iz / a.out_d * 2.0 - 1.0, 1.0};
+      double vs[4];
+      for (int r = 0; r < 4; ++r) {
+        vs[r] = 0.0;
+        for (int c = 0; c < 4; ++c) vs[r] += static_cast<double>(cp.p_inv[r * 4 + c]) * n[c];
+      }
+      for (int r = 0; r < 4; ++r) vs[r] /= vs[3];
+      for (int r = 0; r < 3; ++r) {
+        o[r] = 0.0;
+        for (int c = 0; c < 4; ++c) o[r] += static_cast<double>(cp.mv_inv[r * 4 + c]) * vs[c];
+      }
+    };
+    for (int y : ys)
+      for (int x : xs) {
+        double s0[3], s1[3];
+        to_os(x + 0.5, y + 0.5, 0.5, s0);
+        to_os(x + 0.5, y + 0.5, zmax + 0.5, s1);
+        const double dx = (s1[0] - s0[0]) / zmax, dy = (s1[1] - s0[1]) / zmax, dz = (s1[2] - s0[2]) / zmax;
+        const double l = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (l > 0.0 && l < min_step) min_step = l;
+      }
+  }
+  const double diag = std::sqrt(static_cast<double>(a.din.x - 1) * (a.din.x - 1) +
+                                static_cast<double>(a.din.y - 1) * (a.din.y - 1) +
+                                static_cast<double>(a.din.z - 1) * (a.din.z - 1));
+  const int hard_cap = 16 * (a.din.x + a.din.y + a.din.z + a.out_d);
+  if (!(min_step < 1e300)) return hard_cap;
+  const double k = std::ceil((diag + 2.0) / (0.97 * min_step)) + 3.0;
+  return static_cast<int>(std::min<double>(k, hard_cap));
+}
+
+struct GatherPlan {
+  int kcap;
+  RayRec* rays;
+  float* G;
+};
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+size_t gather_workspace_bytes(const int64_t* shape, const float* M, const float* V, const float* P, const float* F,
+                              int num_cameras, const int32_t* out_dhw, int interpolation, int boundary, int separate,
+                              int grad_channels) {
+  if (!shape || !M || !V || !P || !F || !out_dhw || num_cameras <= 0) return 0;
+  if (interpolation != LINEAR || boundary != BORDER) return 0;
+  std::vector<Pair> pairs;
+  if (make_pairs(static_cast<int>(shape[0]), num_cameras, separate != 0, pairs) != NGT_OK) return 0;
+  const RayArgs a = make_ray_args(nullptr, shape, out_dhw);
+  const int kcap = ray_step_cap(pairs, M, V, P, F, a);
+  const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
+  return align_up(pairs.size() * npix * sizeof(RayRec)) +
+         align_up(pairs.size() * static_cast<size_t>(kcap) * npix * grad_channels * sizeof(float));
+}
+
+bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* F,
+                 const RayArgs& a, int interpolation, int boundary, int grad_channels, void* workspace,
+                 size_t workspace_bytes, GatherPlan& gp) {
+  if (interpolation != LINEAR || boundary != BORDER || workspace == nullptr) return false;
+  gp.kcap = ray_step_cap(pairs, M, V, P, F, a);
+  const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
+  const size_t rays_bytes = align_up(pairs.size() * npix * sizeof(RayRec));
+  const size_t g_bytes = align_up(pairs.size() * static_cast<size_t>(gp.kcap) * npix * grad_channels * sizeof(float));
+  if (workspace_bytes < rays_bytes + g_bytes) return false;
+  gp.rays = static_cast<RayRec*>(workspace);
+  gp.G = reinterpret_cast<float*>(static_cast<char*>(workspace) + rays_bytes);
+  return true;
+}
+
+template <bool LIGHT>
+void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* F,
+                const RayArgs& a, int blending_mode, int C, const float* output, const float* output_grad,
+                const GatherPlan& gp, float* in_grad, int n_data, hipStream_t s) {
+  for (size_t b = 0; b < pairs.size(); b += kMaxCamsPerLaunch) {
+    CamBatch cb;
+    cb.count = static_cast<int>(std::min<size_t>(kMaxCamsPerLaunch, pairs.size() - b));
+    for (int i = 0; i < cb.count; ++i) {
+      const Pair& p = pairs[b + i];
+      cb.cam[i] = make_cam(M + 16 * p.n, V + 16 * p.v, P + 16 * p.v, F + 6 * p.v, p.n, p.out);
+    }
+    const dim3 rgrid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    const int base = static_cast<int>(b);
+    if constexpr (LIGHT) {
+      NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<4, BORDER, MM, true>), rgrid, dim3(256), 0, s, a, cb, base,
+                             output, output_grad, gp.G, gp.rays, gp.kcap));
+    } else {
+      NGT_SWITCH_C(C, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false>), rgrid, dim3(256), 0, s, a, cb, base,
+                             output, output_grad, gp.G, gp.rays, gp.kcap)));
+    }
+    const dim3 vgrid(blocks_for(a.din.x, 64), blocks_for(a.din.y, 4), static_cast<unsigned>(a.din.z) * n_data);
+    const int accumulate = b > 0 ? 1 : 0;
+    const int CG = LIGHT ? 1 : C;
+    NGT_SWITCH_C(CG,
+        hipLaunchKernelGGL((raymarch_bwd_gather_kernel<CC>), vgrid, dim3(64, 4), 0, s, a, cb, base, gp.rays, gp.G,
+                           gp.kcap, in_grad, accumulate));
+  }
+}
+
 }  // namespace
 }  // namespace ngt
 
@@ -366,12 +674,18 @@ int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_sha
   return check_launch("ngt_raymarch_grid_transform_fwd");
 }
 
-size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape, int num_cameras,
-                                                      int gradient_semantics) {
-  (void)num_cameras;
-  if (gradient_semantics != NGT_GRAD_REFERENCE || input_shape == nullptr) return 0;
-  const int64_t cells = input_shape[1] * input_shape[2] * input_shape[3];
-  return static_cast<size_t>(cells * (input_shape[4] + 1)) * sizeof(float);
+size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape, const float* matrix_m,
+                                                      const float* matrix_v, const float* matrix_p,
+                                                      const float* frustum, int num_cameras,
+                                                      const int32_t* output_dhw, int interpolation, int boundary,
+                                                      int separate_camera_batch, int gradient_semantics) {
+  if (input_shape == nullptr) return 0;
+  if (gradient_semantics == NGT_GRAD_REFERENCE) {
+    const int64_t cells = input_shape[1] * input_shape[2] * input_shape[3];
+    return static_cast<size_t>(cells * (input_shape[4] + 1)) * sizeof(float);
+  }
+  return gather_workspace_bytes(input_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw,
+                                interpolation, boundary, separate_camera_batch, static_cast<int>(input_shape[4]));
 }
 
 int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_shape, const float* output,
@@ -388,9 +702,10 @@ int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_sha
   hipStream_t s = as_stream(stream);
   const Shape5 si = shape5(input_shape);
   const int C = static_cast<int>(si.c);
-  fill_zero(input_grad, si.elems(), s);
   if (gradient_semantics == NGT_GRAD_REFERENCE) {
-    const size_t need = ngt_raymarch_grid_transform_bwd_workspace_size(input_shape, num_cameras, gradient_semantics);
+    fill_zero(input_grad, si.elems(), s);
+    const int64_t cells = si.cells();
+    const size_t need = static_cast<size_t>(cells * (C + 1)) * sizeof(float);
     NGT_REQUIRE(workspace && workspace_bytes >= need, NGT_ERR_WORKSPACE,
                 "workspace too small for ngt_raymarch_grid_transform_bwd");
     if (interpolation != LINEAR) return check_launch("ngt_raymarch_grid_transform_bwd");  // reference scatters LINEAR only
@@ -409,7 +724,15 @@ int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_sha
           hipLaunchKernelGGL((normalize_by_weight_kernel<CC>), dim3(blocks_for(si.cells(), 256)), dim3(256), 0, s, gbuf,
                              wsum, input_grad + p.n * si.cells() * C, si.cells()));
     }
+    return check_launch("ngt_raymarch_grid_transform_bwd");
+  }
+  GatherPlan gp;
+  if (plan_gather(pairs, matrix_m, matrix_v, matrix_p, frustum, a, interpolation, boundary, C, workspace,
+                  workspace_bytes, gp)) {
+    run_gather<false>(pairs, matrix_m, matrix_v, matrix_p, frustum, a, blending_mode, C, output, output_grad, gp,
+                      input_grad, static_cast<int>(si.n), s);
   } else {
+    fill_zero(input_grad, si.elems(), s);
     for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
       const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
       NGT_SWITCH_C(C, NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
@@ -441,11 +764,19 @@ int ngt_render_density_fwd(const float* density, const int64_t* density_shape, c
   return check_launch("ngt_render_density_fwd");
 }
 
+size_t ngt_render_density_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
+                                             const float* matrix_v, const float* matrix_p, const float* frustum,
+                                             int num_cameras, const int32_t* output_dhw, int interpolation,
+                                             int boundary) {
+  return gather_workspace_bytes(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw,
+                                interpolation, boundary, 1, 1);
+}
+
 int ngt_render_density_bwd(const float* density, const int64_t* density_shape, const float* light_rgb,
                            float depth_step, const float* output, const float* output_grad, const float* matrix_m,
                            const float* matrix_v, const float* matrix_p, const float* frustum, int num_cameras,
                            const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
-                           float* density_grad, ngt_stream_t stream) {
+                           float* density_grad, void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
   std::vector<Pair> pairs;
   int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
                           boundary, blending_mode, 1, pairs, true);
@@ -455,13 +786,20 @@ int ngt_render_density_bwd(const float* density, const int64_t* density_shape, c
   a.light_r = light_rgb[0]; a.light_g = light_rgb[1]; a.light_b = light_rgb[2];
   a.depth_step = depth_step;
   hipStream_t s = as_stream(stream);
-  fill_zero(density_grad, shape5(density_shape).elems(), s);
-  for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
-    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
-    NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-        hipLaunchKernelGGL((raymarch_bwd_exact_kernel<4, FF, BB, MM, true>), grid, dim3(256), 0, s, a, cb, output,
-                           output_grad, density_grad))));
-  });
+  GatherPlan gp;
+  if (plan_gather(pairs, matrix_m, matrix_v, matrix_p, frustum, a, interpolation, boundary, 1, workspace,
+                  workspace_bytes, gp)) {
+    run_gather<true>(pairs, matrix_m, matrix_v, matrix_p, frustum, a, blending_mode, 4, output, output_grad, gp,
+                     density_grad, static_cast<int>(density_shape[0]), s);
+  } else {
+    fill_zero(density_grad, shape5(density_shape).elems(), s);
+    for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
+      const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+      NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_bwd_exact_kernel<4, FF, BB, MM, true>), grid, dim3(256), 0, s, a, cb, output,
+                             output_grad, density_grad))));
+    });
+  }
   return check_launch("ngt_render_density_bwd");
 }
 

@@ -33,19 +33,42 @@ def hipcc():
     return os.path.join(rocm, "bin", "hipcc")
 
 
-def build(verbose=False, out=LIB_PATH):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = out + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", INCLUDE, "-I", CSRC, "-o", tmp] + srcs
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    msgs = [l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l]
-    if res.returncode != 0:
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
+def _finish(proc):
+    out, _ = proc.communicate()
+    msgs = [l for l in out.splitlines() if "packed-fp32-ops" not in l]
+    if proc.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + "\n".join(msgs))
-    if verbose and msgs:
-        print("\n".join(msgs))
-    os.replace(tmp, out)
+    return msgs
+
+
+def build(verbose=False, out=LIB_PATH, jobs=None):
+    """Compile every source to an object in parallel, then link the shared
+    library (no relocatable device code: kernels never call across files)."""
+    import tempfile
+    jobs = jobs or min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)))
+    with tempfile.TemporaryDirectory(prefix="ngt_build_") as tmpdir:
+        objs, procs = [], []
+        for src in SOURCES:
+            obj = os.path.join(tmpdir, src.replace(".hip", ".o"))
+            objs.append(obj)
+            cmd = [hipcc()] + [f for f in HIPCC_FLAGS if f != "-shared"] + [
+                "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+            procs.append(_run(cmd, verbose))
+            if len(procs) >= jobs:
+                _finish(procs.pop(0))
+        for p in procs:
+            _finish(p)
+        tmp = out + ".tmp"
+        msgs = _finish(_run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose))
+        if verbose and msgs:
+            print("\n".join(msgs))
+        os.replace(tmp, out)
     return out
 
 

@@ -28,6 +28,11 @@ from . import _lib
 
 gradient_semantics = "exact"
 
+# Exact raymarch / render gradients with BORDER + LINEAR use the atomics-free
+# gather adjoint (per-sample gradient pass + per-voxel gather); False selects
+# the fp32-atomic scatter kernels instead (same result up to summation order).
+use_gather_adjoint = True
+
 INTERPOLATION = {"NEAREST": 0, "LINEAR": 1, "MIN": 2, "MAX": 3, "QUADRATIC": 4}
 BOUNDARY = {"BORDER": 0, "CLAMP": 1, "WRAP": 2}
 MIPMAPPING = {"NONE": 0, "NEAREST": 1, "LINEAR": 2}
@@ -327,7 +332,12 @@ def _ray_bwd(inp, out, out_grad, cams, out_shape, a, sem):
     cams.check_batch(inp.shape[0])
     in_grad = torch.empty_like(inp)
     s_in = _lib.shape5(inp)
-    ws, nbytes = _lib.workspace(lib.ngt_raymarch_grid_transform_bwd_workspace_size(s_in, cams.nv, sem), inp.device)
+    nb = lib.ngt_raymarch_grid_transform_bwd_workspace_size(
+        s_in, cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape), a["interpolation"], a["boundary"],
+        int(a["separate_camera_batch"]), sem)
+    if sem == 0 and not use_gather_adjoint:
+        nb = 0
+    ws, nbytes = _lib.workspace(nb, inp.device)
     _lib.check(lib.ngt_raymarch_grid_transform_bwd(
         _lib.ptr(inp), s_in, _lib.ptr(out), _lib.ptr(out_grad), cams.m, cams.v, cams.p, cams.f, cams.nv,
         _lib.dhw(out_shape), a["interpolation"], a["boundary"], a["blending_mode"], int(a["separate_camera_batch"]),
@@ -415,10 +425,14 @@ def _render_bwd(density, light_rgb, depth_step, out, out_grad, cams, out_shape, 
     out_grad = _lib.device_tensor(out_grad, "output_grad")
     g = torch.empty_like(density)
     lp, _, keep = _lib.host_floats(light_rgb, 3)
+    s_d = _lib.shape5(density)
+    nb = lib.ngt_render_density_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv,
+                                                   _lib.dhw(out_shape), interp, boundary) if use_gather_adjoint else 0
+    ws, nbytes = _lib.workspace(nb, density.device)
     _lib.check(lib.ngt_render_density_bwd(
-        _lib.ptr(density), _lib.shape5(density), lp, float(depth_step), _lib.ptr(out), _lib.ptr(out_grad), cams.m,
+        _lib.ptr(density), s_d, lp, float(depth_step), _lib.ptr(out), _lib.ptr(out_grad), cams.m,
         cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape), interp, boundary, mode, _lib.ptr(g),
-        _lib.stream(density.device)))
+        _lib.ptr(ws), nbytes, _lib.stream(density.device)))
     return g
 
 
