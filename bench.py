@@ -150,6 +150,7 @@ def op_breakdown(torch, prob, iters):
     g_in, gx, gy, gz = (torch.empty_like(t) for t in (rho, vx, vy, vz))
     wsb = lib.ngt_advect_workspace_size(s_rho, 1, 0, 2, 1)
     ws = torch.empty(wsb, dtype=torch.uint8, device=rho.device)
+    g_out = torch.randn_like(adv)  # gradient of the advected field, [1,1,D,H,W,1]
     img = torch.empty((1, VIEWS) + IMAGE_HW + (4,), device=rho.device)
     dimg = torch.randn_like(img)
     g_adv = torch.empty_like(rho)
@@ -160,7 +161,7 @@ def op_breakdown(torch, prob, iters):
         "advect_fwd": lambda: _lib.check(lib.ngt_advect_fwd(
             P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(adv), P(sl), None, 0, st())),
         "advect_bwd": lambda: _lib.check(lib.ngt_advect_bwd(
-            P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(dimg), P(sl), 0, P(g_in), P(gx), P(gy),
+            P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(g_out), P(sl), 0, P(g_in), P(gx), P(gy),
             P(gz), P(ws), wsb, st())),
         "render_fwd": lambda: _lib.check(lib.ngt_render_density_fwd(
             P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(img), st())),
@@ -168,6 +169,12 @@ def op_breakdown(torch, prob, iters):
             P(rho), s_rho, lp, prob["ds"], P(img), P(dimg), cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0,
             0, P(g_adv), P(rws), rws_b, st())),
     }
+    # every raw buffer handed to the C ABI has the element count its entry
+    # point reads (the ABI takes plain pointers and cannot check them)
+    cells = int(np.prod(GRID))
+    for t, n in ((adv, cells), (sl, cells), (g_out, cells), (g_in, cells), (g_adv, cells),
+                 (img, VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * 4), (dimg, VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * 4)):
+        assert t.numel() == n, (tuple(t.shape), n)
     calls["render_fwd"]()
     return {k: time_call(torch, c, iters) for k, c in calls.items()}
 
