@@ -82,7 +82,17 @@ def fused_step(torch, ops, prob, leaves):
 
 
 def plain_step(torch, ops, prob, leaves):
-    """The reference's composition on the same kernels' per-op entry points."""
+    """The reference's composition on the same library's per-op entry points,
+    with the reference's scatter-form (fp32 atomic) raymarch adjoint."""
+    old = ops.use_gather_adjoint
+    ops.use_gather_adjoint = False
+    try:
+        return _plain_step(torch, ops, prob, leaves)
+    finally:
+        ops.use_gather_adjoint = old
+
+
+def _plain_step(torch, ops, prob, leaves):
     rho, vx, vy, vz = leaves
     r = prob["renderer"]
     from ngt.data_structures import VelocityGrid

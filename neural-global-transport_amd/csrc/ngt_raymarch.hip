@@ -252,12 +252,12 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
 }
 
 // ---------------------------------------------------------------------------
-// Gather-form adjoint (BORDER, LINEAR).  Pass 1 stores every sample's
+// Brick-gather adjoint (BORDER, LINEAR).  Pass 1 stores every sample's
 // gradient G[pair][k][pixel] and the ray parameters; pass 2 runs one
-// work-item per voxel that collects, from the rays whose pixel centres fall in
-// the projection of the voxel's trilinear support, exactly the samples whose
-// corners include the voxel - the same terms the scatter would add, with no
-// atomics and a single write per voxel.
+// work-group per brick of voxels that collects, from the rays whose pixel
+// centres fall in the projection of the brick's trilinear support, exactly the
+// samples with a corner in the brick - the same terms the scatter would add -
+// accumulates them in LDS and writes each voxel once, with no global atomics.
 
 struct RayRec {
   float sx, sy, sz, dx, dy, dz;
@@ -298,113 +298,157 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
   walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
 }
 
-// Trilinear weight of voxel coordinate v for a sample coordinate q along one
-// axis, summed over the floor and ceil corners (both when they coincide), as
-// scatter_linear assigns it; false if v is neither corner.
-NGT_DEVICE bool axis_weight(float q, int v, float& w) {
-  const float f = floorf(q);
-  const float c = ceilf(q);
-  const float t = q - f;
-  bool hit = false;
-  w = 0.f;
-  if (static_cast<int>(f) == v) { w += 1.f - t; hit = true; }
-  if (static_cast<int>(c) == v) { w += t; hit = true; }
-  return hit;
-}
+// Pass 2: one work-group per brick of BX x BY x BZ voxels.  For every view it
+// projects the brick's sample-support box (positions whose trilinear corners
+// can fall in the brick: p in [b0 - 0.5, b1 + 1.5)) to a pixel rectangle,
+// walks the rays of those pixels over the k interval inside the box, and adds
+// each sample's stored gradient times its corner weights to the brick's voxels
+// in LDS.  The brick owns its voxels, so the result is written with plain
+// stores; a sample on a brick boundary is visited by each brick it touches
+// and contributes only that brick's corners.
+template <int CG, int BX, int BY, int BZ>
+__global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamBatch cb, int pair_base,
+                                                                 const RayRec* __restrict__ rays,
+                                                                 const float* __restrict__ G, int kcap,
+                                                                 float* __restrict__ in_grad, int accumulate) {
+  constexpr int NV = BX * BY * BZ;
+  __shared__ float acc[NV * CG];
+  __shared__ int rect[kMaxCamsPerLaunch][4];  // x0, y0, nx, ny (nx = 0: view skipped)
+  __shared__ int ray_begin[kMaxCamsPerLaunch + 1];
+  const int tid = threadIdx.x;
+  const int bricks_x = (a.din.x + BX - 1) / BX, bricks_y = (a.din.y + BY - 1) / BY;
+  const int bricks_z = (a.din.z + BZ - 1) / BZ;
+  const int bid = blockIdx.x;
+  const int n = bid / (bricks_x * bricks_y * bricks_z);
+  const int brem = bid % (bricks_x * bricks_y * bricks_z);
+  const int b0x = (brem % bricks_x) * BX;
+  const int b0y = ((brem / bricks_x) % bricks_y) * BY;
+  const int b0z = (brem / (bricks_x * bricks_y)) * BZ;
+  const int b1x = min(b0x + BX, a.din.x) - 1, b1y = min(b0y + BY, a.din.y) - 1, b1z = min(b0z + BZ, a.din.z) - 1;
+  for (int i = tid; i < NV * CG; i += blockDim.x) acc[i] = 0.f;
 
-template <int CG>
-__global__ __launch_bounds__(256) void raymarch_bwd_gather_kernel(RayArgs a, CamBatch cb, int pair_base,
-                                                                  const RayRec* __restrict__ rays,
-                                                                  const float* __restrict__ G, int kcap,
-                                                                  float* __restrict__ in_grad, int accumulate) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z % a.din.z;
-  const int n = blockIdx.z / a.din.z;
-  if (x >= a.din.x || y >= a.din.y) return;
+  // pixel rectangles of the views (one work-item per view)
+  if (tid < cb.count) {
+    const CamParams& cp = cb.cam[tid];
+    int x0 = 0, y0 = 0, nx = 0, ny = 0;
+    if (cp.data_index == n) {
+      float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
+      bool behind = false;
+      for (int c = 0; c < 8; ++c) {
+        const float px = (c & 1) ? static_cast<float>(b1x) + 1.5f : static_cast<float>(b0x) - 0.5f;
+        const float py = (c & 2) ? static_cast<float>(b1y) + 1.5f : static_cast<float>(b0y) - 0.5f;
+        const float pz = (c & 4) ? static_cast<float>(b1z) + 1.5f : static_cast<float>(b0z) - 0.5f;
+        float vx, vy, vz, vw;
+        mat4_apply(cp.mv, px, py, pz, 1.f, vx, vy, vz, vw);
+        float cx, cy, cz, cw;
+        mat4_apply(cp.p, vx, vy, vz, vw, cx, cy, cz, cw);
+        if (!(cw > 0.f)) { behind = true; continue; }
+        const float iw = 1.f / cw;
+        const float u = (cx * iw * 0.5f + 0.5f) * static_cast<float>(a.out_w) - 0.5f;
+        const float w = (cy * iw * 0.5f + 0.5f) * static_cast<float>(a.out_h) - 0.5f;
+        umin = fminf(umin, u); umax = fmaxf(umax, u);
+        vmin = fminf(vmin, w); vmax = fmaxf(vmax, w);
+      }
+      int x1 = a.out_w - 1, y1 = a.out_h - 1;
+      if (!behind) {
+        // clamp before the float -> int conversions (far-off corners project anywhere)
+        const float fw = static_cast<float>(a.out_w), fh = static_cast<float>(a.out_h);
+        umin = fminf(fmaxf(umin, -2.f), fw + 1.f); umax = fminf(fmaxf(umax, -2.f), fw + 1.f);
+        vmin = fminf(fmaxf(vmin, -2.f), fh + 1.f); vmax = fminf(fmaxf(vmax, -2.f), fh + 1.f);
+        x0 = max(0, static_cast<int>(ceilf(umin - 1e-3f))); x1 = min(x1, static_cast<int>(floorf(umax + 1e-3f)));
+        y0 = max(0, static_cast<int>(ceilf(vmin - 1e-3f))); y1 = min(y1, static_cast<int>(floorf(vmax + 1e-3f)));
+      }
+      nx = max(0, x1 - x0 + 1);
+      ny = max(0, y1 - y0 + 1);
+    }
+    rect[tid][0] = x0; rect[tid][1] = y0; rect[tid][2] = nx; rect[tid][3] = ny;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int total = 0;
+    for (int ci = 0; ci < cb.count; ++ci) {
+      ray_begin[ci] = total;
+      total += rect[ci][2] * rect[ci][3];
+    }
+    ray_begin[cb.count] = total;
+  }
+  __syncthreads();
+
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
   const float fdx = static_cast<float>(a.din.x), fdy = static_cast<float>(a.din.y), fdz = static_cast<float>(a.din.z);
-  Vec<CG> acc = vzero<CG>();
-  for (int ci = 0; ci < cb.count; ++ci) {
-    const CamParams& cp = cb.cam[ci];
-    if (cp.data_index != n) continue;
-    // pixel range: pixel centres inside the projection of p in [v-0.5, v+1.5]^3
-    float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
-    bool behind = false;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const float px = static_cast<float>(x) + ((c & 1) ? 1.5f : -0.5f);
-      const float py = static_cast<float>(y) + ((c & 2) ? 1.5f : -0.5f);
-      const float pz = static_cast<float>(z) + ((c & 4) ? 1.5f : -0.5f);
-      float vx, vy, vz, vw;
-      mat4_apply(cp.mv, px, py, pz, 1.f, vx, vy, vz, vw);
-      float cx, cy, cz, cw;
-      mat4_apply(cp.p, vx, vy, vz, vw, cx, cy, cz, cw);
-      if (!(cw > 0.f)) { behind = true; continue; }
-      const float iw = 1.f / cw;
-      const float u = (cx * iw * 0.5f + 0.5f) * static_cast<float>(a.out_w) - 0.5f;
-      const float w = (cy * iw * 0.5f + 0.5f) * static_cast<float>(a.out_h) - 0.5f;
-      umin = fminf(umin, u); umax = fmaxf(umax, u);
-      vmin = fminf(vmin, w); vmax = fmaxf(vmax, w);
-    }
-    int x0 = 0, x1 = a.out_w - 1, y0 = 0, y1 = a.out_h - 1;
-    if (!behind) {
-      x0 = max(x0, static_cast<int>(ceilf(umin - 1e-3f))); x1 = min(x1, static_cast<int>(floorf(umax + 1e-3f)));
-      y0 = max(y0, static_cast<int>(ceilf(vmin - 1e-3f))); y1 = min(y1, static_cast<int>(floorf(vmax + 1e-3f)));
-    }
+  const float lo3[3] = {static_cast<float>(b0x) - 0.5f, static_cast<float>(b0y) - 0.5f, static_cast<float>(b0z) - 0.5f};
+  const float hi3[3] = {static_cast<float>(b1x) + 1.5f, static_cast<float>(b1y) + 1.5f, static_cast<float>(b1z) + 1.5f};
+  const int total = ray_begin[cb.count];
+  for (int r = tid; r < total; r += blockDim.x) {
+    int ci = 0;
+    while (ci + 1 < cb.count && r >= ray_begin[ci + 1]) ++ci;
+    const int local = r - ray_begin[ci];
+    const int px = rect[ci][0] + local % rect[ci][2];
+    const int py = rect[ci][1] + local / rect[ci][2];
     const int pair = pair_base + ci;
-    const RayRec* rp = rays + pair * npix;
-    const float* gp = G + static_cast<int64_t>(pair) * kcap * npix * CG;
-    for (int py = y0; py <= y1; ++py) {
-      for (int px = x0; px <= x1; ++px) {
-        const RayRec rr = rp[static_cast<int64_t>(py) * a.out_w + px];
-        if (rr.steps <= 0) continue;
-        // k interval where s + k d lies in (v-0.5, v+1.5) on every axis
-        float lo = -1.0e30f, hi = 1.0e30f;
-        bool miss = false;
-        const float sv[3] = {rr.sx, rr.sy, rr.sz};
-        const float dv[3] = {rr.dx, rr.dy, rr.dz};
-        const int vv[3] = {x, y, z};
+    const RayRec rr = rays[pair * npix + static_cast<int64_t>(py) * a.out_w + px];
+    if (rr.steps <= 0) continue;
+    const float sv[3] = {rr.sx, rr.sy, rr.sz};
+    const float dv[3] = {rr.dx, rr.dy, rr.dz};
+    float lo = -1.0e30f, hi = 1.0e30f;
+    bool miss = false;
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-          const float b0 = static_cast<float>(vv[ax]) - 0.5f, b1 = static_cast<float>(vv[ax]) + 1.5f;
-          if (fabsf(dv[ax]) > 1e-30f) {
-            const float t0 = (b0 - sv[ax]) / dv[ax], t1 = (b1 - sv[ax]) / dv[ax];
-            lo = fmaxf(lo, fminf(t0, t1));
-            hi = fminf(hi, fmaxf(t0, t1));
-          } else if (!(sv[ax] > b0 - 1e-3f && sv[ax] < b1 + 1e-3f)) {
-            miss = true;
-          }
-        }
-        if (miss || lo > hi + 2.f) continue;
-        const int k0 = max(0, static_cast<int>(floorf(fmaxf(lo, -1.0f))) - 1);
-        const int k1 = min(rr.steps - 1, static_cast<int>(floorf(fminf(hi, static_cast<float>(rr.steps)))) + 1);
-        for (int k = k0; k <= k1; ++k) {
-          const float kf = static_cast<float>(k);
-          const float qx = fmaf(kf, rr.dx, rr.sx) - 0.5f;
-          const float qy = fmaf(kf, rr.dy, rr.sy) - 0.5f;
-          const float qz = fmaf(kf, rr.dz, rr.sz) - 0.5f;
-          // forward BORDER validity of the sample position
-          if (!((qx + 0.5f >= 0.f) && (qx + 0.5f < fdx) && (qy + 0.5f >= 0.f) && (qy + 0.5f < fdy) &&
-                (qz + 0.5f >= 0.f) && (qz + 0.5f < fdz)))
-            continue;
-          float wx, wy, wz;
-          if (!axis_weight(qx, x, wx) || !axis_weight(qy, y, wy) || !axis_weight(qz, z, wz)) continue;
-          const float w = wx * wy * wz;
-          const Vec<CG> gs = vload<CG>(gp + (static_cast<int64_t>(k) * npix + static_cast<int64_t>(py) * a.out_w + px) * CG);
+    for (int ax = 0; ax < 3; ++ax) {
+      if (fabsf(dv[ax]) > 1e-30f) {
+        const float inv = 1.f / dv[ax];
+        const float t0 = (lo3[ax] - sv[ax]) * inv, t1 = (hi3[ax] - sv[ax]) * inv;
+        lo = fmaxf(lo, fminf(t0, t1));
+        hi = fminf(hi, fmaxf(t0, t1));
+      } else if (!(sv[ax] > lo3[ax] - 1e-3f && sv[ax] < hi3[ax] + 1e-3f)) {
+        miss = true;
+      }
+    }
+    if (miss || !(lo <= hi + 2.f)) continue;
+    const float fsteps = static_cast<float>(rr.steps);
+    const int k0 = max(0, static_cast<int>(floorf(fminf(fmaxf(lo, -1.f), fsteps))) - 1);
+    const int k1 = min(rr.steps - 1, static_cast<int>(floorf(fminf(fmaxf(hi, -2.f), fsteps))) + 1);
+    const float* gp = G + (static_cast<int64_t>(pair) * kcap * npix + static_cast<int64_t>(py) * a.out_w + px) * CG;
+    for (int k = k0; k <= k1; ++k) {
+      const float kf = static_cast<float>(k);
+      const float qx = fmaf(kf, rr.dx, rr.sx) - 0.5f;
+      const float qy = fmaf(kf, rr.dy, rr.sy) - 0.5f;
+      const float qz = fmaf(kf, rr.dz, rr.sz) - 0.5f;
+      if (!((qx + 0.5f >= 0.f) && (qx + 0.5f < fdx) && (qy + 0.5f >= 0.f) && (qy + 0.5f < fdy) &&
+            (qz + 0.5f >= 0.f) && (qz + 0.5f < fdz)))
+        continue;
+      const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+      const int ix0 = static_cast<int>(fx), iy0 = static_cast<int>(fy), iz0 = static_cast<int>(fz);
+      const int ix1 = static_cast<int>(ceilf(qx)), iy1 = static_cast<int>(ceilf(qy)), iz1 = static_cast<int>(ceilf(qz));
+      // corners of this brick only (skips the k margins outside the box too)
+      if (ix1 < b0x || ix0 > b1x || iy1 < b0y || iy0 > b1y || iz1 < b0z || iz0 > b1z) continue;
+      const float wx = qx - fx, wy = qy - fy, wz = qz - fz;
+      const Vec<CG> gs = vload<CG>(gp + static_cast<int64_t>(k) * npix * CG);
+      const float fw[2][3] = {{1.f - wx, 1.f - wy, 1.f - wz}, {wx, wy, wz}};
 #pragma unroll
-          for (int i = 0; i < CG; ++i) acc.v[i] += gs.v[i] * w;
-        }
+      for (int c = 0; c < 8; ++c) {
+        const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+        const int ix = bx ? ix1 : ix0, iy = by ? iy1 : iy0, iz = bz ? iz1 : iz0;
+        if (ix < b0x || ix > b1x || iy < b0y || iy > b1y || iz < b0z || iz > b1z) continue;
+        const float w = fw[bx][0] * fw[by][1] * fw[bz][2];
+        float* dst = acc + (((iz - b0z) * BY + (iy - b0y)) * BX + (ix - b0x)) * CG;
+#pragma unroll
+        for (int i = 0; i < CG; ++i) atomicAdd(dst + i, gs.v[i] * w);
       }
     }
   }
-  float* dst = in_grad + (static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z + cell_offset(a.din, x, y, z)) * CG;
-  if (accumulate) {
-    Vec<CG> prev = vload<CG>(dst);
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z;
+  for (int i = tid; i < NV; i += blockDim.x) {
+    const int lx = i % BX, ly = (i / BX) % BY, lz = i / (BX * BY);
+    const int x = b0x + lx, y = b0y + ly, z = b0z + lz;
+    if (x > b1x || y > b1y || z > b1z) continue;
+    float* d = in_grad + (base + cell_offset(a.din, x, y, z)) * CG;
 #pragma unroll
-    for (int i = 0; i < CG; ++i) acc.v[i] += prev.v[i];
+    for (int c = 0; c < CG; ++c) {
+      const float v = acc[i * CG + c];
+      d[c] = accumulate ? d[c] + v : v;
+    }
   }
-  vstore<CG>(dst, acc);
 }
 
 // Reference gradient of one camera: reverse walk from `end` with the
@@ -638,12 +682,13 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
           hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false>), rgrid, dim3(256), 0, s, a, cb, base,
                              output, output_grad, gp.G, gp.rays, gp.kcap)));
     }
-    const dim3 vgrid(blocks_for(a.din.x, 64), blocks_for(a.din.y, 4), static_cast<unsigned>(a.din.z) * n_data);
+    constexpr int BX = 8, BY = 8, BZ = 8;
+    const unsigned bricks = blocks_for(a.din.x, BX) * blocks_for(a.din.y, BY) * blocks_for(a.din.z, BZ);
     const int accumulate = b > 0 ? 1 : 0;
     const int CG = LIGHT ? 1 : C;
     NGT_SWITCH_C(CG,
-        hipLaunchKernelGGL((raymarch_bwd_gather_kernel<CC>), vgrid, dim3(64, 4), 0, s, a, cb, base, gp.rays, gp.G,
-                           gp.kcap, in_grad, accumulate));
+        hipLaunchKernelGGL((raymarch_bwd_brick_kernel<CC, BX, BY, BZ>), dim3(bricks * n_data), dim3(256), 0, s, a, cb,
+                           base, gp.rays, gp.G, gp.kcap, in_grad, accumulate));
   }
 }
 

@@ -29,7 +29,7 @@ from . import _lib
 gradient_semantics = "exact"
 
 # Exact raymarch / render gradients with BORDER + LINEAR use the atomics-free
-# gather adjoint (per-sample gradient pass + per-voxel gather); False selects
+# brick-gather adjoint (per-sample gradient pass + per-brick LDS gather); False selects
 # the fp32-atomic scatter kernels instead (same result up to summation order).
 use_gather_adjoint = True
 
