@@ -17,7 +17,12 @@
 //       g_sl at x - ts*v, g_v(x) -= ts * <grad S(sl), -g/2>
 //   B2  scatter g_sl into g_rho at x + ts*v, g_v(x) += ts * <grad S(rho), g_sl>
 //   B3  staggered: g_face = (g_v(left cell) + g_v(right cell)) / 2
-// Scatters use hardware fp32 atomics.
+// For 1-2 channels the scatters of B1 / B2 accumulate in LDS tiles per brick
+// of cells and are summed by the consumer (no global atomics unless a
+// displacement leaves the tile halo); the reference-normalised gradient and
+// 3-4 channels scatter with hardware fp32 atomics.
+#include <algorithm>
+
 #include "ngt_host.hpp"
 
 namespace ngt {
@@ -304,12 +309,294 @@ __global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, i
   }
 }
 
+// ---------------------------------------------------------------------------
+// Brick-tiled exact backward (C <= 2).  A work-group owns a brick of
+// kBrickX x kBrickY x kBrickZ cells and an LDS tile of the brick grown by
+// kHalo cells on every side.  The trilinear scatter of the brick's samples
+// lands in the tile whenever the displacement |ts*v| stays within kHalo
+// cells (corners of x +- s lie in [x - kHalo, x + kHalo]); rarer corners
+// further out (large velocities, WRAP across the domain edge) go to an
+// overflow field with global fp32 atomics.  Each tile is stored whole to a
+// per-brick partial buffer with plain coalesced stores, and the consumer
+// gathers, for every cell, the (at most 2 per axis) tiles that cover it in
+// a fixed order: no global atomics in the common case and a deterministic
+// sum.
+constexpr int kBrickX = 32, kBrickY = 8, kBrickZ = 8, kHalo = 3;
+constexpr int kTileX = kBrickX + 2 * kHalo, kTileY = kBrickY + 2 * kHalo, kTileZ = kBrickZ + 2 * kHalo;
+constexpr int kTileCells = kTileX * kTileY * kTileZ;
+
+struct BrickGrid {
+  int nx, ny, nz;
+  __host__ __device__ int count() const { return nx * ny * nz; }
+};
+
+inline BrickGrid brick_grid(const Dims& d) {
+  return BrickGrid{static_cast<int>(blocks_for(d.x, kBrickX)), static_cast<int>(blocks_for(d.y, kBrickY)),
+                   static_cast<int>(blocks_for(d.z, kBrickZ))};
+}
+
+// Scatter val * w to the 8 corners of q: into the LDS tile (origin o*) when
+// the corner lies in it, else atomically into `ovf` (a full field).
+template <int C, int BM>
+NGT_DEVICE void scatter_linear_tile(float* tile, int ox, int oy, int oz, float* __restrict__ ovf, const Dims& d,
+                                    float qx, float qy, float qz, const Vec<C>& val) {
+  const Corners c = make_corners<BM>(d, qx, qy, qz);
+  if constexpr (BM == BORDER) {
+    if (!c.inside) return;
+  }
+  const float fw[2][3] = {{1.f - c.wx, 1.f - c.wy, 1.f - c.wz}, {c.wx, c.wy, c.wz}};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
+    if constexpr (BM == BORDER) {
+      const bool ok = (bx ? c.vx1 : c.vx0) && (by ? c.vy1 : c.vy0) && (bz ? c.vz1 : c.vz0);
+      if (!ok) continue;
+    }
+    const int ix = bx ? c.x1 : c.x0, iy = by ? c.y1 : c.y0, iz = bz ? c.z1 : c.z0;
+    const float w = fw[bx][0] * fw[by][1] * fw[bz][2];
+    const int lx = ix - ox, ly = iy - oy, lz = iz - oz;
+    if (static_cast<unsigned>(lx) < static_cast<unsigned>(kTileX) &&
+        static_cast<unsigned>(ly) < static_cast<unsigned>(kTileY) &&
+        static_cast<unsigned>(lz) < static_cast<unsigned>(kTileZ)) {
+      float* t = tile + ((lz * kTileY + ly) * kTileX + lx) * C;
+#pragma unroll
+      for (int i = 0; i < C; ++i) atomicAdd(t + i, val.v[i] * w);
+    } else {
+      const int64_t o = cell_offset(d, ix, iy, iz);
+#pragma unroll
+      for (int i = 0; i < C; ++i) atomic_add(ovf + o * C + i, val.v[i] * w);
+    }
+  }
+}
+
+// Sum of the tile partials of `slot` covering cell (x, y, z), bricks in
+// ascending z, y, x order.
+template <int C>
+NGT_DEVICE Vec<C> gather_tiles(const float* __restrict__ tiles, int64_t slot, const BrickGrid& bg, int x, int y,
+                               int z) {
+  Vec<C> s = vzero<C>();
+  const int bx1 = min((x + kHalo) / kBrickX, bg.nx - 1), by1 = min((y + kHalo) / kBrickY, bg.ny - 1);
+  const int bz1 = min((z + kHalo) / kBrickZ, bg.nz - 1);
+  for (int bz = max(bz1 - 1, 0); bz <= bz1; ++bz) {
+    const int lz = z - (bz * kBrickZ - kHalo);
+    if (lz < 0 || lz >= kTileZ) continue;
+    for (int by = max(by1 - 1, 0); by <= by1; ++by) {
+      const int ly = y - (by * kBrickY - kHalo);
+      if (ly < 0 || ly >= kTileY) continue;
+      for (int bx = max(bx1 - 1, 0); bx <= bx1; ++bx) {
+        const int lx = x - (bx * kBrickX - kHalo);
+        if (lx < 0 || lx >= kTileX) continue;
+        const int64_t brick = (slot * bg.nz + bz) * bg.ny * bg.nx + static_cast<int64_t>(by) * bg.nx + bx;
+        const Vec<C> t = vload<C>(tiles + (brick * kTileCells + (lz * kTileY + ly) * kTileX + lx) * C);
+#pragma unroll
+        for (int i = 0; i < C; ++i) s.v[i] += t.v[i];
+      }
+    }
+  }
+  return s;
+}
+
+struct BrickCell {
+  int pair, brick, ox, oy, oz, x, y, z0;
+};
+
+NGT_DEVICE BrickCell brick_cell(const BrickGrid& bg) {
+  BrickCell b;
+  b.pair = blockIdx.x / bg.count();
+  b.brick = blockIdx.x % bg.count();
+  const int bx = b.brick % bg.nx, by = (b.brick / bg.nx) % bg.ny, bz = b.brick / (bg.nx * bg.ny);
+  b.x = bx * kBrickX + static_cast<int>(threadIdx.x);
+  b.y = by * kBrickY + static_cast<int>(threadIdx.y);
+  b.z0 = bz * kBrickZ;
+  b.ox = bx * kBrickX - kHalo;
+  b.oy = by * kBrickY - kHalo;
+  b.oz = b.z0 - kHalo;
+  return b;
+}
+
+template <int C>
+NGT_DEVICE void tile_clear(float* tile) {
+  const int tid = threadIdx.y * kBrickX + threadIdx.x;
+  for (int i = tid; i < kTileCells * C; i += kBrickX * kBrickY) tile[i] = 0.f;
+  __syncthreads();
+}
+
+template <int C>
+NGT_DEVICE void tile_store(const float* tile, float* __restrict__ tiles, const BrickGrid& bg, const BrickCell& b) {
+  __syncthreads();
+  const int tid = threadIdx.y * kBrickX + threadIdx.x;
+  float* dst = tiles + (static_cast<int64_t>(b.pair) * bg.count() + b.brick) * kTileCells * C;
+  for (int i = tid; i < kTileCells * C; i += kBrickX * kBrickY) dst[i] = tile[i];
+}
+
+// Velocity-gradient contribution of one cell: plain store / read-add-store
+// when each velocity has one (data, velocity) pair, atomics when several
+// pairs share it.
+NGT_DEVICE void add_vel_grad(float* gv, float gx, float gy, float gz, int mode) {
+  if (mode == 0) {
+    gv[0] = gx; gv[1] = gy; gv[2] = gz;
+  } else if (mode == 1) {
+    gv[0] += gx; gv[1] += gy; gv[2] += gz;
+  } else {
+    atomic_add(gv + 0, gx); atomic_add(gv + 1, gy); atomic_add(gv + 2, gz);
+  }
+}
+
+// B1 brick form: g_dir = g (direct path to the SL field), r_dir = g/2 where the
+// corrected value is kept (direct path to rho), the back-step adjoint into
+// the tile (overflow into ovf), and the back-step velocity gradient.
+template <int C, int BM, int CLAMPM>
+__global__ __launch_bounds__(256) void advect_mc_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+                                                                  const float* __restrict__ sl,
+                                                                  const float* __restrict__ g_out,
+                                                                  float* __restrict__ g_dir, float* __restrict__ r_dir,
+                                                                  float* __restrict__ ovf, float* __restrict__ tiles,
+                                                                  float* __restrict__ g_vc, int vel_mode) {
+  __shared__ float tile[kTileCells * C];
+  tile_clear<C>(tile);
+  const BrickCell b = brick_cell(bg);
+  int n, v;
+  pair_indices(a, b.pair, n, v);
+  const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
+  const float* src = rho + n * cells * C;
+  const float* slp = sl + b.pair * cells * C;
+  float* ovp = ovf + b.pair * cells * C;
+  if (b.x < a.d.x && b.y < a.d.y) {
+    const int z1 = min(b.z0 + kBrickZ, a.d.z);
+    for (int z = b.z0; z < z1; ++z) {
+      float ux, uy, uz;
+      load_velocity(a, v, b.x, b.y, z, ux, uy, uz);
+      const int64_t cell = cell_offset(a.d, b.x, b.y, z);
+      const float fx = static_cast<float>(b.x), fy = static_cast<float>(b.y), fz = static_cast<float>(z);
+      const float tx = a.ts * ux, ty = a.ts * uy, tz = a.ts * uz;
+      const float bx = fx + (-tx), by = fy + (-ty), bz = fz + (-tz);
+      Vec<C> g = vload<C>(g_out + (b.pair * cells + cell) * C);
+      const Vec<C> fwd = vload<C>(slp + cell * C);
+      const Vec<C> back = read_linear<C, BM>(slp, a.d, bx, by, bz);
+      const Vec<C> cur = vload<C>(src + cell * C);
+      Vec<C> corr;
+#pragma unroll
+      for (int i = 0; i < C; ++i) corr.v[i] = fwd.v[i] + 0.5f * (cur.v[i] - back.v[i]);
+      Vec<C> keep = vfill<C>(1.f);
+      if constexpr (CLAMPM == NGT_CLAMP_MC_SMOOTH) {
+        const Vec<C> mn = read_minmax<C, BM, false>(src, a.d, fx + tx, fy + ty, fz + tz);
+        const Vec<C> mx = read_minmax<C, BM, true>(src, a.d, fx + tx, fy + ty, fz + tz);
+#pragma unroll
+        for (int i = 0; i < C; ++i)
+          if (corr.v[i] < mn.v[i] || corr.v[i] > mx.v[i]) keep.v[i] = 0.f;
+      }
+      if constexpr (CLAMPM == NGT_CLAMP_NEGATIVE) {
+#pragma unroll
+        for (int i = 0; i < C; ++i)
+          if (!(corr.v[i] >= 0.f)) g.v[i] = 0.f;
+      }
+      Vec<C> rd, gback;
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        rd.v[i] = 0.5f * keep.v[i] * g.v[i];
+        gback.v[i] = -rd.v[i];
+      }
+      vstore<C>(g_dir + (b.pair * cells + cell) * C, g);
+      vstore<C>(r_dir + (b.pair * cells + cell) * C, rd);
+      scatter_linear_tile<C, BM>(tile, b.ox, b.oy, b.oz, ovp, a.d, bx, by, bz, gback);
+      Vec<C> dx, dy, dz;
+      read_gradient<C, BM>(slp, a.d, bx, by, bz, dx, dy, dz);
+      add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, -a.ts * vdot(gback, dx),
+                   -a.ts * vdot(gback, dy), -a.ts * vdot(gback, dz), vel_mode == 1 ? 0 : vel_mode);
+    }
+  }
+  tile_store<C>(tile, tiles, bg, b);
+}
+
+// B2 brick form: g_sl = g_dir + ovf + tiles1 (order 2) or g_out with the
+// NEGATIVE mask (order 1); scatter into the tile (overflow straight into
+// g_rho, which starts at zero), SL velocity gradient.
+template <int C, int BM, bool NEG, bool ORDER2>
+__global__ __launch_bounds__(256) void advect_sl_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+                                                                  const float* __restrict__ g_in,
+                                                                  const float* __restrict__ ovf1,
+                                                                  const float* __restrict__ tiles1,
+                                                                  float* __restrict__ g_rho, float* __restrict__ tiles2,
+                                                                  float* __restrict__ g_vc, int vel_mode) {
+  __shared__ float tile[kTileCells * C];
+  tile_clear<C>(tile);
+  const BrickCell b = brick_cell(bg);
+  int n, v;
+  pair_indices(a, b.pair, n, v);
+  const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
+  const float* src = rho + n * cells * C;
+  float* ovp = g_rho + n * cells * C;
+  if (b.x < a.d.x && b.y < a.d.y) {
+    const int z1 = min(b.z0 + kBrickZ, a.d.z);
+    for (int z = b.z0; z < z1; ++z) {
+      float ux, uy, uz;
+      load_velocity(a, v, b.x, b.y, z, ux, uy, uz);
+      const int64_t cell = cell_offset(a.d, b.x, b.y, z);
+      const float px = static_cast<float>(b.x) + a.ts * ux;
+      const float py = static_cast<float>(b.y) + a.ts * uy;
+      const float pz = static_cast<float>(z) + a.ts * uz;
+      Vec<C> g = vload<C>(g_in + (b.pair * cells + cell) * C);
+      if constexpr (ORDER2) {
+        const Vec<C> o = vload<C>(ovf1 + (b.pair * cells + cell) * C);
+        const Vec<C> t = gather_tiles<C>(tiles1, b.pair, bg, b.x, b.y, z);
+#pragma unroll
+        for (int i = 0; i < C; ++i) g.v[i] += o.v[i] + t.v[i];
+      }
+      if constexpr (NEG) {
+        const Vec<C> r = read_linear<C, BM>(src, a.d, px, py, pz);
+#pragma unroll
+        for (int i = 0; i < C; ++i)
+          if (!(r.v[i] >= 0.f)) g.v[i] = 0.f;
+      }
+      scatter_linear_tile<C, BM>(tile, b.ox, b.oy, b.oz, ovp, a.d, px, py, pz, g);
+      Vec<C> dx, dy, dz;
+      read_gradient<C, BM>(src, a.d, px, py, pz, dx, dy, dz);
+      add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, a.ts * vdot(g, dx), a.ts * vdot(g, dy),
+                   a.ts * vdot(g, dz), vel_mode);
+    }
+  }
+  tile_store<C>(tile, tiles2, bg, b);
+}
+
+// g_rho(n) += sum over the pairs of n of r_dir (order 2) + the B2 tiles.
+template <int C>
+__global__ __launch_bounds__(256) void advect_rho_gather_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ r_dir,
+                                                                const float* __restrict__ tiles2,
+                                                                float* __restrict__ g_rho) {
+  const int x = blockIdx.x * kBX + threadIdx.x;
+  const int y = blockIdx.y * kBY + threadIdx.y;
+  const int z = blockIdx.z % a.d.z;
+  const int n = blockIdx.z / a.d.z;
+  if (x >= a.d.x || y >= a.d.y) return;
+  const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
+  const int64_t cell = cell_offset(a.d, x, y, z);
+  Vec<C> s = vload<C>(g_rho + (n * cells + cell) * C);
+  const int np = a.separate ? a.n_vel : 1;
+  for (int j = 0; j < np; ++j) {
+    const int pair = a.separate ? n * a.n_vel + j : n;
+    if (r_dir) {
+      const Vec<C> r = vload<C>(r_dir + (pair * cells + cell) * C);
+#pragma unroll
+      for (int i = 0; i < C; ++i) s.v[i] += r.v[i];
+    }
+    const Vec<C> t = gather_tiles<C>(tiles2, pair, bg, x, y, z);
+#pragma unroll
+    for (int i = 0; i < C; ++i) s.v[i] += t.v[i];
+  }
+  vstore<C>(g_rho + (n * cells + cell) * C, s);
+}
+
 #define NGT_SWITCH_C(C_, ...)                              \
   switch (C_) {                                            \
     case 1: { constexpr int CC = 1; __VA_ARGS__; } break;  \
     case 2: { constexpr int CC = 2; __VA_ARGS__; } break;  \
     case 3: { constexpr int CC = 3; __VA_ARGS__; } break;  \
     default: { constexpr int CC = 4; __VA_ARGS__; } break; \
+  }
+#define NGT_SWITCH_C12(C_, ...)                           \
+  switch (C_) {                                           \
+    case 1: { constexpr int CC = 1; __VA_ARGS__; } break; \
+    default: { constexpr int CC = 2; __VA_ARGS__; } break; \
   }
 #define NGT_SWITCH_BM(B_, ...)                                      \
   switch (B_) {                                                     \
@@ -364,6 +651,14 @@ dim3 adv_grid(const AdvPlan& p) {
   return dim3(blocks_for(p.a.d.x, kBX), blocks_for(p.a.d.y, kBY), static_cast<unsigned>(p.a.d.z * p.pairs));
 }
 
+void launch_faces(const AdvPlan& p, const float* g_vc, float* gx, float* gy, float* gz, hipStream_t s) {
+  const Dims d = p.a.d;
+  const int64_t faces = (static_cast<int64_t>(d.z) * d.y * (d.x + 1) + static_cast<int64_t>(d.z) * (d.y + 1) * d.x +
+                         static_cast<int64_t>(d.z + 1) * d.y * d.x) * p.a.n_vel;
+  hipLaunchKernelGGL(faces_from_centered_grad_kernel, dim3(blocks_for(faces, 256)), dim3(256), 0, s, d, p.a.n_vel,
+                     g_vc, gx, gy, gz);
+}
+
 }  // namespace
 }  // namespace ngt
 
@@ -382,8 +677,16 @@ size_t ngt_advect_workspace_size(const int64_t* input_shape, int num_velocities,
   if (backward) {
     floats += pairs * cells * C;                         // g_sl
     floats += static_cast<int64_t>(num_velocities) * cells * 3;  // centered velocity gradient
-    floats += pairs * cells * C;                         // reference: scatter staging
-    floats += pairs * cells;                             // reference: hit counts (uint32)
+    const int64_t ref_extra = pairs * cells * C          // reference: scatter staging
+                              + pairs * cells;           // reference: hit counts (uint32)
+    int64_t brick_extra = 0;
+    if (C <= 2) {
+      const Dims d{static_cast<int>(input_shape[3]), static_cast<int>(input_shape[2]),
+                   static_cast<int>(input_shape[1])};
+      const int64_t tiles = pairs * brick_grid(d).count() * static_cast<int64_t>(kTileCells) * C;
+      brick_extra = tiles + (order == 2 ? 2 * pairs * cells * C + tiles : 0);  // tiles2 (+ r_dir, ovf1, tiles1)
+    }
+    floats += std::max(ref_extra, brick_extra);
   }
   return static_cast<size_t>(floats) * sizeof(float) + 256;
 }
@@ -443,14 +746,47 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
   uint32_t* hits = reinterpret_cast<uint32_t*>(tmp + field);
   if (!p.a.staggered) g_vc = vel_x_grad;  // centered: accumulate in place
 
-  fill_zero(input_grad, p.a.n_data * p.cells * p.C, s);
-  fill_zero(g_vc, static_cast<int64_t>(p.a.n_vel) * p.cells * 3, s);
   const float* slp = sl;
   if (order == 2 && slp == nullptr) {
     NGT_SWITCH_C(p.C, NGT_SWITCH_BM(boundary,
         hipLaunchKernelGGL((advect_sl_kernel<CC, BB, false>), grid, block, 0, s, p.a, input, sl_buf)));
     slp = sl_buf;
   }
+  fill_zero(input_grad, p.a.n_data * p.cells * p.C, s);
+  if (!ref && p.C <= 2) {
+    // brick-tiled exact adjoint: workspace after g_vc = [tiles2 | r_dir | ovf1 | tiles1]
+    const BrickGrid bg = brick_grid(p.a.d);
+    const int64_t tiles_n = static_cast<int64_t>(p.pairs) * bg.count() * kTileCells * p.C;
+    float* tiles2 = tmp;
+    float* r_dir = tiles2 + tiles_n;
+    float* ovf1 = r_dir + field;
+    float* tiles1 = ovf1 + field;
+    const bool shared_v = p.a.separate && p.a.n_data > 1;
+    const int vel_mode = shared_v ? 2 : (order == 2 ? 1 : 0);
+    if (shared_v) fill_zero(g_vc, static_cast<int64_t>(p.a.n_vel) * p.cells * 3, s);
+    const dim3 bgrid(static_cast<unsigned>(bg.count() * p.pairs)), bblock(kBrickX, kBrickY);
+    const bool neg = order == 1 && clamp == NGT_CLAMP_NEGATIVE;
+    if (order == 2) {
+      fill_zero(ovf1, field, s);
+      NGT_SWITCH_C12(p.C, NGT_SWITCH_BM(boundary, NGT_SWITCH_CLAMP(clamp,
+          hipLaunchKernelGGL((advect_mc_bwd_brick_kernel<CC, BB, KK>), bgrid, bblock, 0, s, p.a, bg, input, slp,
+                             output_grad, g_sl, r_dir, ovf1, tiles1, g_vc, vel_mode);
+          hipLaunchKernelGGL((advect_sl_bwd_brick_kernel<CC, BB, false, true>), bgrid, bblock, 0, s, p.a, bg, input,
+                             g_sl, ovf1, tiles1, input_grad, tiles2, g_vc, vel_mode))));
+    } else {
+      NGT_SWITCH_C12(p.C, NGT_SWITCH_BM(boundary,
+          if (neg) hipLaunchKernelGGL((advect_sl_bwd_brick_kernel<CC, BB, true, false>), bgrid, bblock, 0, s, p.a, bg,
+                                      input, output_grad, nullptr, nullptr, input_grad, tiles2, g_vc, vel_mode);
+          else hipLaunchKernelGGL((advect_sl_bwd_brick_kernel<CC, BB, false, false>), bgrid, bblock, 0, s, p.a, bg,
+                                  input, output_grad, nullptr, nullptr, input_grad, tiles2, g_vc, vel_mode)));
+    }
+    const dim3 dgrid(blocks_for(p.a.d.x, kBX), blocks_for(p.a.d.y, kBY), static_cast<unsigned>(p.a.d.z * p.a.n_data));
+    NGT_SWITCH_C12(p.C, hipLaunchKernelGGL((advect_rho_gather_kernel<CC>), dgrid, block, 0, s, p.a, bg,
+                                           order == 2 ? r_dir : nullptr, tiles2, input_grad));
+    if (p.a.staggered) launch_faces(p, g_vc, vel_x_grad, vel_y_grad, vel_z_grad, s);
+    return check_launch("ngt_advect_bwd");
+  }
+  fill_zero(g_vc, static_cast<int64_t>(p.a.n_vel) * p.cells * 3, s);
   const float* gsl_in = output_grad;
   if (order == 2) {
     fill_zero(g_sl, field, s);
@@ -488,13 +824,7 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
   if (ref) {
     NGT_SWITCH_C(p.C, hipLaunchKernelGGL((add_normalized_kernel<CC>), grid, block, 0, s, p.a, tmp, hits, input_grad, 1));
   }
-  if (p.a.staggered) {
-    const Dims d = p.a.d;
-    const int64_t faces = (static_cast<int64_t>(d.z) * d.y * (d.x + 1) + static_cast<int64_t>(d.z) * (d.y + 1) * d.x +
-                           static_cast<int64_t>(d.z + 1) * d.y * d.x) * p.a.n_vel;
-    hipLaunchKernelGGL(faces_from_centered_grad_kernel, dim3(blocks_for(faces, 256)), dim3(256), 0, s, d, p.a.n_vel,
-                       g_vc, vel_x_grad, vel_y_grad, vel_z_grad);
-  }
+  if (p.a.staggered) launch_faces(p, g_vc, vel_x_grad, vel_y_grad, vel_z_grad, s);
   return check_launch("ngt_advect_bwd");
 }
 

@@ -313,3 +313,39 @@ def test_error_messages(ngt_ops):
         ngt_ops.sample_grid_lut(x, lut)
     with pytest.raises(ValueError, match="camera batch must match data batch"):
         ngt_ops.sample_grid_lut(torch.zeros((2, 4, 4, 4, 1), device="cuda:0"), lut, separate_camera_batch=False)
+
+
+@pytest.mark.parametrize("vmax,C,separate", [(5.0, 1, False), (1.5, 2, True), (4.0, 2, True), (1.5, 4, False)])
+def test_advect_overflow_channels_batches(ngt_ops, vmax, C, separate):
+    """Displacements beyond the backward's LDS tile halo (vmax 4-5 cells),
+    2 / 4 channels, and separate velocity batches where several data entries
+    share a velocity (N=2 x V=2 pairs)."""
+    rng = np.random.default_rng(int(vmax * 10) + C)
+    D, H, W = 9, 17, 37
+    N, V = (2, 2) if separate else (1, 1)
+    rho = f32(np.stack([np.stack([np.clip(smooth_field(rng, (D, H, W), 1.0), 0, 1) for _ in range(C)], -1)
+                        for _ in range(N)]))
+    vx = f32(np.stack([smooth_field(rng, (D, H, W + 1), vmax) for _ in range(V)]))[..., None]
+    vy = f32(np.stack([smooth_field(rng, (D, H + 1, W), vmax) for _ in range(V)]))[..., None]
+    vz = f32(np.stack([smooth_field(rng, (D + 1, H, W), vmax) for _ in range(V)]))[..., None]
+    vel = tuple(dev(c).requires_grad_() for c in (vx, vy, vz))
+    rt = dev(rho).requires_grad_()
+    out = ngt_ops.advect_grid_semi_langrange(rt, vel, timestep=-1.0, order=2, clamp_extrema="MC_SMOOTH",
+                                             boundary="BORDER", separate_velocity_batch=separate)
+    pairs = [(n, v, v) for n in range(N) for v in range(V)] if separate else [(n, n, 0) for n in range(N)]
+    g = f32(rng.standard_normal(out.shape))
+    out.backward(dev(g))
+    eg_rho = np.zeros_like(rho)
+    eg_v = [np.zeros_like(c) for c in (vx, vy, vz)]
+    for n, v, slot in pairs:
+        vc = A.centered_from_staggered(vx[v], vy[v], vz[v])
+        exp = A.warp_forward(rho[n], vc, 1.0, 2, "MC_SMOOTH", BORDER)
+        got = host(out)[n, slot]
+        assert rel(got, exp) < FWD_TOL
+        gr, gv = A.warp_backward(rho[n], vc, g[n, slot], 1.0, 2, "MC_SMOOTH", BORDER)
+        eg_rho[n] += gr
+        for k, e in enumerate(A.centered_from_staggered_adjoint(gv)):
+            eg_v[k][v] += e
+    assert rel(host(rt.grad), eg_rho) < GRAD_TOL
+    for t, e in zip(vel, eg_v):
+        assert rel(host(t.grad), e) < GRAD_TOL
