@@ -45,7 +45,94 @@ def parse():
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--graph", action="store_true", help="replay the fused step as a captured HIP graph")
+    p.add_argument("--traffic-probe", action="store_true",
+                   help="run a 256 MiB copy and each op of the step once (for rocprofv3 --pmc FETCH_SIZE / "
+                        "WRITE_SIZE runs) and exit")
+    p.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                   help="turn the two counter_collection.csv files of the --traffic-probe runs into the "
+                        "per-kernel / per-op HBM traffic JSON (stdout) and exit")
     return p.parse_args()
+
+
+# kernels launched by each op of the step (substring of the kernel name)
+OP_KERNELS = {
+    "advect_fwd": ["advect_sl_kernel", "advect_mc_kernel"],
+    "advect_bwd": ["advect_mc_bwd_brick_kernel", "advect_sl_bwd_brick_kernel", "advect_rho_gather_kernel",
+                   "faces_from_centered_grad_kernel", "fill_kernel"],
+    "render_fwd": ["raymarch_fwd_kernel"],
+    "render_bwd": ["raymarch_bwd_prepass_kernel", "raymarch_bwd_brick_kernel"],
+}
+COPY_BYTES = 256 << 20
+
+
+def traffic_probe():
+    """One 256 MiB device copy (the calibration kernel: reads and writes
+    COPY_BYTES), then every op of the step twice (the second call of each is
+    the measured dispatch)."""
+    import torch
+    from ngt import _lib
+    _lib.load()
+    dev = torch.device("cuda:0")
+    x = torch.rand(COPY_BYTES // 4, device=dev)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    torch.mul(x, 1.0, out=y)  # an elementwise kernel reading and writing COPY_BYTES
+    torch.cuda.synchronize()
+    del x, y
+    prob = build_problem(torch, dev)
+    calls = op_calls(torch, prob)
+    for name in ("advect_fwd", "advect_bwd", "render_fwd", "render_bwd"):
+        for _ in range(2):
+            calls[name]()
+        torch.cuda.synchronize()
+        print("probe: %s done" % name, file=sys.stderr, flush=True)
+
+
+def _read_counters(path, counter):
+    """{kernel name: [value per dispatch]} of one counter from a rocprofv3
+    counter_collection.csv."""
+    import csv
+    out = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r.get("Counter_Name") != counter:
+                continue
+            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return out
+
+
+def traffic_summary(fetch_csv, write_csv):
+    fetch = _read_counters(fetch_csv, "FETCH_SIZE")
+    write = _read_counters(write_csv, "WRITE_SIZE")
+    copy_f = [v for k, vs in fetch.items() if "elementwise" in k for v in vs]
+    copy_w = [v for k, vs in write.items() if "elementwise" in k for v in vs]
+    # the calibration copy is the largest elementwise dispatch
+    scale_f = COPY_BYTES / (max(copy_f) * 1024.0)
+    scale_w = COPY_BYTES / (max(copy_w) * 1024.0)
+    kernels = {}
+    for k in fetch:
+        if k not in write:
+            continue
+        # the last dispatch of each kernel is the warm one
+        kernels[k if len(k) <= 160 else k[:157] + "..."] = {"fetch_kib": fetch[k][-1], "write_kib": write[k][-1],
+                      "hbm_bytes": fetch[k][-1] * 1024.0 * scale_f + write[k][-1] * 1024.0 * scale_w}
+    per_op = {}
+    for op, names in OP_KERNELS.items():
+        tot = 0.0
+        for k, v in kernels.items():
+            if any(n in k for n in names) and "raymarch_bwd_exact" not in k:
+                tot += v["hbm_bytes"] * (2 if (op == "advect_bwd" and "fill_kernel" in k) else 1)
+        per_op[op] = tot
+    cmd = ("export TMPDIR=/tmp; rocprofv3 --pmc FETCH_SIZE --output-format csv -d gpurun_out/pmc_fetch -o f -- "
+           "python -u bench.py --traffic-probe; rocprofv3 --pmc WRITE_SIZE --output-format csv -d gpurun_out/pmc_write "
+           "-o w -- python -u bench.py --traffic-probe; python bench.py --traffic-summary "
+           "gpurun_out/pmc_fetch/f_counter_collection.csv gpurun_out/pmc_write/w_counter_collection.csv")
+    return {"command": cmd,
+            "note": "FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the warm dispatch of each kernel, scaled by "
+                    "COPY_BYTES / counter of a 256 MiB elementwise copy; bytes fetched into the L2s (each XCD's "
+                    "L2 fetches separately; MALL hits count as fetched)",
+            "calibration": {"copy_bytes": COPY_BYTES, "fetch_scale": scale_f, "write_scale": scale_w},
+            "kernels": kernels, "hbm_bytes_per_launch": per_op}
 
 
 def build_problem(torch, dev):
@@ -147,6 +234,14 @@ def time_call(torch, call, iters):
 
 def op_breakdown(torch, prob, iters):
     """Per-op call durations of the fused step (direct C-ABI calls)."""
+    calls = op_calls(torch, prob)
+    calls["render_fwd"]()
+    return {k: time_call(torch, c, iters) for k, c in calls.items()}
+
+
+def op_calls(torch, prob):
+    """The four ops of the fused step as direct C-ABI calls on preallocated
+    buffers."""
     from ngt import _lib, ops
     lib = _lib.load()
     st = lambda: ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -186,7 +281,7 @@ def op_breakdown(torch, prob, iters):
                  (img, VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * 4), (dimg, VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * 4)):
         assert t.numel() == n, (tuple(t.shape), n)
     calls["render_fwd"]()
-    return {k: time_call(torch, c, iters) for k, c in calls.items()}
+    return calls
 
 
 def cpu_baseline():
@@ -225,9 +320,15 @@ def cpu_baseline():
 
 def main():
     args = parse()
+    if args.traffic_summary:
+        print(json.dumps(traffic_summary(*args.traffic_summary), indent=1))
+        return
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only; sharding the step across GPUs is not implemented. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import torch
     from ngt import ops, _lib
@@ -296,10 +397,10 @@ def main():
     step_bytes = cells * (20 + 36 + 4 + 8) + pixels * (16 + 32)
     best_ms = ms_graph if ms_graph is not None else ms
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "%s_128_traffic.json" % dominant)
+    tpath = os.path.join(ROOT, "profiles", "traffic_step128.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f)["hbm_bytes_per_launch"].get(dominant)
     rec = {
         "metric": "fwd+bwd voxel-updates/s + ray-samples/s at 128^3 (1 GPU), % HBM peak, vs plain",
         "value": cells / (best_ms * 1e-3),
@@ -323,7 +424,8 @@ def main():
         "ms_per_step_graph": ms_graph,
         "ms_per_step_plain": ms_plain,
         "vs_plain": (ms_plain / best_ms) if ms_plain else None,
-        "roofline": {"kernel": "ngt_" + dominant.replace("render", "render_density").replace("advect_", "advect_"),
+        "roofline": {"kernel": "ngt_" + dominant.replace("render", "render_density") + " (" +
+                               " + ".join(OP_KERNELS[dominant]) + ")",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "ms_per_launch": ms_kernel,
                      "algorithmic_bytes_per_launch": kbytes},
