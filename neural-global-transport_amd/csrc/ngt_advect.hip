@@ -463,6 +463,7 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_brick_kernel(AdvArgs a, Bri
   float* ovp = ovf + b.pair * cells * C;
   if (b.x < a.d.x && b.y < a.d.y) {
     const int z1 = min(b.z0 + kBrickZ, a.d.z);
+#pragma unroll 2
     for (int z = b.z0; z < z1; ++z) {
       float ux, uy, uz;
       load_velocity(a, v, b.x, b.y, z, ux, uy, uz);
@@ -528,6 +529,7 @@ __global__ __launch_bounds__(256) void advect_sl_bwd_brick_kernel(AdvArgs a, Bri
   float* ovp = g_rho + n * cells * C;
   if (b.x < a.d.x && b.y < a.d.y) {
     const int z1 = min(b.z0 + kBrickZ, a.d.z);
+#pragma unroll 2
     for (int z = b.z0; z < z1; ++z) {
       float ux, uy, uz;
       load_velocity(a, v, b.x, b.y, z, ux, uy, uz);

@@ -142,12 +142,13 @@ NGT_DEVICE Corners make_corners(const Dims& d, float qx, float qy, float qz) {
   c.x0 = static_cast<int>(fx); c.y0 = static_cast<int>(fy); c.z0 = static_cast<int>(fz);
   c.x1 = static_cast<int>(ceilf(qx)); c.y1 = static_cast<int>(ceilf(qy)); c.z1 = static_cast<int>(ceilf(qz));
   if constexpr (BM == BORDER) {
-    c.inside = (qx + 0.5f >= 0.f) && (qx + 0.5f < static_cast<float>(d.x)) &&
-               (qy + 0.5f >= 0.f) && (qy + 0.5f < static_cast<float>(d.y)) &&
-               (qz + 0.5f >= 0.f) && (qz + 0.5f < static_cast<float>(d.z));
-    c.vx0 = c.x0 > -1 && c.x0 < d.x; c.vx1 = c.x1 > -1 && c.x1 < d.x;
-    c.vy0 = c.y0 > -1 && c.y0 < d.y; c.vy1 = c.y1 > -1 && c.y1 < d.y;
-    c.vz0 = c.z0 > -1 && c.z0 < d.z; c.vz1 = c.z1 > -1 && c.z1 < d.z;
+    // bitwise & keeps the masks branch-free
+    c.inside = (qx + 0.5f >= 0.f) & (qx + 0.5f < static_cast<float>(d.x)) &
+               (qy + 0.5f >= 0.f) & (qy + 0.5f < static_cast<float>(d.y)) &
+               (qz + 0.5f >= 0.f) & (qz + 0.5f < static_cast<float>(d.z));
+    c.vx0 = (c.x0 > -1) & (c.x0 < d.x); c.vx1 = (c.x1 > -1) & (c.x1 < d.x);
+    c.vy0 = (c.y0 > -1) & (c.y0 < d.y); c.vy1 = (c.y1 > -1) & (c.y1 < d.y);
+    c.vz0 = (c.z0 > -1) & (c.z0 < d.z); c.vz1 = (c.z1 > -1) & (c.z1 < d.z);
   } else {
     c.inside = true;
     c.vx0 = c.vx1 = c.vy0 = c.vy1 = c.vz0 = c.vz1 = true;
@@ -171,20 +172,64 @@ struct Corner8 {
   bool ok[8];
 };
 
+// The 8 corner loads, branch-free: issued unconditionally from in-grid
+// addresses (BORDER corners clamped into the grid for the address only; the
+// out-of-grid ones are zeroed by mask8 afterwards), so the loads of several
+// samples can be in flight together.  Offsets are 32-bit (a grid slice holds
+// fewer than 2^32 floats, checked on the host).
+template <int C, int BM>
+NGT_DEVICE void fetch8_raw(const float* __restrict__ g, const Dims& d, const Corners& c, Vec<C> (&v)[8]) {
+  int x0 = c.x0, x1 = c.x1, y0 = c.y0, y1 = c.y1, z0 = c.z0, z1 = c.z1;
+  if constexpr (BM == BORDER) {
+    x0 = min(max(x0, 0), d.x - 1); x1 = min(max(x1, 0), d.x - 1);
+    y0 = min(max(y0, 0), d.y - 1); y1 = min(max(y1, 0), d.y - 1);
+    z0 = min(max(z0, 0), d.z - 1); z1 = min(max(z1, 0), d.z - 1);
+  }
+  const uint32_t W = static_cast<uint32_t>(d.x), H = static_cast<uint32_t>(d.y);
+  const uint32_t rows[4] = {(static_cast<uint32_t>(z0) * H + static_cast<uint32_t>(y0)) * W,
+                            (static_cast<uint32_t>(z0) * H + static_cast<uint32_t>(y1)) * W,
+                            (static_cast<uint32_t>(z1) * H + static_cast<uint32_t>(y0)) * W,
+                            (static_cast<uint32_t>(z1) * H + static_cast<uint32_t>(y1)) * W};
+  const uint32_t xs[2] = {static_cast<uint32_t>(x0), static_cast<uint32_t>(x1)};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = vload<C>(g + (rows[k >> 1] + xs[k & 1]) * static_cast<uint32_t>(C));
+}
+
+NGT_DEVICE bool corner_ok(const Corners& c, int k) {
+  return ((k & 1) ? c.vx1 : c.vx0) & ((k & 2) ? c.vy1 : c.vy0) & ((k & 4) ? c.vz1 : c.vz0);
+}
+
+template <int C, int BM>
+NGT_DEVICE void mask8(const Corners& c, Vec<C> (&v)[8]) {
+  if constexpr (BM == BORDER) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const bool ok = corner_ok(c, k);
+#pragma unroll
+      for (int i = 0; i < C; ++i) v[k].v[i] = ok ? v[k].v[i] : 0.f;
+    }
+  }
+}
+
 template <int C, int BM>
 NGT_DEVICE Corner8<C> fetch8(const float* __restrict__ g, const Dims& d, const Corners& c) {
   Corner8<C> r;
+  fetch8_raw<C, BM>(g, d, c, r.v);
+  mask8<C, BM>(c, r.v);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int ix = (k & 1) ? c.x1 : c.x0;
-    const int iy = (k & 2) ? c.y1 : c.y0;
-    const int iz = (k & 4) ? c.z1 : c.z0;
-    bool ok = true;
-    if constexpr (BM == BORDER) {
-      ok = ((k & 1) ? c.vx1 : c.vx0) && ((k & 2) ? c.vy1 : c.vy0) && ((k & 4) ? c.vz1 : c.vz0);
-    }
-    r.ok[k] = ok;
-    r.v[k] = ok ? vload<C>(g + cell_offset(d, ix, iy, iz) * C) : vzero<C>();
+  for (int k = 0; k < 8; ++k) r.ok[k] = BM == BORDER ? corner_ok(c, k) : true;
+  return r;
+}
+
+// Trilinear value from masked corners; 0 outside the BORDER domain.
+template <int C, int BM>
+NGT_DEVICE Vec<C> lerp8(const Vec<C> (&v)[8], const Corners& c) {
+  const Vec<C> v0 = vlerp(vlerp(v[0], v[1], c.wx), vlerp(v[2], v[3], c.wx), c.wy);
+  const Vec<C> v1 = vlerp(vlerp(v[4], v[5], c.wx), vlerp(v[6], v[7], c.wx), c.wy);
+  Vec<C> r = vlerp(v0, v1, c.wz);
+  if constexpr (BM == BORDER) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) r.v[i] = c.inside ? r.v[i] : 0.f;
   }
   return r;
 }
@@ -192,13 +237,27 @@ NGT_DEVICE Corner8<C> fetch8(const float* __restrict__ g, const Dims& d, const C
 template <int C, int BM>
 NGT_DEVICE Vec<C> read_linear(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz) {
   const Corners c = make_corners<BM>(d, qx, qy, qz);
-  if constexpr (BM == BORDER) {
-    if (!c.inside) return vzero<C>();
+  Vec<C> v[8];
+  fetch8_raw<C, BM>(g, d, c, v);
+  mask8<C, BM>(c, v);
+  return lerp8<C, BM>(v, c);
+}
+
+// U trilinear reads whose 8*U loads are all issued before any is consumed.
+template <int C, int BM, int U>
+NGT_DEVICE void read_linear_batch(const float* __restrict__ g, const Dims& d, const float (&qx)[U],
+                                  const float (&qy)[U], const float (&qz)[U], Vec<C> (&out)[U]) {
+  Corners c[U];
+  Vec<C> v[U][8];
+#pragma unroll
+  for (int u = 0; u < U; ++u) c[u] = make_corners<BM>(d, qx[u], qy[u], qz[u]);
+#pragma unroll
+  for (int u = 0; u < U; ++u) fetch8_raw<C, BM>(g, d, c[u], v[u]);
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    mask8<C, BM>(c[u], v[u]);
+    out[u] = lerp8<C, BM>(v[u], c[u]);
   }
-  const Corner8<C> k = fetch8<C, BM>(g, d, c);
-  const Vec<C> v0 = vlerp(vlerp(k.v[0], k.v[1], c.wx), vlerp(k.v[2], k.v[3], c.wx), c.wy);
-  const Vec<C> v1 = vlerp(vlerp(k.v[4], k.v[5], c.wx), vlerp(k.v[6], k.v[7], c.wx), c.wy);
-  return vlerp(v0, v1, c.wz);
 }
 
 // Nearest cell: int(q + 0.5) truncates toward zero (sampling.hpp:570-584).
@@ -236,19 +295,24 @@ NGT_DEVICE Vec<C> read_minmax(const float* __restrict__ g, const Dims& d, float 
     for (int i = 0; i < C; ++i) sel[i] = -1;
   }
   if constexpr (BM == BORDER) {
-    if (!c.inside) return vzero<C>();
     if (!(c.vx0 && c.vx1 && c.vy0 && c.vy1 && c.vz0 && c.vz1)) acc = vzero<C>();
   }
   const Corner8<C> k = fetch8<C, BM>(g, d, c);
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    if (!k.ok[j]) continue;
 #pragma unroll
     for (int i = 0; i < C; ++i) {
-      const bool better = IS_MAX ? (k.v[j].v[i] > acc.v[i]) : (k.v[j].v[i] < acc.v[i]);
-      if (better) {
-        acc.v[i] = k.v[j].v[i];
-        if (sel) sel[i] = j;
+      const bool better = k.ok[j] && (IS_MAX ? (k.v[j].v[i] > acc.v[i]) : (k.v[j].v[i] < acc.v[i]));
+      acc.v[i] = better ? k.v[j].v[i] : acc.v[i];
+      if (sel) sel[i] = better ? j : sel[i];
+    }
+  }
+  if constexpr (BM == BORDER) {
+    if (!c.inside) {
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        acc.v[i] = 0.f;
+        if (sel) sel[i] = -1;
       }
     }
   }
@@ -269,12 +333,6 @@ template <int C, int BM>
 NGT_DEVICE void read_gradient(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz,
                               Vec<C>& dx, Vec<C>& dy, Vec<C>& dz) {
   const Corners c = make_corners<BM>(d, qx, qy, qz);
-  if constexpr (BM == BORDER) {
-    if (!c.inside) {
-      dx = dy = dz = vzero<C>();
-      return;
-    }
-  }
   const Corner8<C> k = fetch8<C, BM>(g, d, c);
   dx = vlerp(vlerp(vsub(k.v[1], k.v[0]), vsub(k.v[3], k.v[2]), c.wy),
              vlerp(vsub(k.v[5], k.v[4]), vsub(k.v[7], k.v[6]), c.wy), c.wz);
@@ -282,6 +340,14 @@ NGT_DEVICE void read_gradient(const float* __restrict__ g, const Dims& d, float 
              vlerp(vsub(k.v[6], k.v[4]), vsub(k.v[7], k.v[5]), c.wx), c.wz);
   dz = vlerp(vlerp(vsub(k.v[4], k.v[0]), vsub(k.v[5], k.v[1]), c.wx),
              vlerp(vsub(k.v[6], k.v[2]), vsub(k.v[7], k.v[3]), c.wx), c.wy);
+  if constexpr (BM == BORDER) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      dx.v[i] = c.inside ? dx.v[i] : 0.f;
+      dy.v[i] = c.inside ? dy.v[i] : 0.f;
+      dz.v[i] = c.inside ? dz.v[i] : 0.f;
+    }
+  }
 }
 
 // Scatter ``val`` with trilinear weights to the 8 corners.

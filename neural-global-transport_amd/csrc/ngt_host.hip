@@ -42,8 +42,8 @@ int check_grid_shape(const int64_t* s, const char* what, bool allow_c3) {
     set_error("Only 1,2 and 4 Channel supported.");
     return NGT_ERR_UNIMPLEMENTED;
   }
-  if (s[1] > (1 << 24) || s[2] > (1 << 24) || s[3] > (1 << 24)) {
-    set_error(std::string(what) + " spatial size too large");
+  if (s[1] > (1 << 24) || s[2] > (1 << 24) || s[3] > (1 << 24) || s[1] * s[2] * s[3] * s[4] >= (int64_t(1) << 32)) {
+    set_error(std::string(what) + " spatial size too large (a grid slice must hold fewer than 2^32 values)");
     return NGT_ERR_INVALID_ARGUMENT;
   }
   return NGT_OK;

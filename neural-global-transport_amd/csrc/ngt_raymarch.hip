@@ -38,6 +38,9 @@ struct RayArgs {
 
 NGT_DEVICE int kMaxRaySteps(const RayArgs& a) { return 16 * (a.din.x + a.din.y + a.din.z + a.out_d); }
 
+// Ray samples whose gathers are issued together.
+constexpr int kUnroll = 4;
+
 struct Ray {
   float sx, sy, sz;  // start
   float ex, ey, ez;  // end
@@ -118,6 +121,32 @@ NGT_DEVICE Vec<C> sample_cell(const RayArgs& a, const float* __restrict__ src, f
   }
 }
 
+// kUnroll cell values at once; for LINEAR all their corner loads are issued
+// before any is used.
+template <int C, int FM, int BM, bool LIGHT>
+NGT_DEVICE void sample_cells(const RayArgs& a, const float* __restrict__ src, const float (&qx)[kUnroll],
+                             const float (&qy)[kUnroll], const float (&qz)[kUnroll], Vec<C> (&cells)[kUnroll]) {
+  if constexpr (FM == LINEAR) {
+    constexpr int CIN = LIGHT ? 1 : C;
+    Vec<CIN> raw[kUnroll];
+    read_linear_batch<CIN, BM, kUnroll>(src, a.din, qx, qy, qz, raw);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if constexpr (LIGHT) {
+        const float s = raw[u].v[0] * a.depth_step;
+        cells[u].v[0] = a.light_r * s; cells[u].v[1] = a.light_g * s; cells[u].v[2] = a.light_b * s;
+        cells[u].v[3] = s;
+      } else {
+#pragma unroll
+        for (int i = 0; i < C; ++i) cells[u].v[i] = raw[u].v[i];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) cells[u] = sample_cell<C, FM, BM, LIGHT>(a, src, qx[u], qy[u], qz[u]);
+  }
+}
+
 // Scatter a cell gradient at a sample position (exact adjoint of sample_cell).
 template <int C, int FM, int BM, bool LIGHT>
 NGT_DEVICE void scatter_cell(const RayArgs& a, const float* __restrict__ src, float* __restrict__ g, float qx, float qy,
@@ -141,7 +170,22 @@ __global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch c
   const float* src = a.input + static_cast<int64_t>(cp.data_index) * a.din.x * a.din.y * a.din.z * CIN;
   const Ray r = make_ray<BM>(a, cp, x, y);
   Vec<C> acc = vzero<C>();
-  for (int k = 0; k < r.steps; ++k) {
+  // Samples are fetched kUnroll at a time so their gathers are in flight
+  // together; the blend recurrence then consumes them in order.
+  int k = 0;
+  for (; k + kUnroll <= r.steps; k += kUnroll) {
+    float qx[kUnroll], qy[kUnroll], qz[kUnroll];
+    Vec<C> cells[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const float kf = static_cast<float>(k + u);
+      qx[u] = fmaf(kf, r.dx, r.sx) - 0.5f; qy[u] = fmaf(kf, r.dy, r.sy) - 0.5f; qz[u] = fmaf(kf, r.dz, r.sz) - 0.5f;
+    }
+    sample_cells<C, FM, BM, LIGHT>(a, src, qx, qy, qz, cells);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) blend_step<C, MODE>(acc, cells[u]);
+  }
+  for (; k < r.steps; ++k) {
     const float kf = static_cast<float>(k);
     const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
     blend_step<C, MODE>(acc, sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz));
@@ -181,14 +225,12 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
       for (int i = 0; i < C - 1; ++i) total += L.v[i] * g.v[i];
     }
     float prefix = 0.f, dacc = 0.f;
-    for (int k = 0; k < r.steps; ++k) {
-      const float kf = static_cast<float>(k);
-      const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
+    auto step = [&](int k, float qx, float qy, float qz, const Vec<C>& cell) {
       Vec<C> gc;
       if constexpr (C == 1 || MODE == ADDITIVE) {
         gc = g;
+        (void)cell;
       } else {
-        const Vec<C> cell = sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz);
         float lg = 0.f;
 #pragma unroll
         for (int i = 0; i < C - 1; ++i) lg += cell.v[i] * g.v[i];
@@ -207,6 +249,26 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
         }
       }
       sink(k, qx, qy, qz, gc);
+    };
+    constexpr bool kNeedCell = !(C == 1 || MODE == ADDITIVE);
+    int k = 0;
+    for (; k + kUnroll <= r.steps; k += kUnroll) {
+      float qx[kUnroll], qy[kUnroll], qz[kUnroll];
+      Vec<C> cells[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const float kf = static_cast<float>(k + u);
+        qx[u] = fmaf(kf, r.dx, r.sx) - 0.5f; qy[u] = fmaf(kf, r.dy, r.sy) - 0.5f; qz[u] = fmaf(kf, r.dz, r.sz) - 0.5f;
+        cells[u] = vzero<C>();
+      }
+      if constexpr (kNeedCell) sample_cells<C, FM, BM, LIGHT>(a, src, qx, qy, qz, cells);
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) step(k + u, qx[u], qy[u], qz[u], cells[u]);
+    }
+    for (; k < r.steps; ++k) {
+      const float kf = static_cast<float>(k);
+      const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
+      step(k, qx, qy, qz, kNeedCell ? sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz) : vzero<C>());
     }
   }
 }
@@ -376,6 +438,8 @@ __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamB
 
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
   const float fdx = static_cast<float>(a.din.x), fdy = static_cast<float>(a.din.y), fdz = static_cast<float>(a.din.z);
+  const float fb0x = static_cast<float>(b0x), fb0y = static_cast<float>(b0y), fb0z = static_cast<float>(b0z);
+  const float fb1x = static_cast<float>(b1x), fb1y = static_cast<float>(b1y), fb1z = static_cast<float>(b1z);
   const float lo3[3] = {static_cast<float>(b0x) - 0.5f, static_cast<float>(b0y) - 0.5f, static_cast<float>(b0z) - 0.5f};
   const float hi3[3] = {static_cast<float>(b1x) + 1.5f, static_cast<float>(b1y) + 1.5f, static_cast<float>(b1z) + 1.5f};
   const int total = ray_begin[cb.count];
@@ -408,31 +472,44 @@ __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamB
     const int k0 = max(0, static_cast<int>(floorf(fminf(fmaxf(lo, -1.f), fsteps))) - 1);
     const int k1 = min(rr.steps - 1, static_cast<int>(floorf(fminf(fmaxf(hi, -2.f), fsteps))) + 1);
     const float* gp = G + (static_cast<int64_t>(pair) * kcap * npix + static_cast<int64_t>(py) * a.out_w + px) * CG;
-    for (int k = k0; k <= k1; ++k) {
-      const float kf = static_cast<float>(k);
-      const float qx = fmaf(kf, rr.dx, rr.sx) - 0.5f;
-      const float qy = fmaf(kf, rr.dy, rr.sy) - 0.5f;
-      const float qz = fmaf(kf, rr.dz, rr.sz) - 0.5f;
-      if (!((qx + 0.5f >= 0.f) && (qx + 0.5f < fdx) && (qy + 0.5f >= 0.f) && (qy + 0.5f < fdy) &&
-            (qz + 0.5f >= 0.f) && (qz + 0.5f < fdz)))
-        continue;
-      const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
-      const int ix0 = static_cast<int>(fx), iy0 = static_cast<int>(fy), iz0 = static_cast<int>(fz);
-      const int ix1 = static_cast<int>(ceilf(qx)), iy1 = static_cast<int>(ceilf(qy)), iz1 = static_cast<int>(ceilf(qz));
-      // corners of this brick only (skips the k margins outside the box too)
-      if (ix1 < b0x || ix0 > b1x || iy1 < b0y || iy0 > b1y || iz1 < b0z || iz0 > b1z) continue;
-      const float wx = qx - fx, wy = qy - fy, wz = qz - fz;
-      const Vec<CG> gs = vload<CG>(gp + static_cast<int64_t>(k) * npix * CG);
-      const float fw[2][3] = {{1.f - wx, 1.f - wy, 1.f - wz}, {wx, wy, wz}};
+    // kUnroll samples per round: positions, masks and G loads first, then the
+    // LDS accumulation, so the G loads of a round are in flight together.
+    for (int k = k0; k <= k1; k += kUnroll) {
+      float qx[kUnroll], qy[kUnroll], qz[kUnroll];
+      bool ok[kUnroll];
+      Vec<CG> gs[kUnroll];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-        const int ix = bx ? ix1 : ix0, iy = by ? iy1 : iy0, iz = bz ? iz1 : iz0;
-        if (ix < b0x || ix > b1x || iy < b0y || iy > b1y || iz < b0z || iz > b1z) continue;
-        const float w = fw[bx][0] * fw[by][1] * fw[bz][2];
-        float* dst = acc + (((iz - b0z) * BY + (iy - b0y)) * BX + (ix - b0x)) * CG;
+      for (int u = 0; u < kUnroll; ++u) {
+        const float kf = static_cast<float>(k + u);
+        qx[u] = fmaf(kf, rr.dx, rr.sx) - 0.5f;
+        qy[u] = fmaf(kf, rr.dy, rr.sy) - 0.5f;
+        qz[u] = fmaf(kf, rr.dz, rr.sz) - 0.5f;
+        // in the forward mask, and some corner in this brick (floor <= b1, ceil >= b0)
+        ok[u] = (k + u <= k1) && (qx[u] + 0.5f >= 0.f) && (qx[u] + 0.5f < fdx) && (qy[u] + 0.5f >= 0.f) &&
+                (qy[u] + 0.5f < fdy) && (qz[u] + 0.5f >= 0.f) && (qz[u] + 0.5f < fdz) &&
+                !(ceilf(qx[u]) < fb0x || floorf(qx[u]) > fb1x || ceilf(qy[u]) < fb0y || floorf(qy[u]) > fb1y ||
+                  ceilf(qz[u]) < fb0z || floorf(qz[u]) > fb1z);
+        gs[u] = ok[u] ? vload<CG>(gp + static_cast<int64_t>(k + u) * npix * CG) : vzero<CG>();
+      }
 #pragma unroll
-        for (int i = 0; i < CG; ++i) atomicAdd(dst + i, gs.v[i] * w);
+      for (int u = 0; u < kUnroll; ++u) {
+        if (!ok[u]) continue;
+        const float fx = floorf(qx[u]), fy = floorf(qy[u]), fz = floorf(qz[u]);
+        const int ix0 = static_cast<int>(fx), iy0 = static_cast<int>(fy), iz0 = static_cast<int>(fz);
+        const int ix1 = static_cast<int>(ceilf(qx[u])), iy1 = static_cast<int>(ceilf(qy[u]));
+        const int iz1 = static_cast<int>(ceilf(qz[u]));
+        const float wx = qx[u] - fx, wy = qy[u] - fy, wz = qz[u] - fz;
+        const float fw[2][3] = {{1.f - wx, 1.f - wy, 1.f - wz}, {wx, wy, wz}};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+          const int ix = bx ? ix1 : ix0, iy = by ? iy1 : iy0, iz = bz ? iz1 : iz0;
+          if (ix < b0x || ix > b1x || iy < b0y || iy > b1y || iz < b0z || iz > b1z) continue;
+          const float w = fw[bx][0] * fw[by][1] * fw[bz][2];
+          float* dst = acc + (((iz - b0z) * BY + (iy - b0y)) * BX + (ix - b0x)) * CG;
+#pragma unroll
+          for (int i = 0; i < CG; ++i) atomicAdd(dst + i, gs[u].v[i] * w);
+        }
       }
     }
   }
