@@ -59,8 +59,8 @@ OP_KERNELS = {
     "advect_fwd": ["advect_sl_kernel", "advect_mc_kernel"],
     "advect_bwd": ["advect_mc_bwd_brick_kernel", "advect_sl_bwd_brick_kernel", "advect_rho_gather_kernel",
                    "faces_from_centered_grad_kernel", "fill_kernel"],
-    "render_fwd": ["raymarch_fwd_kernel"],
-    "render_bwd": ["raymarch_bwd_prepass_kernel", "raymarch_bwd_brick_kernel"],
+    "render_fwd": ["to_bricked_kernel", "raymarch_fwd_kernel"],
+    "render_bwd": ["to_bricked_kernel", "raymarch_bwd_prepass_kernel", "raymarch_bwd_brick_kernel"],
 }
 COPY_BYTES = 256 << 20
 
@@ -261,6 +261,8 @@ def op_calls(torch, prob):
     g_adv = torch.empty_like(rho)
     rws_b = lib.ngt_render_density_bwd_workspace_size(s_rho, cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0)
     rws = torch.empty(max(rws_b, 1), dtype=torch.uint8, device=rho.device)
+    fws_b = lib.ngt_render_density_fwd_workspace_size(s_rho, 1)
+    fws = torch.empty(max(fws_b, 1), dtype=torch.uint8, device=rho.device)
     P = _lib.ptr
     calls = {
         "advect_fwd": lambda: _lib.check(lib.ngt_advect_fwd(
@@ -269,7 +271,8 @@ def op_calls(torch, prob):
             P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(g_out), P(sl), 0, P(g_in), P(gx), P(gy),
             P(gz), P(ws), wsb, st())),
         "render_fwd": lambda: _lib.check(lib.ngt_render_density_fwd(
-            P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(img), st())),
+            P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(img), P(fws),
+            fws_b, st())),
         "render_bwd": lambda: _lib.check(lib.ngt_render_density_bwd(
             P(rho), s_rho, lp, prob["ds"], P(img), P(dimg), cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0,
             0, P(g_adv), P(rws), rws_b, st())),

@@ -132,16 +132,21 @@ int ngt_reduce_grid_blend_bwd(const float* output_grad, const float* output, con
 
 /* ---- RaymarchGridTransform (+Grad) (raymarch_grid.cc:29-92) ----------------
  * input [N,D,H,W,C] -> output [N,V|1,H',W',C] for output_dhw = (D',H',W')
- * (D' = samples per ray before BORDER clipping). */
+ * (D' = samples per ray before BORDER clipping).  With LINEAR interpolation
+ * and at least _fwd_workspace_size bytes of workspace the input is first
+ * copied to a 4x4x4-bricked layout that the ray gathers read (fewer cache
+ * lines per wavefront); with a null / smaller workspace the rays read the
+ * NDHWC input directly.  Both give the same result. */
+size_t ngt_raymarch_grid_transform_fwd_workspace_size(const int64_t* input_shape, int interpolation);
 int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_shape, const float* matrix_m,
                                     const float* matrix_v, const float* matrix_p, const float* frustum,
                                     int num_cameras, const int32_t* output_dhw, int interpolation, int boundary,
-                                    int blending_mode, int separate_camera_batch, float* output,
-                                    ngt_stream_t stream);
+                                    int blending_mode, int separate_camera_batch, float* output, void* workspace,
+                                    size_t workspace_bytes, ngt_stream_t stream);
 /* Exact gradients with BORDER + LINEAR run atomics-free (a per-sample
- * gradient pass, then one gather per voxel); they need this much workspace
- * (0 otherwise).  With less workspace the call falls back to the fp32-atomic
- * scatter.  Reference semantics need C+1 floats per voxel. */
+ * gradient pass, then one gather per brick of voxels); they need this much
+ * workspace (0 otherwise).  With less workspace the call falls back to the
+ * fp32-atomic scatter.  Reference semantics need C+1 floats per voxel. */
 size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape, const float* matrix_m,
                                                       const float* matrix_v, const float* matrix_p,
                                                       const float* frustum, int num_cameras,
@@ -164,10 +169,12 @@ int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_sha
  * [N,V,H',W',4] = (RGB, optical depth), exactly the raymarch of the
  * reference's concatenated grid.  BEER_LAMBERT / ADDITIVE / ALPHA /
  * ALPHA_ADDITIVE as the raymarch op. */
+size_t ngt_render_density_fwd_workspace_size(const int64_t* density_shape, int interpolation);
 int ngt_render_density_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
                            float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
                            const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
-                           int boundary, int blending_mode, float* output, ngt_stream_t stream);
+                           int boundary, int blending_mode, float* output, void* workspace, size_t workspace_bytes,
+                           ngt_stream_t stream);
 size_t ngt_render_density_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
                                              const float* matrix_v, const float* matrix_p, const float* frustum,
                                              int num_cameras, const int32_t* output_dhw, int interpolation,

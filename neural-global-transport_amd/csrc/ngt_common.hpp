@@ -195,6 +195,47 @@ NGT_DEVICE void fetch8_raw(const float* __restrict__ g, const Dims& d, const Cor
   for (int k = 0; k < 8; ++k) v[k] = vload<C>(g + (rows[k >> 1] + xs[k & 1]) * static_cast<uint32_t>(C));
 }
 
+// Bricked layout of a grid slice: 4x4x4-cell bricks (x fastest inside a
+// brick, bricks in x, y, z order), so a trilinear footprint and the 8x8-ray
+// footprint of a wavefront touch few cache lines whatever the view direction.
+// Cell (x, y, z) sits at  (x>>2)*64 + (x&3)  +  (y>>2)*sy + (y&3)*4
+//                       + (z>>2)*sz + (z&3)*16,  sy = 64*nbx, sz = 64*nbx*nby.
+struct BrickDims {
+  uint32_t sy, sz;       // strides of one brick step in y / z (cells)
+  uint32_t slice_cells;  // cells of one bricked slice (bricks * 64)
+};
+
+inline BrickDims brick_dims(const Dims& d) {
+  const uint32_t nbx = (d.x + 3) / 4, nby = (d.y + 3) / 4, nbz = (d.z + 3) / 4;
+  return BrickDims{64u * nbx, 64u * nbx * nby, 64u * nbx * nby * nbz};
+}
+
+NGT_DEVICE uint32_t brick_offset(const BrickDims& b, int x, int y, int z) {
+  const uint32_t ux = static_cast<uint32_t>(x), uy = static_cast<uint32_t>(y), uz = static_cast<uint32_t>(z);
+  return (ux >> 2) * 64u + (ux & 3u) + (uy >> 2) * b.sy + ((uy & 3u) << 2) + (uz >> 2) * b.sz + ((uz & 3u) << 4);
+}
+
+// fetch8_raw on a bricked slice.
+template <int C, int BM>
+NGT_DEVICE void fetch8_raw_bricked(const float* __restrict__ g, const Dims& d, const BrickDims& b, const Corners& c,
+                                   Vec<C> (&v)[8]) {
+  int x0 = c.x0, x1 = c.x1, y0 = c.y0, y1 = c.y1, z0 = c.z0, z1 = c.z1;
+  if constexpr (BM == BORDER) {
+    x0 = min(max(x0, 0), d.x - 1); x1 = min(max(x1, 0), d.x - 1);
+    y0 = min(max(y0, 0), d.y - 1); y1 = min(max(y1, 0), d.y - 1);
+    z0 = min(max(z0, 0), d.z - 1); z1 = min(max(z1, 0), d.z - 1);
+  }
+  const uint32_t X[2] = {(static_cast<uint32_t>(x0) >> 2) * 64u + (static_cast<uint32_t>(x0) & 3u),
+                         (static_cast<uint32_t>(x1) >> 2) * 64u + (static_cast<uint32_t>(x1) & 3u)};
+  const uint32_t Y[2] = {(static_cast<uint32_t>(y0) >> 2) * b.sy + ((static_cast<uint32_t>(y0) & 3u) << 2),
+                         (static_cast<uint32_t>(y1) >> 2) * b.sy + ((static_cast<uint32_t>(y1) & 3u) << 2)};
+  const uint32_t Z[2] = {(static_cast<uint32_t>(z0) >> 2) * b.sz + ((static_cast<uint32_t>(z0) & 3u) << 4),
+                         (static_cast<uint32_t>(z1) >> 2) * b.sz + ((static_cast<uint32_t>(z1) & 3u) << 4)};
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    v[k] = vload<C>(g + (X[k & 1] + Y[(k >> 1) & 1] + Z[k >> 2]) * static_cast<uint32_t>(C));
+}
+
 NGT_DEVICE bool corner_ok(const Corners& c, int k) {
   return ((k & 1) ? c.vx1 : c.vx0) & ((k & 2) ? c.vy1 : c.vy0) & ((k & 4) ? c.vz1 : c.vz0);
 }
@@ -243,16 +284,20 @@ NGT_DEVICE Vec<C> read_linear(const float* __restrict__ g, const Dims& d, float 
   return lerp8<C, BM>(v, c);
 }
 
-// U trilinear reads whose 8*U loads are all issued before any is consumed.
-template <int C, int BM, int U>
-NGT_DEVICE void read_linear_batch(const float* __restrict__ g, const Dims& d, const float (&qx)[U],
+// U trilinear reads whose 8*U loads are all issued before any is consumed;
+// BRICKED selects the bricked slice layout (BrickDims b).
+template <int C, int BM, int U, bool BRICKED>
+NGT_DEVICE void read_linear_batch(const float* __restrict__ g, const Dims& d, const BrickDims& b, const float (&qx)[U],
                                   const float (&qy)[U], const float (&qz)[U], Vec<C> (&out)[U]) {
   Corners c[U];
   Vec<C> v[U][8];
 #pragma unroll
   for (int u = 0; u < U; ++u) c[u] = make_corners<BM>(d, qx[u], qy[u], qz[u]);
 #pragma unroll
-  for (int u = 0; u < U; ++u) fetch8_raw<C, BM>(g, d, c[u], v[u]);
+  for (int u = 0; u < U; ++u) {
+    if constexpr (BRICKED) fetch8_raw_bricked<C, BM>(g, d, b, c[u], v[u]);
+    else fetch8_raw<C, BM>(g, d, c[u], v[u]);
+  }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     mask8<C, BM>(c[u], v[u]);

@@ -34,6 +34,9 @@ struct RayArgs {
   int out_w, out_h, out_d;
   float inv_w, inv_h, inv_d;
   float light_r, light_g, light_b, depth_step;  // fused density renderer only
+  const float* bricked;                         // bricked copy of input (kernels with BRICKED = true)
+  BrickDims bd;
+  int n_data;
 };
 
 NGT_DEVICE int kMaxRaySteps(const RayArgs& a) { return 16 * (a.din.x + a.din.y + a.din.z + a.out_d); }
@@ -108,28 +111,49 @@ NGT_DEVICE void pixel_of_thread(int& x, int& y) {
 
 // Cell value read by a sample: the C-channel input (LIGHT = false) or the
 // fused renderer's (rho*rgb*ds, rho*ds) from the 1-channel density.
-template <int C, int FM, int BM, bool LIGHT>
+// BRICKED (LINEAR only): src is a slice of the bricked copy.
+template <int CIN, int FM, int BM, bool BRICKED>
+NGT_DEVICE Vec<CIN> read_cell(const RayArgs& a, const float* __restrict__ src, float qx, float qy, float qz) {
+  if constexpr (BRICKED) {
+    static_assert(FM == LINEAR, "the bricked layout serves LINEAR sampling only");
+    const float x[1] = {qx}, y[1] = {qy}, z[1] = {qz};
+    Vec<CIN> r[1];
+    read_linear_batch<CIN, BM, 1, true>(src, a.din, a.bd, x, y, z, r);
+    return r[0];
+  } else {
+    return read_filtered<CIN, FM, BM>(src, a.din, qx, qy, qz);
+  }
+}
+
+template <int C, int FM, int BM, bool LIGHT, bool BRICKED = false>
 NGT_DEVICE Vec<C> sample_cell(const RayArgs& a, const float* __restrict__ src, float qx, float qy, float qz) {
   if constexpr (LIGHT) {
-    const Vec<1> rho = read_filtered<1, FM, BM>(src, a.din, qx, qy, qz);
+    const Vec<1> rho = read_cell<1, FM, BM, BRICKED>(a, src, qx, qy, qz);
     const float s = rho.v[0] * a.depth_step;
     Vec<C> c;
     c.v[0] = a.light_r * s; c.v[1] = a.light_g * s; c.v[2] = a.light_b * s; c.v[3] = s;
     return c;
   } else {
-    return read_filtered<C, FM, BM>(src, a.din, qx, qy, qz);
+    return read_cell<C, FM, BM, BRICKED>(a, src, qx, qy, qz);
   }
+}
+
+// Slice of data entry n read by a kernel (the bricked copy when BRICKED).
+template <int CIN, bool BRICKED>
+NGT_DEVICE const float* ray_source(const RayArgs& a, int n) {
+  if constexpr (BRICKED) return a.bricked + static_cast<int64_t>(n) * a.bd.slice_cells * CIN;
+  else return a.input + static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z * CIN;
 }
 
 // kUnroll cell values at once; for LINEAR all their corner loads are issued
 // before any is used.
-template <int C, int FM, int BM, bool LIGHT>
+template <int C, int FM, int BM, bool LIGHT, bool BRICKED>
 NGT_DEVICE void sample_cells(const RayArgs& a, const float* __restrict__ src, const float (&qx)[kUnroll],
                              const float (&qy)[kUnroll], const float (&qz)[kUnroll], Vec<C> (&cells)[kUnroll]) {
   if constexpr (FM == LINEAR) {
     constexpr int CIN = LIGHT ? 1 : C;
     Vec<CIN> raw[kUnroll];
-    read_linear_batch<CIN, BM, kUnroll>(src, a.din, qx, qy, qz, raw);
+    read_linear_batch<CIN, BM, kUnroll, BRICKED>(src, a.din, a.bd, qx, qy, qz, raw);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       if constexpr (LIGHT) {
@@ -160,20 +184,21 @@ NGT_DEVICE void scatter_cell(const RayArgs& a, const float* __restrict__ src, fl
   }
 }
 
-template <int C, int FM, int BM, int MODE, bool LIGHT>
+template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
 __global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch cb, float* __restrict__ out) {
   int x, y;
   pixel_of_thread(x, y);
   if (x >= a.out_w || y >= a.out_h) return;
   const CamParams& cp = cb.cam[blockIdx.z];
   constexpr int CIN = LIGHT ? 1 : C;
-  const float* src = a.input + static_cast<int64_t>(cp.data_index) * a.din.x * a.din.y * a.din.z * CIN;
+  const float* src = ray_source<CIN, BRICKED>(a, cp.data_index);
   const Ray r = make_ray<BM>(a, cp, x, y);
   Vec<C> acc = vzero<C>();
   // Samples are fetched kUnroll at a time so their gathers are in flight
-  // together; the blend recurrence then consumes them in order.
-  int k = 0;
-  for (; k + kUnroll <= r.steps; k += kUnroll) {
+  // together; the blend recurrence then consumes them in order.  The last
+  // round's positions past the ray end are fetched (in-grid addresses) but
+  // not blended.
+  for (int k = 0; k < r.steps; k += kUnroll) {
     float qx[kUnroll], qy[kUnroll], qz[kUnroll];
     Vec<C> cells[kUnroll];
 #pragma unroll
@@ -181,21 +206,17 @@ __global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch c
       const float kf = static_cast<float>(k + u);
       qx[u] = fmaf(kf, r.dx, r.sx) - 0.5f; qy[u] = fmaf(kf, r.dy, r.sy) - 0.5f; qz[u] = fmaf(kf, r.dz, r.sz) - 0.5f;
     }
-    sample_cells<C, FM, BM, LIGHT>(a, src, qx, qy, qz, cells);
+    sample_cells<C, FM, BM, LIGHT, BRICKED>(a, src, qx, qy, qz, cells);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) blend_step<C, MODE>(acc, cells[u]);
-  }
-  for (; k < r.steps; ++k) {
-    const float kf = static_cast<float>(k);
-    const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
-    blend_step<C, MODE>(acc, sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz));
+    for (int u = 0; u < kUnroll; ++u)
+      if (k + u < r.steps) blend_step<C, MODE>(acc, cells[u]);
   }
   vstore<C>(out + ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C, acc);
 }
 
 // Per-sample cell gradients of one ray (exact adjoint of the forward walk),
 // handed to ``sink(k, qx, qy, qz, gc)``.
-template <int C, int FM, int BM, int MODE, bool LIGHT, typename Sink>
+template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED, typename Sink>
 NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ src, const Ray& r, Vec<C> g,
                                   const float* __restrict__ out_pix, Sink& sink) {
   if constexpr (C > 1 && MODE == ALPHA) {
@@ -203,7 +224,7 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
     for (int k = r.steps - 1; k >= 0; --k) {
       const float kf = static_cast<float>(k);
       const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
-      const Vec<C> cell = sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz);
+      const Vec<C> cell = sample_cell<C, FM, BM, LIGHT, BRICKED>(a, src, qx, qy, qz);
       const float aprev = 1.f - (1.f - acc.v[C - 1]) / (1.f - cell.v[C - 1]);
       const float t = 1.f - aprev;
       float lg = 0.f;
@@ -251,8 +272,7 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
       sink(k, qx, qy, qz, gc);
     };
     constexpr bool kNeedCell = !(C == 1 || MODE == ADDITIVE);
-    int k = 0;
-    for (; k + kUnroll <= r.steps; k += kUnroll) {
+    for (int k = 0; k < r.steps; k += kUnroll) {
       float qx[kUnroll], qy[kUnroll], qz[kUnroll];
       Vec<C> cells[kUnroll];
 #pragma unroll
@@ -261,14 +281,10 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
         qx[u] = fmaf(kf, r.dx, r.sx) - 0.5f; qy[u] = fmaf(kf, r.dy, r.sy) - 0.5f; qz[u] = fmaf(kf, r.dz, r.sz) - 0.5f;
         cells[u] = vzero<C>();
       }
-      if constexpr (kNeedCell) sample_cells<C, FM, BM, LIGHT>(a, src, qx, qy, qz, cells);
+      if constexpr (kNeedCell) sample_cells<C, FM, BM, LIGHT, BRICKED>(a, src, qx, qy, qz, cells);
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) step(k + u, qx[u], qy[u], qz[u], cells[u]);
-    }
-    for (; k < r.steps; ++k) {
-      const float kf = static_cast<float>(k);
-      const float qx = fmaf(kf, r.dx, r.sx) - 0.5f, qy = fmaf(kf, r.dy, r.sy) - 0.5f, qz = fmaf(kf, r.dz, r.sz) - 0.5f;
-      step(k, qx, qy, qz, kNeedCell ? sample_cell<C, FM, BM, LIGHT>(a, src, qx, qy, qz) : vzero<C>());
+      for (int u = 0; u < kUnroll; ++u)
+        if (k + u < r.steps) step(k + u, qx[u], qy[u], qz[u], cells[u]);
     }
   }
 }
@@ -310,7 +326,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
   auto sink = [&](int, float qx, float qy, float qz, const Vec<C>& gc) {
     scatter_filtered<CIN, FM, BM>(src, gdst, a.din, qx, qy, qz, fold_cell_grad<C, LIGHT>(a, gc));
   };
-  walk_ray_gradient<C, FM, BM, MODE, LIGHT>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
+  walk_ray_gradient<C, FM, BM, MODE, LIGHT, false>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
 }
 
 // ---------------------------------------------------------------------------
@@ -327,7 +343,7 @@ struct RayRec {
   int pad;
 };
 
-template <int C, int BM, int MODE, bool LIGHT>
+template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
 __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, CamBatch cb, int pair_base,
                                                                    const float* __restrict__ out,
                                                                    const float* __restrict__ out_grad,
@@ -339,8 +355,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
   const CamParams& cp = cb.cam[blockIdx.z];
   constexpr int CIN = LIGHT ? 1 : C;
   constexpr int CG = GradChannels<C, LIGHT>::value;
-  const int64_t slice = static_cast<int64_t>(a.din.x) * a.din.y * a.din.z * CIN;
-  const float* src = a.input + cp.data_index * slice;
+  const float* src = ray_source<CIN, BRICKED>(a, cp.data_index);
   Ray r = make_ray<BM>(a, cp, x, y);
   r.steps = min(r.steps, kcap);
   const int pair = pair_base + static_cast<int>(blockIdx.z);
@@ -357,7 +372,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
   auto sink = [&](int k, float, float, float, const Vec<C>& gc) {
     vstore<CG>(gp + (static_cast<int64_t>(k) * npix + pixel) * CG, fold_cell_grad<C, LIGHT>(a, gc));
   };
-  walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
+  walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT, BRICKED>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
 }
 
 // Pass 2: one work-group per brick of BX x BY x BZ voxels.  For every view it
@@ -627,6 +642,9 @@ RayArgs make_ray_args(const float* input, const int64_t* shape, const int32_t* o
   RayArgs a;
   a.input = input;
   a.din = shape5(shape).dims();
+  a.bricked = nullptr;
+  a.bd = brick_dims(a.din);
+  a.n_data = static_cast<int>(shape[0]);
   a.out_d = out_dhw[0]; a.out_h = out_dhw[1]; a.out_w = out_dhw[2];
   a.inv_w = 1.f / a.out_w; a.inv_h = 1.f / a.out_h; a.inv_d = 1.f / a.out_d;
   a.light_r = a.light_g = a.light_b = 0.f;
@@ -646,6 +664,32 @@ void for_cam_batches(const std::vector<Pair>& pairs, const float* M, const float
     }
     fn(cb);
   }
+}
+
+// Copy of an NDHWC grid in the bricked slice layout (one work-item per cell,
+// coalesced reads; a brick's 64 cells are written as 16-byte runs).
+template <int C>
+__global__ __launch_bounds__(256) void to_bricked_kernel(const float* __restrict__ in, Dims d, BrickDims b,
+                                                         float* __restrict__ out) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y = blockIdx.y * 4 + threadIdx.y;
+  const int z = blockIdx.z % d.z;
+  const int n = blockIdx.z / d.z;
+  if (x >= d.x || y >= d.y) return;
+  const int64_t cells = static_cast<int64_t>(d.x) * d.y * d.z;
+  const Vec<C> v = vload<C>(in + (n * cells + cell_offset(d, x, y, z)) * C);
+  vstore<C>(out + (static_cast<int64_t>(n) * b.slice_cells + brick_offset(b, x, y, z)) * C, v);
+}
+
+void launch_to_bricked(const float* in, const Dims& d, const BrickDims& b, int n_data, int C, float* out,
+                       hipStream_t s) {
+  const dim3 grid(blocks_for(d.x, 64), blocks_for(d.y, 4), static_cast<unsigned>(d.z * n_data));
+  NGT_SWITCH_C(C, hipLaunchKernelGGL((to_bricked_kernel<CC>), grid, dim3(64, 4), 0, s, in, d, b, out));
+}
+
+size_t bricked_bytes(const int64_t* shape, int channels) {
+  const Dims d{static_cast<int>(shape[3]), static_cast<int>(shape[2]), static_cast<int>(shape[1])};
+  return static_cast<size_t>(shape[0]) * brick_dims(d).slice_cells * channels * sizeof(float);
 }
 
 // ------------------------------------------------------ gather planning ----
@@ -703,6 +747,7 @@ struct GatherPlan {
   int kcap;
   RayRec* rays;
   float* G;
+  float* bricked;  // bricked copy of the sampled grid (LINEAR: always, it follows G)
 };
 
 constexpr size_t kAlign = 256;
@@ -720,7 +765,8 @@ size_t gather_workspace_bytes(const int64_t* shape, const float* M, const float*
   const int kcap = ray_step_cap(pairs, M, V, P, F, a);
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   return align_up(pairs.size() * npix * sizeof(RayRec)) +
-         align_up(pairs.size() * static_cast<size_t>(kcap) * npix * grad_channels * sizeof(float));
+         align_up(pairs.size() * static_cast<size_t>(kcap) * npix * grad_channels * sizeof(float)) +
+         align_up(bricked_bytes(shape, grad_channels));
 }
 
 bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* F,
@@ -731,16 +777,21 @@ bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V,
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   const size_t rays_bytes = align_up(pairs.size() * npix * sizeof(RayRec));
   const size_t g_bytes = align_up(pairs.size() * static_cast<size_t>(gp.kcap) * npix * grad_channels * sizeof(float));
-  if (workspace_bytes < rays_bytes + g_bytes) return false;
+  const size_t b_bytes = align_up(static_cast<size_t>(a.n_data) * a.bd.slice_cells * grad_channels * sizeof(float));
+  if (workspace_bytes < rays_bytes + g_bytes + b_bytes) return false;
   gp.rays = static_cast<RayRec*>(workspace);
   gp.G = reinterpret_cast<float*>(static_cast<char*>(workspace) + rays_bytes);
+  gp.bricked = reinterpret_cast<float*>(static_cast<char*>(workspace) + rays_bytes + g_bytes);
   return true;
 }
 
 template <bool LIGHT>
 void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* F,
-                const RayArgs& a, int blending_mode, int C, const float* output, const float* output_grad,
+                const RayArgs& a_in, int blending_mode, int C, const float* output, const float* output_grad,
                 const GatherPlan& gp, float* in_grad, int n_data, hipStream_t s) {
+  RayArgs a = a_in;
+  launch_to_bricked(a.input, a.din, a.bd, n_data, LIGHT ? 1 : C, gp.bricked, s);
+  a.bricked = gp.bricked;
   for (size_t b = 0; b < pairs.size(); b += kMaxCamsPerLaunch) {
     CamBatch cb;
     cb.count = static_cast<int>(std::min<size_t>(kMaxCamsPerLaunch, pairs.size() - b));
@@ -752,12 +803,12 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
     const int base = static_cast<int>(b);
     if constexpr (LIGHT) {
       NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<4, BORDER, MM, true>), rgrid, dim3(256), 0, s, a, cb, base,
-                             output, output_grad, gp.G, gp.rays, gp.kcap));
+          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<4, BORDER, MM, true, true>), rgrid, dim3(256), 0, s, a, cb,
+                             base, output, output_grad, gp.G, gp.rays, gp.kcap));
     } else {
       NGT_SWITCH_C(C, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false>), rgrid, dim3(256), 0, s, a, cb, base,
-                             output, output_grad, gp.G, gp.rays, gp.kcap)));
+          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false, true>), rgrid, dim3(256), 0, s, a, cb,
+                             base, output, output_grad, gp.G, gp.rays, gp.kcap)));
     }
     constexpr int BX = 8, BY = 8, BZ = 8;
     const unsigned bricks = blocks_for(a.din.x, BX) * blocks_for(a.din.y, BY) * blocks_for(a.din.z, BZ);
@@ -776,22 +827,40 @@ using namespace ngt;
 
 extern "C" {
 
+size_t ngt_raymarch_grid_transform_fwd_workspace_size(const int64_t* input_shape, int interpolation) {
+  if (input_shape == nullptr || interpolation != LINEAR) return 0;
+  return bricked_bytes(input_shape, static_cast<int>(input_shape[4]));
+}
+
 int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_shape, const float* matrix_m,
                                     const float* matrix_v, const float* matrix_p, const float* frustum,
                                     int num_cameras, const int32_t* output_dhw, int interpolation, int boundary,
-                                    int blending_mode, int separate_camera_batch, float* output,
-                                    ngt_stream_t stream) {
+                                    int blending_mode, int separate_camera_batch, float* output, void* workspace,
+                                    size_t workspace_bytes, ngt_stream_t stream) {
   std::vector<Pair> pairs;
   int rc = check_ray_args(input_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
                           boundary, blending_mode, separate_camera_batch, pairs, false);
   if (rc) return rc;
-  const RayArgs a = make_ray_args(input, input_shape, output_dhw);
+  RayArgs a = make_ray_args(input, input_shape, output_dhw);
   hipStream_t s = as_stream(stream);
   const int C = static_cast<int>(input_shape[4]);
+  const size_t need = ngt_raymarch_grid_transform_fwd_workspace_size(input_shape, interpolation);
+  const bool bricked = need > 0 && workspace != nullptr && workspace_bytes >= need;
+  if (bricked) {
+    launch_to_bricked(input, a.din, a.bd, a.n_data, C, static_cast<float*>(workspace), s);
+    a.bricked = static_cast<const float*>(workspace);
+  }
   for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
     const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
-    NGT_SWITCH_C(C, NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-        hipLaunchKernelGGL((raymarch_fwd_kernel<CC, FF, BB, MM, false>), grid, dim3(256), 0, s, a, cb, output)))));
+    if (bricked) {
+      NGT_SWITCH_C(C, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<CC, LINEAR, BB, MM, false, true>), grid, dim3(256), 0, s, a, cb,
+                             output))));
+    } else {
+      NGT_SWITCH_C(C, NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<CC, FF, BB, MM, false, false>), grid, dim3(256), 0, s, a, cb,
+                             output)))));
+    }
   });
   return check_launch("ngt_raymarch_grid_transform_fwd");
 }
@@ -865,10 +934,16 @@ int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_sha
   return check_launch("ngt_raymarch_grid_transform_bwd");
 }
 
+size_t ngt_render_density_fwd_workspace_size(const int64_t* density_shape, int interpolation) {
+  if (density_shape == nullptr || interpolation != LINEAR) return 0;
+  return bricked_bytes(density_shape, 1);
+}
+
 int ngt_render_density_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
                            float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
                            const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
-                           int boundary, int blending_mode, float* output, ngt_stream_t stream) {
+                           int boundary, int blending_mode, float* output, void* workspace, size_t workspace_bytes,
+                           ngt_stream_t stream) {
   std::vector<Pair> pairs;
   int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
                           boundary, blending_mode, 1, pairs, true);
@@ -878,10 +953,23 @@ int ngt_render_density_fwd(const float* density, const int64_t* density_shape, c
   a.light_r = light_rgb[0]; a.light_g = light_rgb[1]; a.light_b = light_rgb[2];
   a.depth_step = depth_step;
   hipStream_t s = as_stream(stream);
+  const size_t need = ngt_render_density_fwd_workspace_size(density_shape, interpolation);
+  const bool bricked = need > 0 && workspace != nullptr && workspace_bytes >= need;
+  if (bricked) {
+    launch_to_bricked(density, a.din, a.bd, a.n_data, 1, static_cast<float*>(workspace), s);
+    a.bricked = static_cast<const float*>(workspace);
+  }
   for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
     const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
-    NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-        hipLaunchKernelGGL((raymarch_fwd_kernel<4, FF, BB, MM, true>), grid, dim3(256), 0, s, a, cb, output))));
+    if (bricked) {
+      NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<4, LINEAR, BB, MM, true, true>), grid, dim3(256), 0, s, a, cb,
+                             output)));
+    } else {
+      NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<4, FF, BB, MM, true, false>), grid, dim3(256), 0, s, a, cb,
+                             output))));
+    }
   });
   return check_launch("ngt_render_density_fwd");
 }

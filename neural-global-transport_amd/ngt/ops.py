@@ -33,6 +33,11 @@ gradient_semantics = "exact"
 # the fp32-atomic scatter kernels instead (same result up to summation order).
 use_gather_adjoint = True
 
+# Raymarch / render forwards with LINEAR interpolation hand the library a
+# workspace for a 4x4x4-bricked copy of the grid that the rays read; False
+# makes them read the NDHWC grid directly (same result).
+use_bricked_layout = True
+
 INTERPOLATION = {"NEAREST": 0, "LINEAR": 1, "MIN": 2, "MAX": 3, "QUADRATIC": 4}
 BOUNDARY = {"BORDER": 0, "CLAMP": 1, "WRAP": 2}
 MIPMAPPING = {"NONE": 0, "NEAREST": 1, "LINEAR": 2}
@@ -316,10 +321,15 @@ def _ray_fwd(inp, cams, out_shape, a):
     inp = _lib.device_tensor(inp, "input")
     cams.check_batch(inp.shape[0])
     out = torch.empty(_ray_out_shape(inp, cams, out_shape, a), dtype=torch.float32, device=inp.device)
+    s_in = _lib.shape5(inp)
+    ws, nbytes = (None, 0)
+    if use_bricked_layout:
+        ws, nbytes = _lib.workspace(lib.ngt_raymarch_grid_transform_fwd_workspace_size(s_in, a["interpolation"]),
+                                    inp.device)
     _lib.check(lib.ngt_raymarch_grid_transform_fwd(
-        _lib.ptr(inp), _lib.shape5(inp), cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape),
+        _lib.ptr(inp), s_in, cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape),
         a["interpolation"], a["boundary"], a["blending_mode"], int(a["separate_camera_batch"]), _lib.ptr(out),
-        _lib.stream(inp.device)))
+        _lib.ptr(ws), nbytes, _lib.stream(inp.device)))
     return out
 
 
@@ -412,9 +422,13 @@ def _render_fwd(density, light_rgb, depth_step, cams, out_shape, interp, boundar
     out = torch.empty((density.shape[0], cams.nv, int(out_shape[1]), int(out_shape[2]), 4), dtype=torch.float32,
                       device=density.device)
     lp, _, keep = _lib.host_floats(light_rgb, 3)
+    s_in = _lib.shape5(density)
+    ws, nbytes = (None, 0)
+    if use_bricked_layout:
+        ws, nbytes = _lib.workspace(lib.ngt_render_density_fwd_workspace_size(s_in, interp), density.device)
     _lib.check(lib.ngt_render_density_fwd(
-        _lib.ptr(density), _lib.shape5(density), lp, float(depth_step), cams.m, cams.v, cams.p, cams.f, cams.nv,
-        _lib.dhw(out_shape), interp, boundary, mode, _lib.ptr(out), _lib.stream(density.device)))
+        _lib.ptr(density), s_in, lp, float(depth_step), cams.m, cams.v, cams.p, cams.f, cams.nv,
+        _lib.dhw(out_shape), interp, boundary, mode, _lib.ptr(out), _lib.ptr(ws), nbytes, _lib.stream(density.device)))
     return out
 
 

@@ -349,3 +349,34 @@ def test_advect_overflow_channels_batches(ngt_ops, vmax, C, separate):
     assert rel(host(rt.grad), eg_rho) < GRAD_TOL
     for t, e in zip(vel, eg_v):
         assert rel(host(t.grad), e) < GRAD_TOL
+
+
+@pytest.mark.parametrize("C,bound,mode", [(1, BORDER, B.BEER_LAMBERT), (2, CLAMP, B.ALPHA_ADDITIVE),
+                                          (4, BORDER, B.ALPHA), (4, CLAMP, B.ADDITIVE)])
+def test_raymarch_bricked_layout_is_bit_identical(ngt_ops, C, bound, mode):
+    """The forward over the 4x4x4-bricked copy (dims not multiples of 4)
+    equals the direct NDHWC read bit for bit, and so do the gather-adjoint
+    gradients that read it."""
+    rng = np.random.default_rng(800 + C)
+    dims = (13, 10, 11)
+    out_dhw = (17, 9, 12)
+    Ms, Vs, Ps, Fs = ring_args(dims, out_dhw, views=3, generic=True)
+    x = dev(rng.uniform(0.0, 0.3, (1,) + dims + (C,)))
+    old = ngt_ops.use_bricked_layout
+    try:
+        outs = []
+        for bricked in (True, False):
+            ngt_ops.use_bricked_layout = bricked
+            outs.append(ngt_ops.raymarch_grid_transform(x, Ms, Vs, Ps, Fs, out_dhw, boundary=BNAME[bound],
+                                                        blending_mode=MNAME[mode]))
+        assert torch.equal(outs[0], outs[1])
+        rho = dev(rng.uniform(0.0, 0.5, (2,) + dims + (1,)))
+        Ms2 = Ms * 2
+        imgs = []
+        for bricked in (True, False):
+            ngt_ops.use_bricked_layout = bricked
+            imgs.append(ngt_ops.render_density_fused(rho, (0.9, 0.8, 0.7), 0.1, Ms2, Vs, Ps, Fs, out_dhw,
+                                                     boundary=BNAME[bound]))
+        assert torch.equal(imgs[0], imgs[1])
+    finally:
+        ngt_ops.use_bricked_layout = old
