@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch c
 }
 
 // Per-sample cell gradients of one ray (exact adjoint of the forward walk),
-// handed to ``sink(k, qx, qy, qz, gc)``.
+// handed to ``sink.one(k, qx, qy, qz, gc)``.
 template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED, typename Sink>
 NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ src, const Ray& r, Vec<C> g,
                                   const float* __restrict__ out_pix, Sink& sink) {
@@ -236,7 +236,7 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
       gc.v[C - 1] = t * g.v[C - 1];
       g.v[C - 1] = (1.f - cell.v[C - 1]) * g.v[C - 1] - lg;
       acc.v[C - 1] = aprev;
-      sink(k, qx, qy, qz, gc);
+      sink.one(k, qx, qy, qz, gc);
     }
   } else {
     float total = 0.f;
@@ -246,7 +246,7 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
       for (int i = 0; i < C - 1; ++i) total += L.v[i] * g.v[i];
     }
     float prefix = 0.f, dacc = 0.f;
-    auto step = [&](int k, float qx, float qy, float qz, const Vec<C>& cell) {
+    auto step = [&](const Vec<C>& cell) {
       Vec<C> gc;
       if constexpr (C == 1 || MODE == ADDITIVE) {
         gc = g;
@@ -269,7 +269,7 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
           gc.v[C - 1] = g.v[C - 1] - lg;
         }
       }
-      sink(k, qx, qy, qz, gc);
+      return gc;
     };
     constexpr bool kNeedCell = !(C == 1 || MODE == ADDITIVE);
     for (int k = 0; k < r.steps; k += kUnroll) {
@@ -284,7 +284,7 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
       if constexpr (kNeedCell) sample_cells<C, FM, BM, LIGHT, BRICKED>(a, src, qx, qy, qz, cells);
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
-        if (k + u < r.steps) step(k + u, qx[u], qy[u], qz[u], cells[u]);
+        if (k + u < r.steps) sink.one(k + u, qx[u], qy[u], qz[u], step(cells[u]));
     }
   }
 }
@@ -323,9 +323,14 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
   const Ray r = make_ray<BM>(a, cp, x, y);
   if (r.steps <= 0) return;
   const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
-  auto sink = [&](int, float qx, float qy, float qz, const Vec<C>& gc) {
-    scatter_filtered<CIN, FM, BM>(src, gdst, a.din, qx, qy, qz, fold_cell_grad<C, LIGHT>(a, gc));
-  };
+  struct {
+    const RayArgs& a;
+    const float* src;
+    float* gdst;
+    NGT_DEVICE void one(int, float qx, float qy, float qz, const Vec<C>& gc) {
+      scatter_filtered<CIN, FM, BM>(src, gdst, a.din, qx, qy, qz, fold_cell_grad<C, LIGHT>(a, gc));
+    }
+  } sink{a, src, gdst};
   walk_ray_gradient<C, FM, BM, MODE, LIGHT, false>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
 }
 
@@ -368,11 +373,35 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
   rays[pair * npix + pixel] = rec;
   if (r.steps <= 0) return;
   const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
-  float* gp = G + static_cast<int64_t>(pair) * kcap * npix * CG;
-  auto sink = [&](int k, float, float, float, const Vec<C>& gc) {
-    vstore<CG>(gp + (static_cast<int64_t>(k) * npix + pixel) * CG, fold_cell_grad<C, LIGHT>(a, gc));
-  };
+  // G[pair][k][pixel]: the samples k of neighbouring rays are stored side by side
+  struct {
+    const RayArgs& a;
+    float* gp;
+    int64_t npix, pixel;
+    NGT_DEVICE void one(int k, float, float, float, const Vec<C>& gc) {
+      vstore<CG>(gp + (static_cast<int64_t>(k) * npix + pixel) * CG, fold_cell_grad<C, LIGHT>(a, gc));
+    }
+  } sink{a, G + static_cast<int64_t>(pair) * kcap * npix * CG, npix, pixel};
   walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT, BRICKED>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
+}
+
+// Inclusive scan of one int per work-item over a 256-wide work-group: a
+// shuffle scan inside each wavefront, then the wavefront totals through LDS
+// (one barrier).  The caller synchronises before s_wtot is written again.
+NGT_DEVICE int block_inclusive_scan(int v, int* s_wtot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) s_wtot[wave] = v;
+  __syncthreads();
+  int pre = 0;
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+    if (w < wave) pre += s_wtot[w];
+  return v + pre;
 }
 
 // Pass 2: one work-group per brick of BX x BY x BZ voxels.  For every view it
@@ -389,6 +418,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamB
                                                                  const float* __restrict__ G, int kcap,
                                                                  float* __restrict__ in_grad, int accumulate) {
   constexpr int NV = BX * BY * BZ;
+  static_assert(NV == 512, "each of the 256 work-items owns two voxels of the brick");
   __shared__ float acc[NV * CG];
   __shared__ int rect[kMaxCamsPerLaunch][4];  // x0, y0, nx, ny (nx = 0: view skipped)
   __shared__ int ray_begin[kMaxCamsPerLaunch + 1];
@@ -402,7 +432,6 @@ __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamB
   const int b0y = ((brem / bricks_x) % bricks_y) * BY;
   const int b0z = (brem / (bricks_x * bricks_y)) * BZ;
   const int b1x = min(b0x + BX, a.din.x) - 1, b1y = min(b0y + BY, a.din.y) - 1, b1z = min(b0z + BZ, a.din.z) - 1;
-  for (int i = tid; i < NV * CG; i += blockDim.x) acc[i] = 0.f;
 
   // pixel rectangles of the views (one work-item per view)
   if (tid < cb.count) {
@@ -457,77 +486,181 @@ __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamB
   const float fb1x = static_cast<float>(b1x), fb1y = static_cast<float>(b1y), fb1z = static_cast<float>(b1z);
   const float lo3[3] = {static_cast<float>(b0x) - 0.5f, static_cast<float>(b0y) - 0.5f, static_cast<float>(b0z) - 0.5f};
   const float hi3[3] = {static_cast<float>(b1x) + 1.5f, static_cast<float>(b1y) + 1.5f, static_cast<float>(b1z) + 1.5f};
+  // Rays are taken 256 at a time: each work-item clips one ray to the support
+  // box (slab test with a one-sample margin) and counts its samples there;
+  // after a block-wide scan the chunk's samples are dealt out one per
+  // work-item in sub-chunks of kItems.  A sub-chunk is counting-sorted in LDS
+  // by the sample's floor cell (bins = floor cells b0-1 .. b1 per axis; two
+  // integer LDS atomics per sample), and then every work-item sums, for each
+  // voxel it owns, the samples of the 8 bins whose trilinear footprint holds
+  // that voxel, in registers: no floating-point atomics.  (A corner at
+  // floor+1 where the reference uses ceil = floor at integer positions carries
+  // weight exactly 0.)
+  constexpr int NB = (BX + 1) * (BY + 1) * (BZ + 1);
+  constexpr int kItems = 1024;
+  __shared__ int s_off[257];
+  __shared__ int s_k0[256];
+  __shared__ int s_rec[256];
+  __shared__ int bin_cnt[NB];
+  __shared__ int bin_start[NB + 1];
+  __shared__ float4 srec[kItems];
+  __shared__ float sg[CG > 1 ? kItems * CG : 1];
+  __shared__ int s_wtot[4];
+  // owned voxels: local index tid and tid + 256 (NV == 512)
+  float accv[2][CG];
+#pragma unroll
+  for (int o = 0; o < 2; ++o)
+#pragma unroll
+    for (int ch = 0; ch < CG; ++ch) accv[o][ch] = 0.f;
   const int total = ray_begin[cb.count];
-  for (int r = tid; r < total; r += blockDim.x) {
-    int ci = 0;
-    while (ci + 1 < cb.count && r >= ray_begin[ci + 1]) ++ci;
-    const int local = r - ray_begin[ci];
-    const int px = rect[ci][0] + local % rect[ci][2];
-    const int py = rect[ci][1] + local / rect[ci][2];
-    const int pair = pair_base + ci;
-    const RayRec rr = rays[pair * npix + static_cast<int64_t>(py) * a.out_w + px];
-    if (rr.steps <= 0) continue;
-    const float sv[3] = {rr.sx, rr.sy, rr.sz};
-    const float dv[3] = {rr.dx, rr.dy, rr.dz};
-    float lo = -1.0e30f, hi = 1.0e30f;
-    bool miss = false;
+  const int fb0m1x = b0x - 1, fb0m1y = b0y - 1, fb0m1z = b0z - 1;
+  for (int base_r = 0; base_r < total; base_r += 256) {
+    const int r = base_r + tid;
+    int cnt = 0, k0 = 0, rec_index = 0;
+    if (r < total) {
+      int ci = 0;
+      while (ci + 1 < cb.count && r >= ray_begin[ci + 1]) ++ci;
+      const int local = r - ray_begin[ci];
+      const int px = rect[ci][0] + local % rect[ci][2];
+      const int py = rect[ci][1] + local / rect[ci][2];
+      rec_index = (pair_base + ci) * static_cast<int>(npix) + py * a.out_w + px;
+      const RayRec rr = rays[rec_index];
+      if (rr.steps > 0) {
+        const float sv[3] = {rr.sx, rr.sy, rr.sz};
+        const float dv[3] = {rr.dx, rr.dy, rr.dz};
+        float lo = -1.0e30f, hi = 1.0e30f;
+        bool miss = false;
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      if (fabsf(dv[ax]) > 1e-30f) {
-        const float inv = 1.f / dv[ax];
-        const float t0 = (lo3[ax] - sv[ax]) * inv, t1 = (hi3[ax] - sv[ax]) * inv;
-        lo = fmaxf(lo, fminf(t0, t1));
-        hi = fminf(hi, fmaxf(t0, t1));
-      } else if (!(sv[ax] > lo3[ax] - 1e-3f && sv[ax] < hi3[ax] + 1e-3f)) {
-        miss = true;
-      }
-    }
-    if (miss || !(lo <= hi + 2.f)) continue;
-    const float fsteps = static_cast<float>(rr.steps);
-    const int k0 = max(0, static_cast<int>(floorf(fminf(fmaxf(lo, -1.f), fsteps))) - 1);
-    const int k1 = min(rr.steps - 1, static_cast<int>(floorf(fminf(fmaxf(hi, -2.f), fsteps))) + 1);
-    const float* gp = G + (static_cast<int64_t>(pair) * kcap * npix + static_cast<int64_t>(py) * a.out_w + px) * CG;
-    // kUnroll samples per round: positions, masks and G loads first, then the
-    // LDS accumulation, so the G loads of a round are in flight together.
-    for (int k = k0; k <= k1; k += kUnroll) {
-      float qx[kUnroll], qy[kUnroll], qz[kUnroll];
-      bool ok[kUnroll];
-      Vec<CG> gs[kUnroll];
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const float kf = static_cast<float>(k + u);
-        qx[u] = fmaf(kf, rr.dx, rr.sx) - 0.5f;
-        qy[u] = fmaf(kf, rr.dy, rr.sy) - 0.5f;
-        qz[u] = fmaf(kf, rr.dz, rr.sz) - 0.5f;
-        // in the forward mask, and some corner in this brick (floor <= b1, ceil >= b0)
-        ok[u] = (k + u <= k1) && (qx[u] + 0.5f >= 0.f) && (qx[u] + 0.5f < fdx) && (qy[u] + 0.5f >= 0.f) &&
-                (qy[u] + 0.5f < fdy) && (qz[u] + 0.5f >= 0.f) && (qz[u] + 0.5f < fdz) &&
-                !(ceilf(qx[u]) < fb0x || floorf(qx[u]) > fb1x || ceilf(qy[u]) < fb0y || floorf(qy[u]) > fb1y ||
-                  ceilf(qz[u]) < fb0z || floorf(qz[u]) > fb1z);
-        gs[u] = ok[u] ? vload<CG>(gp + static_cast<int64_t>(k + u) * npix * CG) : vzero<CG>();
-      }
-#pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        if (!ok[u]) continue;
-        const float fx = floorf(qx[u]), fy = floorf(qy[u]), fz = floorf(qz[u]);
-        const int ix0 = static_cast<int>(fx), iy0 = static_cast<int>(fy), iz0 = static_cast<int>(fz);
-        const int ix1 = static_cast<int>(ceilf(qx[u])), iy1 = static_cast<int>(ceilf(qy[u]));
-        const int iz1 = static_cast<int>(ceilf(qz[u]));
-        const float wx = qx[u] - fx, wy = qy[u] - fy, wz = qz[u] - fz;
-        const float fw[2][3] = {{1.f - wx, 1.f - wy, 1.f - wz}, {wx, wy, wz}};
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
-          const int ix = bx ? ix1 : ix0, iy = by ? iy1 : iy0, iz = bz ? iz1 : iz0;
-          if (ix < b0x || ix > b1x || iy < b0y || iy > b1y || iz < b0z || iz > b1z) continue;
-          const float w = fw[bx][0] * fw[by][1] * fw[bz][2];
-          float* dst = acc + (((iz - b0z) * BY + (iy - b0y)) * BX + (ix - b0x)) * CG;
-#pragma unroll
-          for (int i = 0; i < CG; ++i) atomicAdd(dst + i, gs[u].v[i] * w);
+        for (int ax = 0; ax < 3; ++ax) {
+          if (fabsf(dv[ax]) > 1e-30f) {
+            const float inv = 1.f / dv[ax];
+            const float t0 = (lo3[ax] - sv[ax]) * inv, t1 = (hi3[ax] - sv[ax]) * inv;
+            lo = fmaxf(lo, fminf(t0, t1));
+            hi = fminf(hi, fmaxf(t0, t1));
+          } else if (!(sv[ax] > lo3[ax] - 1e-3f && sv[ax] < hi3[ax] + 1e-3f)) {
+            miss = true;
+          }
+        }
+        if (!miss && lo <= hi + 2.f) {
+          const float fsteps = static_cast<float>(rr.steps);
+          k0 = max(0, static_cast<int>(floorf(fminf(fmaxf(lo, -1.f), fsteps))) - 1);
+          const int k1 = min(rr.steps - 1, static_cast<int>(floorf(fminf(fmaxf(hi, -2.f), fsteps))) + 1);
+          cnt = max(0, k1 - k0 + 1);
         }
       }
     }
+    // inclusive block scan of the counts
+    s_off[tid + 1] = block_inclusive_scan(cnt, s_wtot);
+    if (tid == 0) s_off[0] = 0;
+    s_k0[tid] = k0;
+    s_rec[tid] = rec_index;
+    __syncthreads();
+    const int items = s_off[256];
+    for (int i0 = 0; i0 < items; i0 += kItems) {
+      const int i_end = min(i0 + kItems, items);
+      for (int b = tid; b < NB; b += 256) bin_cnt[b] = 0;
+      __syncthreads();
+      // sample i -> (bin, weights, gradient); bin < 0 when it does not touch the brick
+      auto locate = [&](int i, float& wx, float& wy, float& wz, int& rec, int& k) -> int {
+        int lo_j = 0, hi_j = 256;
+        while (hi_j - lo_j > 1) {
+          const int mid = (lo_j + hi_j) >> 1;
+          if (s_off[mid] <= i) lo_j = mid; else hi_j = mid;
+        }
+        k = s_k0[lo_j] + (i - s_off[lo_j]);
+        rec = s_rec[lo_j];
+        const RayRec rr = rays[rec];
+        const float kf = static_cast<float>(k);
+        const float qx = fmaf(kf, rr.dx, rr.sx) - 0.5f;
+        const float qy = fmaf(kf, rr.dy, rr.sy) - 0.5f;
+        const float qz = fmaf(kf, rr.dz, rr.sz) - 0.5f;
+        const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+        wx = qx - fx; wy = qy - fy; wz = qz - fz;
+        const int lx = static_cast<int>(fmaxf(fminf(fx, 1e8f), -1e8f)) - fb0m1x;
+        const int ly = static_cast<int>(fmaxf(fminf(fy, 1e8f), -1e8f)) - fb0m1y;
+        const int lz = static_cast<int>(fmaxf(fminf(fz, 1e8f), -1e8f)) - fb0m1z;
+        // forward (BORDER) mask, floor cell in b0-1 .. b1 on every axis
+        const bool ok = (qx + 0.5f >= 0.f) & (qx + 0.5f < fdx) & (qy + 0.5f >= 0.f) & (qy + 0.5f < fdy) &
+                        (qz + 0.5f >= 0.f) & (qz + 0.5f < fdz) & (lx >= 0) & (lx <= b1x - fb0m1x) & (ly >= 0) &
+                        (ly <= b1y - fb0m1y) & (lz >= 0) & (lz <= b1z - fb0m1z);
+        return ok ? (lz * (BY + 1) + ly) * (BX + 1) + lx : -1;
+      };
+      for (int i = i0 + tid; i < i_end; i += 256) {
+        float wx, wy, wz;
+        int rec, k;
+        const int bin = locate(i, wx, wy, wz, rec, k);
+        if (bin >= 0) atomicAdd(&bin_cnt[bin], 1);
+      }
+      __syncthreads();
+      // exclusive scan of the bins: 3 per work-item, then across work-items
+      {
+        constexpr int kPer = (NB + 255) / 256;
+        int local[kPer];
+        int sum = 0;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+          const int b = tid * kPer + j;
+          local[j] = b < NB ? bin_cnt[b] : 0;
+          sum += local[j];
+        }
+        const int incl = block_inclusive_scan(sum, s_wtot);
+        int run = incl - sum;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+          const int b = tid * kPer + j;
+          if (b < NB) {
+            bin_start[b] = run;
+            bin_cnt[b] = 0;  // becomes the placement cursor
+          }
+          run += local[j];
+        }
+        if (tid == 255) bin_start[NB] = incl;
+      }
+      __syncthreads();
+      for (int i = i0 + tid; i < i_end; i += 256) {
+        float wx, wy, wz;
+        int rec, k;
+        const int bin = locate(i, wx, wy, wz, rec, k);
+        if (bin < 0) continue;
+        const int pos = bin_start[bin] + atomicAdd(&bin_cnt[bin], 1);
+        const Vec<CG> gs = vload<CG>(G + (static_cast<int64_t>(k) * npix + (rec % npix) +
+                                          static_cast<int64_t>(rec / npix) * kcap * npix) * CG);
+        srec[pos] = make_float4(wx, wy, wz, gs.v[0]);
+        if constexpr (CG > 1) {
+#pragma unroll
+          for (int ch = 0; ch < CG; ++ch) sg[pos * CG + ch] = gs.v[ch];
+        }
+      }
+      __syncthreads();
+      // owners: voxel v is corner d of the samples whose floor cell is v - d
+#pragma unroll
+      for (int o = 0; o < 2; ++o) {
+        const int v = tid + o * 256;
+        const int vx = v % BX, vy = (v / BX) % BY, vz = v / (BX * BY);
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+          const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
+          const int bin = ((vz + 1 - dz) * (BY + 1) + (vy + 1 - dy)) * (BX + 1) + (vx + 1 - dx);
+          const int e = bin_start[bin + 1];
+          for (int q = bin_start[bin]; q < e; ++q) {
+            const float4 rq = srec[q];
+            const float w = (dx ? rq.x : 1.f - rq.x) * (dy ? rq.y : 1.f - rq.y) * (dz ? rq.z : 1.f - rq.z);
+            if constexpr (CG == 1) {
+              accv[o][0] += rq.w * w;
+            } else {
+#pragma unroll
+              for (int ch = 0; ch < CG; ++ch) accv[o][ch] += sg[q * CG + ch] * w;
+            }
+          }
+        }
+      }
+      __syncthreads();  // bins and records are rewritten by the next sub-chunk
+    }
   }
+#pragma unroll
+  for (int o = 0; o < 2; ++o)
+#pragma unroll
+    for (int ch = 0; ch < CG; ++ch) acc[(tid + o * 256) * CG + ch] = accv[o][ch];
   __syncthreads();
   const int64_t base = static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z;
   for (int i = tid; i < NV; i += blockDim.x) {
@@ -740,7 +873,10 @@ int ray_step_cap(const std::vector<Pair>& pairs, const float* M, const float* V,
   const int hard_cap = 16 * (a.din.x + a.din.y + a.din.z + a.out_d);
   if (!(min_step < 1e300)) return hard_cap;
   const double k = std::ceil((diag + 2.0) / (0.97 * min_step)) + 3.0;
-  return static_cast<int>(std::min<double>(k, hard_cap));
+  // a multiple of kUnroll: the per-sample gradient rows are written kUnroll
+  // samples (16 bytes) at a time
+  const int cap = static_cast<int>(std::min<double>(k, hard_cap));
+  return (cap + kUnroll - 1) / kUnroll * kUnroll;
 }
 
 struct GatherPlan {
