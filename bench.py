@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--graph", action="store_true", help="replay the fused step as a captured HIP graph")
+    p.add_argument("--chain", action="store_true",
+                   help="instead of the single step: the 30-step checkpointed chain at 96x160x96 (render every 5th "
+                        "step, 5 views 240x135x96); prints its own JSON line (time, peak device memory)")
     p.add_argument("--traffic-probe", action="store_true",
                    help="run a 256 MiB copy and each op of the step once (for rocprofv3 --pmc FETCH_SIZE / "
                         "WRITE_SIZE runs) and exit")
@@ -321,8 +324,58 @@ def cpu_baseline():
             "seconds_per_step_est": float(t_step)}
 
 
+def chain_bench(steps, warmup):
+    """30 MacCormack steps of density + centered velocity at 96x160x96 (DHW),
+    5-view render (240x135 pixels, 96 slices) after every 5th step, L2 loss
+    against targets from a perturbed initial density; full backward to the
+    initial density and velocity.  Timed with and without recomputation
+    checkpointing; peak device memory above the inputs is reported."""
+    import torch
+    from ngt import synth, sequence, _lib
+    _lib.load()
+    grid_dhw, img_hw, depth = (96, 160, 96), (135, 240), 96
+    grid_t, cams = synth.ring_setup(grid_dhw, img_hw, depth, VIEWS)
+    spec = sequence.RenderSpec.from_cameras(grid_t, cams)
+    rho0 = torch.from_numpy(synth.blobs(grid_dhw, 16, seed=2)[None]).cuda()
+    vel0 = torch.from_numpy(synth.curl_noise_centered(grid_dhw, seed=4, vmax=2.0)[None]).cuda()
+    rho_t = torch.from_numpy(synth.blobs(grid_dhw, 16, seed=3)[None]).cuda()
+    targets = sequence.render_targets(rho_t, vel0, spec, 6, render_every=5)
+    res = {}
+    for ckpt in (True, False):
+        def run():
+            r = rho0.clone().requires_grad_()
+            v = vel0.clone().requires_grad_()
+            loss = sequence.chain_loss(r, v, spec, targets, render_every=5, checkpoint=ckpt)
+            loss.backward()
+            return loss
+        for _ in range(warmup):
+            run()
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        res["checkpoint" if ckpt else "stored"] = {
+            "ms_per_chain": e0.elapsed_time(e1) / steps,
+            "peak_device_bytes": int(torch.cuda.max_memory_allocated() - base)}
+    cells = int(np.prod(grid_dhw))
+    return {"workload": "30-step MacCormack chain (density + centered velocity, MC_SMOOTH, BORDER) at 96x160x96, "
+                        "5-view 240x135x96 render after every 5th step, L2 loss, full backward",
+            "chains": steps, "warmup": warmup, "results": res,
+            "voxel_updates_per_s_checkpoint": 30 * cells / (res["checkpoint"]["ms_per_chain"] * 1e-3),
+            "peak_budget_bytes": 8 << 30,
+            "within_budget": res["checkpoint"]["peak_device_bytes"] < (8 << 30)}
+
+
 def main():
     args = parse()
+    if args.chain:
+        print(json.dumps(chain_bench(max(1, args.steps // 5), min(args.warmup, 1))), flush=True)
+        return
     if args.traffic_summary:
         print(json.dumps(traffic_summary(*args.traffic_summary), indent=1))
         return

@@ -380,3 +380,21 @@ def test_raymarch_bricked_layout_is_bit_identical(ngt_ops, C, bound, mode):
         assert torch.equal(imgs[0], imgs[1])
     finally:
         ngt_ops.use_bricked_layout = old
+
+
+@pytest.mark.parametrize("clamp", ["NONE", "MC_SMOOTH"])
+def test_advect_three_channels_self_advection(ngt_ops, clamp):
+    """3-channel input (the centered velocity) advected by itself: the input
+    and velocity gradients of the same tensor add up."""
+    rng = np.random.default_rng(900 + len(clamp))
+    D, H, W = 9, 11, 10
+    vc = f32(np.stack([smooth_field(rng, (D, H, W), 1.6) for _ in range(3)], -1))
+    vt = dev(vc[None]).requires_grad_()
+    out = ngt_ops.advect_grid_semi_langrange(vt, vt, timestep=-1.0, order=2, clamp_extrema=clamp,
+                                             boundary="BORDER", separate_velocity_batch=False)
+    exp = A.warp_forward(vc, vc, 1.0, 2, clamp, BORDER)
+    assert rel(host(out)[0, 0], exp) < FWD_TOL
+    g = f32(rng.standard_normal(exp.shape))
+    out.backward(dev(g[None, None]))
+    eg_in, eg_v = A.warp_backward(vc, vc, g, 1.0, 2, clamp, BORDER)
+    assert rel(host(vt.grad)[0], eg_in + eg_v) < GRAD_TOL
