@@ -209,6 +209,70 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
                    float* input_grad, float* vel_x_grad, float* vel_y_grad, float* vel_z_grad, void* workspace,
                    size_t workspace_bytes, ngt_stream_t stream);
 
+/* ---- Uneven batches (no single reference op: the reference launches every
+ * (batch, camera) pair separately, resample_grid.cu.cc:748-790,
+ * raymarch_grid.cu.cc:399-430).  A list of instances of different sizes runs
+ * as one persistent launch per kernel over a work-list of cell tiles / bricks
+ * / ray tiles.  Usage: _plan_bytes -> _plan (fills a host buffer, returns the
+ * workspace size) -> copy the plan to device memory -> _fwd / _bwd with both
+ * copies (the host copy sizes the launches, the kernels read the device
+ * copy).  The plan holds the instances' device pointers: re-plan when a
+ * buffer moves.  blocks_per_cu: work-groups per compute unit of the
+ * persistent grid (0: one work-group per tile).  Launch-only, hence
+ * hipGraph-capturable. */
+typedef struct {
+  int32_t dhw[3];           /* D, H, W of this instance (1-channel density) */
+  float timestep;           /* samples at index + timestep * v (callers pass -dt) */
+  const float* input;       /* [D,H,W,1] */
+  const float* vel_x;       /* MAC faces [D,H,W+1] / [D,H+1,W] / [D+1,H,W], or */
+  const float* vel_y;       /* vel_x = centered [D,H,W,3] with vel_y = vel_z = NULL */
+  const float* vel_z;       /* (the same layout for every instance of a batch) */
+  float* output;            /* fwd: advected [D,H,W,1] */
+  float* sl;                /* fwd, order 2: semi-Lagrangian field, read again by the bwd */
+  const float* output_grad; /* bwd */
+  float* input_grad;        /* bwd: [D,H,W,1] */
+  float* vel_x_grad;        /* bwd: in the velocity's layout */
+  float* vel_y_grad;
+  float* vel_z_grad;
+} ngt_advect_instance;
+
+/* No counterpart in the reference; batched AdvectGridSemiLangrange
+ * (advect.cc:21-71) with VelocityGrid.warp semantics, exact gradients, C = 1. */
+size_t ngt_advect_batch_plan_bytes(int num_instances);
+int ngt_advect_batch_plan(const ngt_advect_instance* instances, int num_instances, int order, int clamp,
+                          int boundary, void* plan, size_t plan_bytes, size_t* workspace_bytes);
+int ngt_advect_batch_fwd(const void* plan_host, const void* plan_device, int blocks_per_cu, ngt_stream_t stream);
+int ngt_advect_batch_bwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                         int blocks_per_cu, ngt_stream_t stream);
+
+typedef struct {
+  int32_t dhw[3];           /* D, H, W of the density */
+  const float* density;     /* [D,H,W,1] */
+  int32_t num_views;        /* 1 .. 8 */
+  const float* matrix_m;    /* host: 16 floats (op layout, as ngt_render_density_fwd) */
+  const float* matrix_v;    /* host: 16 per view */
+  const float* matrix_p;    /* host: 16 per view */
+  const float* frustum;     /* host: 6 per view */
+  int32_t image_dhw[3];     /* samples per ray D', image H', W' (all views of the instance) */
+  float light_rgb[3];
+  float depth_step;
+  float* images;            /* fwd: [V,H',W',4] (RGB, optical depth) */
+  const float* images_grad; /* bwd */
+  float* density_grad;      /* bwd: [D,H,W,1] */
+} ngt_render_instance;
+
+/* No counterpart in the reference; batched fused density render
+ * (Renderer.render_density, renderer.py:718-827), LINEAR + BORDER, exact
+ * gradients by the brick-gather adjoint.  The plan holds the FP64-inverted
+ * camera matrices. */
+size_t ngt_render_batch_plan_bytes(int num_instances);
+int ngt_render_batch_plan(const ngt_render_instance* instances, int num_instances, int blending_mode, void* plan,
+                          size_t plan_bytes, size_t* workspace_bytes);
+int ngt_render_batch_fwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                         int blocks_per_cu, ngt_stream_t stream);
+int ngt_render_batch_bwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                         int blocks_per_cu, ngt_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

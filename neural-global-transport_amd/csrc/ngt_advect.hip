@@ -68,21 +68,21 @@ struct ThreadCell {
   bool ok;
 };
 
-NGT_DEVICE ThreadCell thread_cell(const AdvArgs& a) {
+NGT_DEVICE ThreadCell thread_cell(const AdvArgs& a, const VBlock& vb) {
   ThreadCell t;
-  t.x = blockIdx.x * kBX + threadIdx.x;
-  t.y = blockIdx.y * kBY + threadIdx.y;
-  t.z = blockIdx.z % a.d.z;
-  t.pair = blockIdx.z / a.d.z;
+  t.x = vb.x * kBX + threadIdx.x;
+  t.y = vb.y * kBY + threadIdx.y;
+  t.z = vb.z % a.d.z;
+  t.pair = vb.z / a.d.z;
   t.ok = t.x < a.d.x && t.y < a.d.y;
   return t;
 }
 
 // A: semi-Lagrangian step (also the whole order-1 warp).
 template <int C, int BM, bool NEG>
-__global__ __launch_bounds__(256) void advect_sl_kernel(AdvArgs a, const float* __restrict__ rho,
+NGT_DEVICE void advect_sl_kernel_body(const VBlock& vb, AdvArgs a, const float* __restrict__ rho,
                                                         float* __restrict__ out) {
-  const ThreadCell t = thread_cell(a);
+  const ThreadCell t = thread_cell(a, vb);
   if (!t.ok) return;
   int n, v;
   pair_indices(a, t.pair, n, v);
@@ -100,11 +100,17 @@ __global__ __launch_bounds__(256) void advect_sl_kernel(AdvArgs a, const float* 
   vstore<C>(out + (t.pair * cells + cell_offset(a.d, t.x, t.y, t.z)) * C, r);
 }
 
+template <int C, int BM, bool NEG>
+__global__ __launch_bounds__(256) void advect_sl_kernel(AdvArgs a, const float* __restrict__ rho,
+                                                        float* __restrict__ out) {
+  advect_sl_kernel_body<C, BM, NEG>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, rho, out);
+}
+
 // B: MacCormack correction + clamp.
 template <int C, int BM, int CLAMPM>
-__global__ __launch_bounds__(256) void advect_mc_kernel(AdvArgs a, const float* __restrict__ rho,
+NGT_DEVICE void advect_mc_kernel_body(const VBlock& vb, AdvArgs a, const float* __restrict__ rho,
                                                         const float* __restrict__ sl, float* __restrict__ out) {
-  const ThreadCell t = thread_cell(a);
+  const ThreadCell t = thread_cell(a, vb);
   if (!t.ok) return;
   int n, v;
   pair_indices(a, t.pair, n, v);
@@ -136,6 +142,12 @@ __global__ __launch_bounds__(256) void advect_mc_kernel(AdvArgs a, const float* 
   vstore<C>(out + (t.pair * cells + cell) * C, r);
 }
 
+template <int C, int BM, int CLAMPM>
+__global__ __launch_bounds__(256) void advect_mc_kernel(AdvArgs a, const float* __restrict__ rho,
+                                                        const float* __restrict__ sl, float* __restrict__ out) {
+  advect_mc_kernel_body<C, BM, CLAMPM>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, rho, sl, out);
+}
+
 // B1 (order 2): route g to the SL field and the direct rho term; scatter the
 // back-step adjoint into g_sl.  REF: scatter into tmp with hit counts.
 template <int C, int BM, int CLAMPM, bool REF>
@@ -145,7 +157,7 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_kernel(AdvArgs a, const flo
                                                             float* __restrict__ g_sl, float* __restrict__ g_rho,
                                                             float* __restrict__ g_vc, float* __restrict__ tmp,
                                                             uint32_t* __restrict__ hits) {
-  const ThreadCell t = thread_cell(a);
+  const ThreadCell t = thread_cell(a, VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)});
   if (!t.ok) return;
   int n, v;
   pair_indices(a, t.pair, n, v);
@@ -207,7 +219,7 @@ __global__ __launch_bounds__(256) void advect_sl_bwd_kernel(AdvArgs a, const flo
                                                             const float* __restrict__ sl_out,
                                                             float* __restrict__ g_rho, float* __restrict__ g_vc,
                                                             float* __restrict__ tmp, uint32_t* __restrict__ hits) {
-  const ThreadCell t = thread_cell(a);
+  const ThreadCell t = thread_cell(a, VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)});
   if (!t.ok) return;
   int n, v;
   pair_indices(a, t.pair, n, v);
@@ -246,7 +258,7 @@ template <int C>
 __global__ __launch_bounds__(256) void add_normalized_kernel(AdvArgs a, const float* __restrict__ tmp,
                                                              const uint32_t* __restrict__ hits,
                                                              float* __restrict__ dst, int to_data) {
-  const ThreadCell t = thread_cell(a);
+  const ThreadCell t = thread_cell(a, VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)});
   if (!t.ok) return;
   int n, v;
   pair_indices(a, t.pair, n, v);
@@ -263,14 +275,14 @@ __global__ __launch_bounds__(256) void add_normalized_kernel(AdvArgs a, const fl
 
 // B3: centered-velocity gradient -> MAC face gradients (adjoint of the face
 // average).  One work-item per face of each component.
-__global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, int n_vel, const float* __restrict__ g_vc,
+NGT_DEVICE void faces_from_centered_grad_kernel_body(const VBlock& vb, Dims d, int n_vel, const float* __restrict__ g_vc,
                                                                       float* __restrict__ gx, float* __restrict__ gy,
                                                                       float* __restrict__ gz) {
   const int64_t nx = static_cast<int64_t>(d.z) * d.y * (d.x + 1);
   const int64_t ny = static_cast<int64_t>(d.z) * (d.y + 1) * d.x;
   const int64_t nz = static_cast<int64_t>(d.z + 1) * d.y * d.x;
   const int64_t per_v = nx + ny + nz;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t i = static_cast<int64_t>(vb.x) * blockDim.x + threadIdx.x;
   if (i >= per_v * n_vel) return;
   const int v = static_cast<int>(i / per_v);
   int64_t j = i % per_v;
@@ -307,6 +319,12 @@ __global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, i
     if (z < d.z) s += g[cell_offset(d, x, y, z) * 3 + 2];
     gz[static_cast<int64_t>(v) * nz + j] = 0.5f * s;
   }
+}
+
+__global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, int n_vel, const float* __restrict__ g_vc,
+                                                                      float* __restrict__ gx, float* __restrict__ gy,
+                                                                      float* __restrict__ gz) {
+  faces_from_centered_grad_kernel_body(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, d, n_vel, g_vc, gx, gy, gz);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,10 +418,10 @@ struct BrickCell {
   int pair, brick, ox, oy, oz, x, y, z0;
 };
 
-NGT_DEVICE BrickCell brick_cell(const BrickGrid& bg) {
+NGT_DEVICE BrickCell brick_cell(const BrickGrid& bg, const VBlock& vb) {
   BrickCell b;
-  b.pair = blockIdx.x / bg.count();
-  b.brick = blockIdx.x % bg.count();
+  b.pair = vb.x / bg.count();
+  b.brick = vb.x % bg.count();
   const int bx = b.brick % bg.nx, by = (b.brick / bg.nx) % bg.ny, bz = b.brick / (bg.nx * bg.ny);
   b.x = bx * kBrickX + static_cast<int>(threadIdx.x);
   b.y = by * kBrickY + static_cast<int>(threadIdx.y);
@@ -446,7 +464,7 @@ NGT_DEVICE void add_vel_grad(float* gv, float gx, float gy, float gz, int mode) 
 // corrected value is kept (direct path to rho), the back-step adjoint into
 // the tile (overflow into ovf), and the back-step velocity gradient.
 template <int C, int BM, int CLAMPM>
-__global__ __launch_bounds__(256) void advect_mc_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
                                                                   const float* __restrict__ sl,
                                                                   const float* __restrict__ g_out,
                                                                   float* __restrict__ g_dir, float* __restrict__ r_dir,
@@ -454,7 +472,7 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_brick_kernel(AdvArgs a, Bri
                                                                   float* __restrict__ g_vc, int vel_mode) {
   __shared__ float tile[kTileCells * C];
   tile_clear<C>(tile);
-  const BrickCell b = brick_cell(bg);
+  const BrickCell b = brick_cell(bg, vb);
   int n, v;
   pair_indices(a, b.pair, n, v);
   const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
@@ -509,11 +527,21 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_brick_kernel(AdvArgs a, Bri
   tile_store<C>(tile, tiles, bg, b);
 }
 
+template <int C, int BM, int CLAMPM>
+__global__ __launch_bounds__(256) void advect_mc_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+                                                                  const float* __restrict__ sl,
+                                                                  const float* __restrict__ g_out,
+                                                                  float* __restrict__ g_dir, float* __restrict__ r_dir,
+                                                                  float* __restrict__ ovf, float* __restrict__ tiles,
+                                                                  float* __restrict__ g_vc, int vel_mode) {
+  advect_mc_bwd_brick_kernel_body<C, BM, CLAMPM>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, bg, rho, sl, g_out, g_dir, r_dir, ovf, tiles, g_vc, vel_mode);
+}
+
 // B2 brick form: g_sl = g_dir + ovf + tiles1 (order 2) or g_out with the
 // NEGATIVE mask (order 1); scatter into the tile (overflow straight into
 // g_rho, which starts at zero), SL velocity gradient.
 template <int C, int BM, bool NEG, bool ORDER2>
-__global__ __launch_bounds__(256) void advect_sl_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
                                                                   const float* __restrict__ g_in,
                                                                   const float* __restrict__ ovf1,
                                                                   const float* __restrict__ tiles1,
@@ -521,7 +549,7 @@ __global__ __launch_bounds__(256) void advect_sl_bwd_brick_kernel(AdvArgs a, Bri
                                                                   float* __restrict__ g_vc, int vel_mode) {
   __shared__ float tile[kTileCells * C];
   tile_clear<C>(tile);
-  const BrickCell b = brick_cell(bg);
+  const BrickCell b = brick_cell(bg, vb);
   int n, v;
   pair_indices(a, b.pair, n, v);
   const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
@@ -560,15 +588,25 @@ __global__ __launch_bounds__(256) void advect_sl_bwd_brick_kernel(AdvArgs a, Bri
   tile_store<C>(tile, tiles2, bg, b);
 }
 
+template <int C, int BM, bool NEG, bool ORDER2>
+__global__ __launch_bounds__(256) void advect_sl_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+                                                                  const float* __restrict__ g_in,
+                                                                  const float* __restrict__ ovf1,
+                                                                  const float* __restrict__ tiles1,
+                                                                  float* __restrict__ g_rho, float* __restrict__ tiles2,
+                                                                  float* __restrict__ g_vc, int vel_mode) {
+  advect_sl_bwd_brick_kernel_body<C, BM, NEG, ORDER2>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, bg, rho, g_in, ovf1, tiles1, g_rho, tiles2, g_vc, vel_mode);
+}
+
 // g_rho(n) += sum over the pairs of n of r_dir (order 2) + the B2 tiles.
 template <int C>
-__global__ __launch_bounds__(256) void advect_rho_gather_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ r_dir,
+NGT_DEVICE void advect_rho_gather_kernel_body(const VBlock& vb, AdvArgs a, BrickGrid bg, const float* __restrict__ r_dir,
                                                                 const float* __restrict__ tiles2,
                                                                 float* __restrict__ g_rho) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
-  const int z = blockIdx.z % a.d.z;
-  const int n = blockIdx.z / a.d.z;
+  const int x = vb.x * kBX + threadIdx.x;
+  const int y = vb.y * kBY + threadIdx.y;
+  const int z = vb.z % a.d.z;
+  const int n = vb.z / a.d.z;
   if (x >= a.d.x || y >= a.d.y) return;
   const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
   const int64_t cell = cell_offset(a.d, x, y, z);
@@ -586,6 +624,13 @@ __global__ __launch_bounds__(256) void advect_rho_gather_kernel(AdvArgs a, Brick
     for (int i = 0; i < C; ++i) s.v[i] += t.v[i];
   }
   vstore<C>(g_rho + (n * cells + cell) * C, s);
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void advect_rho_gather_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ r_dir,
+                                                                const float* __restrict__ tiles2,
+                                                                float* __restrict__ g_rho) {
+  advect_rho_gather_kernel_body<C>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, bg, r_dir, tiles2, g_rho);
 }
 
 #define NGT_SWITCH_C(C_, ...)                              \
@@ -659,6 +704,145 @@ void launch_faces(const AdvPlan& p, const float* g_vc, float* gx, float* gy, flo
                          static_cast<int64_t>(d.z + 1) * d.y * d.x) * p.a.n_vel;
   hipLaunchKernelGGL(faces_from_centered_grad_kernel, dim3(blocks_for(faces, 256)), dim3(256), 0, s, d, p.a.n_vel,
                      g_vc, gx, gy, gz);
+}
+
+// ---------------------------------------------------------------------------
+// Uneven batches: instances of different sizes (one density, one velocity
+// each) in one persistent launch per kernel.  The plan blob holds a header,
+// one AdvBatchItem per instance and the tile-range tables; its device copy is
+// what the kernels read.
+struct AdvBatchHeader {
+  int n, order, clamp, boundary, staggered;
+  int tiles_cell, tiles_brick, tiles_face, tiles_fill;
+  size_t off_items, off_cell, off_brick, off_face, off_fill;
+  int64_t ws_bytes;
+};
+
+struct AdvBatchItem {
+  AdvArgs a;
+  BrickGrid bg;
+  int nbx, nby;  // 64 x 4 cell tiles in x / y
+  int64_t cells;
+  const float* input;
+  float* output;
+  float* sl;
+  const float* g_out;
+  float* g_in;
+  float* gvx;
+  float* gvy;
+  float* gvz;
+  int64_t ws_gsl, ws_gvc, ws_t2, ws_rdir, ws_ovf, ws_t1;  // workspace offsets (floats)
+};
+
+constexpr int kFillChunk = 1024;
+
+struct AdvBatchView {
+  const AdvBatchHeader* h;
+  const AdvBatchItem* items;
+  const int* cell;
+  const int* brick;
+  const int* face;
+  const int* fill;
+};
+
+NGT_DEVICE AdvBatchView adv_batch_view(const void* plan) {
+  AdvBatchView v;
+  v.h = static_cast<const AdvBatchHeader*>(plan);
+  v.items = blob_at<AdvBatchItem>(plan, v.h->off_items);
+  v.cell = blob_at<int>(plan, v.h->off_cell);
+  v.brick = blob_at<int>(plan, v.h->off_brick);
+  v.face = blob_at<int>(plan, v.h->off_face);
+  v.fill = blob_at<int>(plan, v.h->off_fill);
+  return v;
+}
+
+NGT_DEVICE VBlock cell_tile(const AdvBatchItem& it, int local) {
+  return VBlock{local % it.nbx, (local / it.nbx) % it.nby, local / (it.nbx * it.nby)};
+}
+
+template <int BM, bool NEG, bool TO_SL>
+__global__ __launch_bounds__(256) void advect_sl_batch_kernel(const void* plan) {
+  const AdvBatchView v = adv_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
+    const int i = find_item(v.cell, v.h->n, t);
+    const AdvBatchItem& it = v.items[i];
+    advect_sl_kernel_body<1, BM, NEG>(cell_tile(it, t - v.cell[i]), it.a, it.input, TO_SL ? it.sl : it.output);
+  }
+}
+
+template <int BM, int CLAMPM>
+__global__ __launch_bounds__(256) void advect_mc_batch_kernel(const void* plan) {
+  const AdvBatchView v = adv_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
+    const int i = find_item(v.cell, v.h->n, t);
+    const AdvBatchItem& it = v.items[i];
+    advect_mc_kernel_body<1, BM, CLAMPM>(cell_tile(it, t - v.cell[i]), it.a, it.input, it.sl, it.output);
+  }
+}
+
+// zero input_grad (range 0) and ovf1 (range 1, order 2) of every instance
+__global__ __launch_bounds__(256) void advect_fill_batch_kernel(const void* plan, float* __restrict__ ws) {
+  const AdvBatchView v = adv_batch_view(plan);
+  const int ranges = v.h->order == 2 ? 2 : 1;
+  for (int t = blockIdx.x; t < v.h->tiles_fill; t += gridDim.x) {
+    const int i = find_item(v.fill, v.h->n, t);
+    const AdvBatchItem& it = v.items[i];
+    const int local = t - v.fill[i];
+    const int chunks = static_cast<int>((it.cells + kFillChunk - 1) / kFillChunk);
+    const int range = local / chunks;
+    if (range >= ranges) continue;
+    float* dst = range == 0 ? it.g_in : ws + it.ws_ovf;
+    const int64_t b = static_cast<int64_t>(local % chunks) * kFillChunk;
+    for (int64_t e = b + threadIdx.x; e < min(b + kFillChunk, it.cells); e += blockDim.x) dst[e] = 0.f;
+  }
+}
+
+template <int BM, int CLAMPM>
+__global__ __launch_bounds__(256) void advect_mc_bwd_batch_kernel(const void* plan, float* __restrict__ ws) {
+  const AdvBatchView v = adv_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
+    const int i = find_item(v.brick, v.h->n, t);
+    const AdvBatchItem& it = v.items[i];
+    float* g_vc = v.h->staggered ? ws + it.ws_gvc : it.gvx;
+    advect_mc_bwd_brick_kernel_body<1, BM, CLAMPM>(VBlock{t - v.brick[i], 0, 0}, it.a, it.bg, it.input, it.sl,
+                                                   it.g_out, ws + it.ws_gsl, ws + it.ws_rdir, ws + it.ws_ovf,
+                                                   ws + it.ws_t1, g_vc, 1);
+    __syncthreads();  // the LDS tile is cleared again by the next brick
+  }
+}
+
+template <int BM, bool NEG, bool ORDER2>
+__global__ __launch_bounds__(256) void advect_sl_bwd_batch_kernel(const void* plan, float* __restrict__ ws) {
+  const AdvBatchView v = adv_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
+    const int i = find_item(v.brick, v.h->n, t);
+    const AdvBatchItem& it = v.items[i];
+    float* g_vc = v.h->staggered ? ws + it.ws_gvc : it.gvx;
+    advect_sl_bwd_brick_kernel_body<1, BM, NEG, ORDER2>(VBlock{t - v.brick[i], 0, 0}, it.a, it.bg, it.input,
+                                                        ORDER2 ? ws + it.ws_gsl : it.g_out, ws + it.ws_ovf,
+                                                        ws + it.ws_t1, it.g_in, ws + it.ws_t2, g_vc, ORDER2 ? 1 : 0);
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void advect_rho_gather_batch_kernel(const void* plan, float* __restrict__ ws) {
+  const AdvBatchView v = adv_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
+    const int i = find_item(v.cell, v.h->n, t);
+    const AdvBatchItem& it = v.items[i];
+    advect_rho_gather_kernel_body<1>(cell_tile(it, t - v.cell[i]), it.a, it.bg,
+                                     v.h->order == 2 ? ws + it.ws_rdir : nullptr, ws + it.ws_t2, it.g_in);
+  }
+}
+
+__global__ __launch_bounds__(256) void advect_faces_batch_kernel(const void* plan, float* __restrict__ ws) {
+  const AdvBatchView v = adv_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_face; t += gridDim.x) {
+    const int i = find_item(v.face, v.h->n, t);
+    const AdvBatchItem& it = v.items[i];
+    faces_from_centered_grad_kernel_body(VBlock{t - v.face[i], 0, 0}, it.a.d, 1, ws + it.ws_gvc, it.gvx, it.gvy,
+                                         it.gvz);
+  }
 }
 
 }  // namespace
@@ -828,6 +1012,152 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
   }
   if (p.a.staggered) launch_faces(p, g_vc, vel_x_grad, vel_y_grad, vel_z_grad, s);
   return check_launch("ngt_advect_bwd");
+}
+
+
+size_t ngt_advect_batch_plan_bytes(int num_instances) {
+  if (num_instances <= 0) return 0;
+  BlobLayout L;
+  L.add<AdvBatchHeader>(1);
+  L.add<AdvBatchItem>(num_instances);
+  for (int k = 0; k < 4; ++k) L.add<int>(num_instances + 1);
+  return L.bytes;
+}
+
+int ngt_advect_batch_plan(const ngt_advect_instance* inst, int n, int order, int clamp, int boundary, void* plan,
+                          size_t plan_bytes, size_t* workspace_bytes) {
+  NGT_REQUIRE(inst != nullptr && n > 0, NGT_ERR_INVALID_ARGUMENT, "batch must not be empty");
+  NGT_REQUIRE(order == 1 || order == 2, NGT_ERR_INVALID_ARGUMENT, "Unsupported warp order");
+  NGT_REQUIRE(clamp >= 0 && clamp <= 2, NGT_ERR_INVALID_ARGUMENT, "invalid clamp mode");
+  NGT_REQUIRE(boundary >= 0 && boundary <= 2, NGT_ERR_INVALID_ARGUMENT, "invalid boundary");
+  NGT_REQUIRE(plan != nullptr && plan_bytes >= ngt_advect_batch_plan_bytes(n), NGT_ERR_WORKSPACE,
+              "plan buffer too small for ngt_advect_batch_plan");
+  BlobLayout L;
+  const size_t off_h = L.add<AdvBatchHeader>(1);
+  const size_t off_items = L.add<AdvBatchItem>(n);
+  const size_t off_cell = L.add<int>(n + 1), off_brick = L.add<int>(n + 1), off_face = L.add<int>(n + 1),
+               off_fill = L.add<int>(n + 1);
+  AdvBatchHeader* h = blob_at<AdvBatchHeader>(plan, off_h);
+  AdvBatchItem* items = blob_at<AdvBatchItem>(plan, off_items);
+  int* cell = blob_at<int>(plan, off_cell);
+  int* brick = blob_at<int>(plan, off_brick);
+  int* face = blob_at<int>(plan, off_face);
+  int* fill = blob_at<int>(plan, off_fill);
+  const int staggered = inst[0].vel_y != nullptr ? 1 : 0;
+  int64_t tc = 0, tb = 0, tf = 0, tl = 0, ws = 0;
+  cell[0] = brick[0] = face[0] = fill[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    const ngt_advect_instance& q = inst[i];
+    NGT_REQUIRE(q.dhw[0] > 0 && q.dhw[1] > 0 && q.dhw[2] > 0, NGT_ERR_INVALID_ARGUMENT, "empty batch instance");
+    NGT_REQUIRE(q.input && q.vel_x, NGT_ERR_INVALID_ARGUMENT, "batch instance without input or velocity");
+    NGT_REQUIRE((q.vel_y != nullptr) == (staggered != 0) && (q.vel_z != nullptr) == (staggered != 0),
+                NGT_ERR_INVALID_ARGUMENT, "all batch instances must use the same velocity layout");
+    const int64_t sh[5] = {1, q.dhw[0], q.dhw[1], q.dhw[2], 1};
+    int rc = check_grid_shape(sh, "batch instance");
+    if (rc) return rc;
+    AdvBatchItem& it = items[i];
+    it.a.d = Dims{q.dhw[2], q.dhw[1], q.dhw[0]};
+    it.a.n_data = 1; it.a.n_vel = 1; it.a.separate = 0;
+    it.a.ts = q.timestep;
+    it.a.vx = q.vel_x; it.a.vy = q.vel_y; it.a.vz = q.vel_z;
+    it.a.staggered = staggered;
+    it.bg = brick_grid(it.a.d);
+    it.nbx = static_cast<int>(blocks_for(it.a.d.x, kBX));
+    it.nby = static_cast<int>(blocks_for(it.a.d.y, kBY));
+    it.cells = static_cast<int64_t>(q.dhw[0]) * q.dhw[1] * q.dhw[2];
+    it.input = q.input; it.output = q.output; it.sl = q.sl;
+    it.g_out = q.output_grad; it.g_in = q.input_grad;
+    it.gvx = q.vel_x_grad; it.gvy = q.vel_y_grad; it.gvz = q.vel_z_grad;
+    const int64_t tiles = static_cast<int64_t>(it.bg.count()) * kTileCells;
+    auto take = [&](int64_t floats) {
+      const int64_t off = ws;
+      ws += (floats + 63) / 64 * 64;  // 256-byte aligned regions
+      return off;
+    };
+    it.ws_gsl = order == 2 ? take(it.cells) : 0;
+    it.ws_gvc = staggered ? take(it.cells * 3) : 0;
+    it.ws_t2 = take(tiles);
+    it.ws_rdir = order == 2 ? take(it.cells) : 0;
+    it.ws_ovf = order == 2 ? take(it.cells) : 0;
+    it.ws_t1 = order == 2 ? take(tiles) : 0;
+    tc += static_cast<int64_t>(it.nbx) * it.nby * q.dhw[0];
+    tb += it.bg.count();
+    if (staggered) {
+      const Dims& d = it.a.d;
+      const int64_t faces = static_cast<int64_t>(d.z) * d.y * (d.x + 1) + static_cast<int64_t>(d.z) * (d.y + 1) * d.x +
+                            static_cast<int64_t>(d.z + 1) * d.y * d.x;
+      tf += blocks_for(faces, 256);
+    }
+    tl += static_cast<int64_t>(order == 2 ? 2 : 1) * ((it.cells + kFillChunk - 1) / kFillChunk);
+    NGT_REQUIRE(tc < (int64_t(1) << 31) && tl < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT, "batch too large");
+    cell[i + 1] = static_cast<int>(tc); brick[i + 1] = static_cast<int>(tb);
+    face[i + 1] = static_cast<int>(tf); fill[i + 1] = static_cast<int>(tl);
+  }
+  h->n = n; h->order = order; h->clamp = clamp; h->boundary = boundary; h->staggered = staggered;
+  h->tiles_cell = static_cast<int>(tc); h->tiles_brick = static_cast<int>(tb);
+  h->tiles_face = static_cast<int>(tf); h->tiles_fill = static_cast<int>(tl);
+  h->off_items = off_items; h->off_cell = off_cell; h->off_brick = off_brick; h->off_face = off_face;
+  h->off_fill = off_fill;
+  h->ws_bytes = ws * static_cast<int64_t>(sizeof(float));
+  if (workspace_bytes) *workspace_bytes = static_cast<size_t>(h->ws_bytes);
+  return NGT_OK;
+}
+
+int ngt_advect_batch_fwd(const void* plan_host, const void* plan_device, int blocks_per_cu, ngt_stream_t stream) {
+  NGT_REQUIRE(plan_host && plan_device, NGT_ERR_INVALID_ARGUMENT, "plan must be given");
+  const AdvBatchHeader& h = *static_cast<const AdvBatchHeader*>(plan_host);
+  const AdvBatchItem* items = blob_at<AdvBatchItem>(plan_host, h.off_items);
+  for (int i = 0; i < h.n; ++i)
+    NGT_REQUIRE(items[i].output && (h.order == 1 || items[i].sl), NGT_ERR_INVALID_ARGUMENT,
+                "batch instance without output (or sl buffer for order 2)");
+  hipStream_t s = as_stream(stream);
+  const dim3 g(persistent_grid(h.tiles_cell, blocks_per_cu)), b(kBX, kBY);
+  if (h.order == 1) {
+    const bool neg = h.clamp == NGT_CLAMP_NEGATIVE;
+    NGT_SWITCH_BM(h.boundary,
+        if (neg) hipLaunchKernelGGL((advect_sl_batch_kernel<BB, true, false>), g, b, 0, s, plan_device);
+        else hipLaunchKernelGGL((advect_sl_batch_kernel<BB, false, false>), g, b, 0, s, plan_device));
+  } else {
+    NGT_SWITCH_BM(h.boundary,
+        hipLaunchKernelGGL((advect_sl_batch_kernel<BB, false, true>), g, b, 0, s, plan_device);
+        NGT_SWITCH_CLAMP(h.clamp, hipLaunchKernelGGL((advect_mc_batch_kernel<BB, KK>), g, b, 0, s, plan_device)));
+  }
+  return check_launch("ngt_advect_batch_fwd");
+}
+
+int ngt_advect_batch_bwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                         int blocks_per_cu, ngt_stream_t stream) {
+  NGT_REQUIRE(plan_host && plan_device, NGT_ERR_INVALID_ARGUMENT, "plan must be given");
+  const AdvBatchHeader& h = *static_cast<const AdvBatchHeader*>(plan_host);
+  NGT_REQUIRE(workspace != nullptr && workspace_bytes >= static_cast<size_t>(h.ws_bytes), NGT_ERR_WORKSPACE,
+              "workspace too small for ngt_advect_batch_bwd");
+  const AdvBatchItem* items = blob_at<AdvBatchItem>(plan_host, h.off_items);
+  for (int i = 0; i < h.n; ++i)
+    NGT_REQUIRE(items[i].g_out && items[i].g_in && items[i].gvx && (!h.staggered || (items[i].gvy && items[i].gvz)) &&
+                    (h.order == 1 || items[i].sl),
+                NGT_ERR_INVALID_ARGUMENT, "batch instance without gradient buffers (or sl for order 2)");
+  hipStream_t s = as_stream(stream);
+  float* ws = static_cast<float*>(workspace);
+  const dim3 b(kBX, kBY);
+  hipLaunchKernelGGL(advect_fill_batch_kernel, dim3(persistent_grid(h.tiles_fill, blocks_per_cu)), dim3(256), 0, s,
+                     plan_device, ws);
+  const dim3 gb(persistent_grid(h.tiles_brick, blocks_per_cu)), bb(kBrickX, kBrickY);
+  if (h.order == 2) {
+    NGT_SWITCH_BM(h.boundary, NGT_SWITCH_CLAMP(h.clamp,
+        hipLaunchKernelGGL((advect_mc_bwd_batch_kernel<BB, KK>), gb, bb, 0, s, plan_device, ws);
+        hipLaunchKernelGGL((advect_sl_bwd_batch_kernel<BB, false, true>), gb, bb, 0, s, plan_device, ws)));
+  } else {
+    const bool neg = h.clamp == NGT_CLAMP_NEGATIVE;
+    NGT_SWITCH_BM(h.boundary,
+        if (neg) hipLaunchKernelGGL((advect_sl_bwd_batch_kernel<BB, true, false>), gb, bb, 0, s, plan_device, ws);
+        else hipLaunchKernelGGL((advect_sl_bwd_batch_kernel<BB, false, false>), gb, bb, 0, s, plan_device, ws));
+  }
+  hipLaunchKernelGGL(advect_rho_gather_batch_kernel, dim3(persistent_grid(h.tiles_cell, blocks_per_cu)), b, 0, s,
+                     plan_device, ws);
+  if (h.staggered)
+    hipLaunchKernelGGL(advect_faces_batch_kernel, dim3(persistent_grid(h.tiles_face, blocks_per_cu)), dim3(256), 0, s,
+                       plan_device, ws);
+  return check_launch("ngt_advect_batch_bwd");
 }
 
 }  // extern "C"

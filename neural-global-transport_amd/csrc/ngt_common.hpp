@@ -110,6 +110,13 @@ NGT_DEVICE float vdot(const Vec<C>& a, const Vec<C>& b) {
 // Hardware fp32 atomic add (global_atomic_add_f32, no CAS loop).
 NGT_DEVICE void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 
+// Block coordinates of a work-group's tile.  Kernel bodies take them as an
+// argument so the same body runs under a plain grid launch (blockIdx) and
+// under the persistent work-list launches of uneven batches.
+struct VBlock {
+  int x, y, z;
+};
+
 // ------------------------------------------------------------------- grid ---
 
 struct Dims {

@@ -161,6 +161,16 @@ void fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(fill_u32_kernel, dim3(b), dim3(256), 0, s, p, n, 0u);
 }
 
+unsigned persistent_grid(int64_t tiles, int blocks_per_cu) {
+  if (tiles <= 0) return 1;
+  if (blocks_per_cu <= 0) return static_cast<unsigned>(tiles);
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  return static_cast<unsigned>(std::min<int64_t>(tiles, static_cast<int64_t>(cus) * blocks_per_cu));
+}
+
 }  // namespace ngt
 
 extern "C" {

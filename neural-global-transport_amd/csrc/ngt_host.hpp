@@ -11,6 +11,8 @@
 #include "ngt_common.hpp"
 #include "ngt_transport.h"
 
+#define NGT_HOST_DEVICE_INLINE __host__ __device__ inline
+
 namespace ngt {
 
 // Thread-local message of the last failure (returned by ngt_last_error).
@@ -68,5 +70,46 @@ void fill_zero(float* p, int64_t n, hipStream_t s);
 void fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s);
 
 inline unsigned blocks_for(int64_t n, int threads) { return static_cast<unsigned>((n + threads - 1) / threads); }
+
+// ------------------------------------------------------------ work lists ---
+// Uneven batches run one persistent launch per kernel: a work-group takes
+// tiles t = blockIdx.x, blockIdx.x + gridDim.x, ... of a flat tile list whose
+// per-instance ranges are begin[i] .. begin[i+1].
+
+// Instance owning tile t (begin is non-decreasing, begin[0] = 0).
+NGT_DEVICE int find_item(const int* __restrict__ begin, int n, int t) {
+  int lo = 0, hi = n;  // begin[lo] <= t < begin[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (begin[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Grid size of a persistent launch: blocks_per_cu work-groups per compute
+// unit of the current device (0: one work-group per tile), at most the tiles.
+unsigned persistent_grid(int64_t tiles, int blocks_per_cu);
+
+// Byte layout of a plan blob (identical on the host and the device copy).
+struct BlobLayout {
+  size_t bytes = 0;
+  template <typename T>
+  size_t add(size_t count) {
+    const size_t align = alignof(T) < 16 ? 16 : alignof(T);
+    bytes = (bytes + align - 1) / align * align;
+    const size_t off = bytes;
+    bytes += count * sizeof(T);
+    return off;
+  }
+};
+
+template <typename T>
+NGT_HOST_DEVICE_INLINE T* blob_at(void* base, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(base) + off);
+}
+template <typename T>
+NGT_HOST_DEVICE_INLINE const T* blob_at(const void* base, size_t off) {
+  return reinterpret_cast<const T*>(static_cast<const char*>(base) + off);
+}
 
 }  // namespace ngt

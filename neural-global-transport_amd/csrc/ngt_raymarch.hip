@@ -102,11 +102,11 @@ NGT_DEVICE Ray make_ray(const RayArgs& a, const CamParams& cp, int x, int y) {
 }
 
 // Work-group 16x16 -> four 8x8 wavefront tiles.
-NGT_DEVICE void pixel_of_thread(int& x, int& y) {
+NGT_DEVICE void pixel_of_thread(const VBlock& vb, int& x, int& y) {
   const int tid = threadIdx.x;  // 0..255
   const int wave = tid >> 6, lane = tid & 63;
-  x = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-  y = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+  x = vb.x * 16 + (wave & 1) * 8 + (lane & 7);
+  y = vb.y * 16 + (wave >> 1) * 8 + (lane >> 3);
 }
 
 // Cell value read by a sample: the C-channel input (LIGHT = false) or the
@@ -185,11 +185,11 @@ NGT_DEVICE void scatter_cell(const RayArgs& a, const float* __restrict__ src, fl
 }
 
 template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
-__global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch cb, float* __restrict__ out) {
+NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, RayArgs a, CamBatch cb, float* __restrict__ out) {
   int x, y;
-  pixel_of_thread(x, y);
+  pixel_of_thread(vb, x, y);
   if (x >= a.out_w || y >= a.out_h) return;
-  const CamParams& cp = cb.cam[blockIdx.z];
+  const CamParams& cp = cb.cam[vb.z];
   constexpr int CIN = LIGHT ? 1 : C;
   const float* src = ray_source<CIN, BRICKED>(a, cp.data_index);
   const Ray r = make_ray<BM>(a, cp, x, y);
@@ -212,6 +212,11 @@ __global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch c
       if (k + u < r.steps) blend_step<C, MODE>(acc, cells[u]);
   }
   vstore<C>(out + ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C, acc);
+}
+
+template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
+__global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch cb, float* __restrict__ out) {
+  raymarch_fwd_kernel_body<C, FM, BM, MODE, LIGHT, BRICKED>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, out);
 }
 
 // Per-sample cell gradients of one ray (exact adjoint of the forward walk),
@@ -313,7 +318,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
                                                                  const float* __restrict__ out_grad,
                                                                  float* __restrict__ in_grad) {
   int x, y;
-  pixel_of_thread(x, y);
+  pixel_of_thread(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, x, y);
   if (x >= a.out_w || y >= a.out_h) return;
   const CamParams& cp = cb.cam[blockIdx.z];
   constexpr int CIN = LIGHT ? 1 : C;
@@ -349,21 +354,21 @@ struct RayRec {
 };
 
 template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
-__global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, CamBatch cb, int pair_base,
+NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, RayArgs a, CamBatch cb, int pair_base,
                                                                    const float* __restrict__ out,
                                                                    const float* __restrict__ out_grad,
                                                                    float* __restrict__ G, RayRec* __restrict__ rays,
                                                                    int kcap) {
   int x, y;
-  pixel_of_thread(x, y);
+  pixel_of_thread(vb, x, y);
   if (x >= a.out_w || y >= a.out_h) return;
-  const CamParams& cp = cb.cam[blockIdx.z];
+  const CamParams& cp = cb.cam[vb.z];
   constexpr int CIN = LIGHT ? 1 : C;
   constexpr int CG = GradChannels<C, LIGHT>::value;
   const float* src = ray_source<CIN, BRICKED>(a, cp.data_index);
   Ray r = make_ray<BM>(a, cp, x, y);
   r.steps = min(r.steps, kcap);
-  const int pair = pair_base + static_cast<int>(blockIdx.z);
+  const int pair = pair_base + static_cast<int>(vb.z);
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
   const int64_t pixel = static_cast<int64_t>(y) * a.out_w + x;
   RayRec rec;
@@ -383,6 +388,15 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
     }
   } sink{a, G + static_cast<int64_t>(pair) * kcap * npix * CG, npix, pixel};
   walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT, BRICKED>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
+}
+
+template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
+__global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, CamBatch cb, int pair_base,
+                                                                   const float* __restrict__ out,
+                                                                   const float* __restrict__ out_grad,
+                                                                   float* __restrict__ G, RayRec* __restrict__ rays,
+                                                                   int kcap) {
+  raymarch_bwd_prepass_kernel_body<C, BM, MODE, LIGHT, BRICKED>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, out, out_grad, G, rays, kcap);
 }
 
 // Inclusive scan of one int per work-item over a 256-wide work-group: a
@@ -413,7 +427,7 @@ NGT_DEVICE int block_inclusive_scan(int v, int* s_wtot) {
 // stores; a sample on a brick boundary is visited by each brick it touches
 // and contributes only that brick's corners.
 template <int CG, int BX, int BY, int BZ>
-__global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamBatch cb, int pair_base,
+NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, RayArgs a, CamBatch cb, int pair_base,
                                                                  const RayRec* __restrict__ rays,
                                                                  const float* __restrict__ G, int kcap,
                                                                  float* __restrict__ in_grad, int accumulate) {
@@ -425,7 +439,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamB
   const int tid = threadIdx.x;
   const int bricks_x = (a.din.x + BX - 1) / BX, bricks_y = (a.din.y + BY - 1) / BY;
   const int bricks_z = (a.din.z + BZ - 1) / BZ;
-  const int bid = blockIdx.x;
+  const int bid = vb.x;
   const int n = bid / (bricks_x * bricks_y * bricks_z);
   const int brem = bid % (bricks_x * bricks_y * bricks_z);
   const int b0x = (brem % bricks_x) * BX;
@@ -676,6 +690,14 @@ __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamB
   }
 }
 
+template <int CG, int BX, int BY, int BZ>
+__global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamBatch cb, int pair_base,
+                                                                 const RayRec* __restrict__ rays,
+                                                                 const float* __restrict__ G, int kcap,
+                                                                 float* __restrict__ in_grad, int accumulate) {
+  raymarch_bwd_brick_kernel_body<CG, BX, BY, BZ>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, rays, G, kcap, in_grad, accumulate);
+}
+
 // Reference gradient of one camera: reverse walk from `end` with the
 // reference's reconstruction, scatter with corner weights into a per-camera
 // buffer (raymarch_grid.cu.cc:482-521).
@@ -686,7 +708,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_reference_kernel(RayArgs a, 
                                                                      float* __restrict__ gbuf,
                                                                      float* __restrict__ wsum) {
   int x, y;
-  pixel_of_thread(x, y);
+  pixel_of_thread(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, x, y);
   if (x >= a.out_w || y >= a.out_h) return;
   const CamParams& cp = cb.cam[blockIdx.z];
   const float* src = a.input + static_cast<int64_t>(cp.data_index) * a.din.x * a.din.y * a.din.z * C;
@@ -802,16 +824,22 @@ void for_cam_batches(const std::vector<Pair>& pairs, const float* M, const float
 // Copy of an NDHWC grid in the bricked slice layout (one work-item per cell,
 // coalesced reads; a brick's 64 cells are written as 16-byte runs).
 template <int C>
-__global__ __launch_bounds__(256) void to_bricked_kernel(const float* __restrict__ in, Dims d, BrickDims b,
+NGT_DEVICE void to_bricked_kernel_body(const VBlock& vb, const float* __restrict__ in, Dims d, BrickDims b,
                                                          float* __restrict__ out) {
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y = blockIdx.y * 4 + threadIdx.y;
-  const int z = blockIdx.z % d.z;
-  const int n = blockIdx.z / d.z;
+  const int x = vb.x * 64 + threadIdx.x;
+  const int y = vb.y * 4 + threadIdx.y;
+  const int z = vb.z % d.z;
+  const int n = vb.z / d.z;
   if (x >= d.x || y >= d.y) return;
   const int64_t cells = static_cast<int64_t>(d.x) * d.y * d.z;
   const Vec<C> v = vload<C>(in + (n * cells + cell_offset(d, x, y, z)) * C);
   vstore<C>(out + (static_cast<int64_t>(n) * b.slice_cells + brick_offset(b, x, y, z)) * C, v);
+}
+
+template <int C>
+__global__ __launch_bounds__(256) void to_bricked_kernel(const float* __restrict__ in, Dims d, BrickDims b,
+                                                         float* __restrict__ out) {
+  to_bricked_kernel_body<C>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, in, d, b, out);
 }
 
 void launch_to_bricked(const float* in, const Dims& d, const BrickDims& b, int n_data, int C, float* out,
@@ -953,6 +981,102 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
     NGT_SWITCH_C(CG,
         hipLaunchKernelGGL((raymarch_bwd_brick_kernel<CC, BX, BY, BZ>), dim3(bricks * n_data), dim3(256), 0, s, a, cb,
                            base, gp.rays, gp.G, gp.kcap, in_grad, accumulate));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Uneven batches of fused density renders: every instance (one density grid,
+// 1-8 views of one image size) in one persistent launch per kernel.
+struct RenBatchHeader {
+  int n, mode;
+  int tiles_ray, tiles_brickify, tiles_brick;
+  size_t off_items, off_ray, off_brickify, off_brick;
+  int64_t ws_bytes;
+};
+
+struct RenBatchItem {
+  RayArgs a;
+  CamBatch cb;
+  int ntx, nty;        // 16 x 16 pixel tiles per view
+  int kcap;
+  int nbx64, nby4;     // tiles of the bricked copy
+  float* images;
+  const float* images_grad;
+  float* density_grad;
+  int64_t ws_bricked, ws_rays, ws_G;  // byte offsets into the workspace
+};
+
+struct RenBatchView {
+  const RenBatchHeader* h;
+  const RenBatchItem* items;
+  const int* ray;
+  const int* brickify;
+  const int* brick;
+};
+
+NGT_DEVICE RenBatchView ren_batch_view(const void* plan) {
+  RenBatchView v;
+  v.h = static_cast<const RenBatchHeader*>(plan);
+  v.items = blob_at<RenBatchItem>(plan, v.h->off_items);
+  v.ray = blob_at<int>(plan, v.h->off_ray);
+  v.brickify = blob_at<int>(plan, v.h->off_brickify);
+  v.brick = blob_at<int>(plan, v.h->off_brick);
+  return v;
+}
+
+NGT_DEVICE VBlock ray_tile(const RenBatchItem& it, int local) {
+  const int per_view = it.ntx * it.nty;
+  const int view = local / per_view, r = local % per_view;
+  return VBlock{r % it.ntx, r / it.ntx, view};
+}
+
+__global__ __launch_bounds__(256) void render_brickify_batch_kernel(const void* plan, char* __restrict__ ws) {
+  const RenBatchView v = ren_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_brickify; t += gridDim.x) {
+    const int i = find_item(v.brickify, v.h->n, t);
+    const RenBatchItem& it = v.items[i];
+    const int local = t - v.brickify[i];
+    const VBlock vb{local % it.nbx64, (local / it.nbx64) % it.nby4, local / (it.nbx64 * it.nby4)};
+    to_bricked_kernel_body<1>(vb, it.a.input, it.a.din, it.a.bd, reinterpret_cast<float*>(ws + it.ws_bricked));
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void render_fwd_batch_kernel(const void* plan, const char* __restrict__ ws) {
+  const RenBatchView v = ren_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_ray; t += gridDim.x) {
+    const int i = find_item(v.ray, v.h->n, t);
+    const RenBatchItem& it = v.items[i];
+    RayArgs a = it.a;
+    a.bricked = reinterpret_cast<const float*>(ws + it.ws_bricked);
+    raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(ray_tile(it, t - v.ray[i]), a, it.cb, it.images);
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void render_prepass_batch_kernel(const void* plan, char* __restrict__ ws) {
+  const RenBatchView v = ren_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_ray; t += gridDim.x) {
+    const int i = find_item(v.ray, v.h->n, t);
+    const RenBatchItem& it = v.items[i];
+    RayArgs a = it.a;
+    a.bricked = reinterpret_cast<const float*>(ws + it.ws_bricked);
+    raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(
+        ray_tile(it, t - v.ray[i]), a, it.cb, 0, it.images, it.images_grad,
+        reinterpret_cast<float*>(ws + it.ws_G), reinterpret_cast<RayRec*>(ws + it.ws_rays), it.kcap);
+  }
+}
+
+__global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* plan, char* __restrict__ ws) {
+  const RenBatchView v = ren_batch_view(plan);
+  for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
+    const int i = find_item(v.brick, v.h->n, t);
+    const RenBatchItem& it = v.items[i];
+    raymarch_bwd_brick_kernel_body<1, 8, 8, 8>(VBlock{t - v.brick[i], 0, 0}, it.a, it.cb, 0,
+                                               reinterpret_cast<const RayRec*>(ws + it.ws_rays),
+                                               reinterpret_cast<const float*>(ws + it.ws_G), it.kcap,
+                                               it.density_grad, 0);
+    __syncthreads();  // the brick's LDS arrays are rewritten by the next brick
   }
 }
 
@@ -1147,6 +1271,122 @@ int ngt_render_density_bwd(const float* density, const int64_t* density_shape, c
     });
   }
   return check_launch("ngt_render_density_bwd");
+}
+
+
+size_t ngt_render_batch_plan_bytes(int num_instances) {
+  if (num_instances <= 0) return 0;
+  BlobLayout L;
+  L.add<RenBatchHeader>(1);
+  L.add<RenBatchItem>(num_instances);
+  for (int k = 0; k < 3; ++k) L.add<int>(num_instances + 1);
+  return L.bytes;
+}
+
+int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_mode, void* plan, size_t plan_bytes,
+                          size_t* workspace_bytes) {
+  NGT_REQUIRE(inst != nullptr && n > 0, NGT_ERR_INVALID_ARGUMENT, "batch must not be empty");
+  NGT_REQUIRE(blending_mode == BEER_LAMBERT || blending_mode == ALPHA || blending_mode == ALPHA_ADDITIVE ||
+                  blending_mode == ADDITIVE,
+              NGT_ERR_INVALID_ARGUMENT, "invalid blending mode");
+  NGT_REQUIRE(plan != nullptr && plan_bytes >= ngt_render_batch_plan_bytes(n), NGT_ERR_WORKSPACE,
+              "plan buffer too small for ngt_render_batch_plan");
+  BlobLayout L;
+  const size_t off_h = L.add<RenBatchHeader>(1);
+  const size_t off_items = L.add<RenBatchItem>(n);
+  const size_t off_ray = L.add<int>(n + 1), off_brickify = L.add<int>(n + 1), off_brick = L.add<int>(n + 1);
+  RenBatchHeader* h = blob_at<RenBatchHeader>(plan, off_h);
+  RenBatchItem* items = blob_at<RenBatchItem>(plan, off_items);
+  int* ray = blob_at<int>(plan, off_ray);
+  int* brickify = blob_at<int>(plan, off_brickify);
+  int* brick = blob_at<int>(plan, off_brick);
+  ray[0] = brickify[0] = brick[0] = 0;
+  int64_t tr = 0, tz = 0, tb = 0;
+  size_t ws = 0;
+  for (int i = 0; i < n; ++i) {
+    const ngt_render_instance& q = inst[i];
+    NGT_REQUIRE(q.density && q.matrix_m && q.matrix_v && q.matrix_p && q.frustum, NGT_ERR_INVALID_ARGUMENT,
+                "batch instance without density or camera parameters");
+    NGT_REQUIRE(q.num_views >= 1 && q.num_views <= kMaxCamsPerLaunch, NGT_ERR_INVALID_ARGUMENT,
+                "a batch instance has 1 to 8 views");
+    NGT_REQUIRE(q.image_dhw[0] > 0 && q.image_dhw[1] > 0 && q.image_dhw[2] > 0, NGT_ERR_INVALID_ARGUMENT,
+                "empty image");
+    const int64_t sh[5] = {1, q.dhw[0], q.dhw[1], q.dhw[2], 1};
+    int rc = check_grid_shape(sh, "batch density");
+    if (rc) return rc;
+    RenBatchItem& it = items[i];
+    it.a = make_ray_args(q.density, sh, q.image_dhw);
+    it.a.light_r = q.light_rgb[0]; it.a.light_g = q.light_rgb[1]; it.a.light_b = q.light_rgb[2];
+    it.a.depth_step = q.depth_step;
+    std::vector<Pair> pairs;
+    make_pairs(1, q.num_views, true, pairs);
+    it.cb.count = q.num_views;
+    for (int v = 0; v < q.num_views; ++v)
+      it.cb.cam[v] = make_cam(q.matrix_m, q.matrix_v + 16 * v, q.matrix_p + 16 * v, q.frustum + 6 * v, 0, v);
+    it.kcap = ray_step_cap(pairs, q.matrix_m, q.matrix_v, q.matrix_p, q.frustum, it.a);
+    it.ntx = static_cast<int>(blocks_for(it.a.out_w, 16));
+    it.nty = static_cast<int>(blocks_for(it.a.out_h, 16));
+    it.nbx64 = static_cast<int>(blocks_for(it.a.din.x, 64));
+    it.nby4 = static_cast<int>(blocks_for(it.a.din.y, 4));
+    it.images = q.images; it.images_grad = q.images_grad; it.density_grad = q.density_grad;
+    const size_t npix = static_cast<size_t>(it.a.out_w) * it.a.out_h;
+    it.ws_bricked = static_cast<int64_t>(ws);
+    ws += align_up(static_cast<size_t>(it.a.bd.slice_cells) * sizeof(float));
+    it.ws_rays = static_cast<int64_t>(ws);
+    ws += align_up(q.num_views * npix * sizeof(RayRec));
+    it.ws_G = static_cast<int64_t>(ws);
+    ws += align_up(q.num_views * static_cast<size_t>(it.kcap) * npix * sizeof(float));
+    tr += static_cast<int64_t>(q.num_views) * it.ntx * it.nty;
+    tz += static_cast<int64_t>(it.nbx64) * it.nby4 * it.a.din.z;
+    tb += static_cast<int64_t>(blocks_for(it.a.din.x, 8)) * blocks_for(it.a.din.y, 8) * blocks_for(it.a.din.z, 8);
+    NGT_REQUIRE(tr < (int64_t(1) << 31) && tz < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT, "batch too large");
+    ray[i + 1] = static_cast<int>(tr); brickify[i + 1] = static_cast<int>(tz); brick[i + 1] = static_cast<int>(tb);
+  }
+  h->n = n; h->mode = blending_mode;
+  h->tiles_ray = static_cast<int>(tr); h->tiles_brickify = static_cast<int>(tz); h->tiles_brick = static_cast<int>(tb);
+  h->off_items = off_items; h->off_ray = off_ray; h->off_brickify = off_brickify; h->off_brick = off_brick;
+  h->ws_bytes = static_cast<int64_t>(ws);
+  if (workspace_bytes) *workspace_bytes = ws;
+  return NGT_OK;
+}
+
+int ngt_render_batch_fwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                         int blocks_per_cu, ngt_stream_t stream) {
+  NGT_REQUIRE(plan_host && plan_device, NGT_ERR_INVALID_ARGUMENT, "plan must be given");
+  const RenBatchHeader& h = *static_cast<const RenBatchHeader*>(plan_host);
+  NGT_REQUIRE(workspace != nullptr && workspace_bytes >= static_cast<size_t>(h.ws_bytes), NGT_ERR_WORKSPACE,
+              "workspace too small for ngt_render_batch_fwd");
+  const RenBatchItem* items = blob_at<RenBatchItem>(plan_host, h.off_items);
+  for (int i = 0; i < h.n; ++i) NGT_REQUIRE(items[i].images, NGT_ERR_INVALID_ARGUMENT, "batch instance without images");
+  hipStream_t s = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
+  hipLaunchKernelGGL(render_brickify_batch_kernel, dim3(persistent_grid(h.tiles_brickify, blocks_per_cu)),
+                     dim3(64, 4), 0, s, plan_device, ws);
+  const dim3 g(persistent_grid(h.tiles_ray, blocks_per_cu));
+  NGT_SWITCH_MODE(h.mode, hipLaunchKernelGGL((render_fwd_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws));
+  return check_launch("ngt_render_batch_fwd");
+}
+
+int ngt_render_batch_bwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                         int blocks_per_cu, ngt_stream_t stream) {
+  NGT_REQUIRE(plan_host && plan_device, NGT_ERR_INVALID_ARGUMENT, "plan must be given");
+  const RenBatchHeader& h = *static_cast<const RenBatchHeader*>(plan_host);
+  NGT_REQUIRE(workspace != nullptr && workspace_bytes >= static_cast<size_t>(h.ws_bytes), NGT_ERR_WORKSPACE,
+              "workspace too small for ngt_render_batch_bwd");
+  const RenBatchItem* items = blob_at<RenBatchItem>(plan_host, h.off_items);
+  for (int i = 0; i < h.n; ++i)
+    NGT_REQUIRE(items[i].images && items[i].images_grad && items[i].density_grad, NGT_ERR_INVALID_ARGUMENT,
+                "batch instance without images, image gradients or density gradient");
+  hipStream_t s = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
+  hipLaunchKernelGGL(render_brickify_batch_kernel, dim3(persistent_grid(h.tiles_brickify, blocks_per_cu)),
+                     dim3(64, 4), 0, s, plan_device, ws);
+  const dim3 g(persistent_grid(h.tiles_ray, blocks_per_cu));
+  NGT_SWITCH_MODE(h.mode,
+      hipLaunchKernelGGL((render_prepass_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws));
+  hipLaunchKernelGGL(render_brick_batch_kernel, dim3(persistent_grid(h.tiles_brick, blocks_per_cu)), dim3(256), 0, s,
+                     plan_device, ws);
+  return check_launch("ngt_render_batch_bwd");
 }
 
 }  // extern "C"

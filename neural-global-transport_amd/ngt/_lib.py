@@ -57,6 +57,35 @@ _SIGNATURES = {
                             _vp, _sz, _vp]),
 }
 
+
+
+class AdvectInstance(ctypes.Structure):
+    """ngt_advect_instance (include/ngt_transport.h)."""
+    _fields_ = [("dhw", ctypes.c_int32 * 3), ("timestep", ctypes.c_float), ("input", _vp), ("vel_x", _vp),
+                ("vel_y", _vp), ("vel_z", _vp), ("output", _vp), ("sl", _vp), ("output_grad", _vp),
+                ("input_grad", _vp), ("vel_x_grad", _vp), ("vel_y_grad", _vp), ("vel_z_grad", _vp)]
+
+
+class RenderInstance(ctypes.Structure):
+    """ngt_render_instance (include/ngt_transport.h)."""
+    _fields_ = [("dhw", ctypes.c_int32 * 3), ("density", _vp), ("num_views", ctypes.c_int32), ("matrix_m", _c_fp),
+                ("matrix_v", _c_fp), ("matrix_p", _c_fp), ("frustum", _c_fp), ("image_dhw", ctypes.c_int32 * 3),
+                ("light_rgb", ctypes.c_float * 3), ("depth_step", ctypes.c_float), ("images", _vp),
+                ("images_grad", _vp), ("density_grad", _vp)]
+
+
+_szp = ctypes.POINTER(ctypes.c_size_t)
+_SIGNATURES.update({
+    "ngt_advect_batch_plan_bytes": (_sz, [_i]),
+    "ngt_advect_batch_plan": (_i, [ctypes.POINTER(AdvectInstance), _i, _i, _i, _i, _vp, _sz, _szp]),
+    "ngt_advect_batch_fwd": (_i, [_vp, _vp, _i, _vp]),
+    "ngt_advect_batch_bwd": (_i, [_vp, _vp, _vp, _sz, _i, _vp]),
+    "ngt_render_batch_plan_bytes": (_sz, [_i]),
+    "ngt_render_batch_plan": (_i, [ctypes.POINTER(RenderInstance), _i, _i, _vp, _sz, _szp]),
+    "ngt_render_batch_fwd": (_i, [_vp, _vp, _vp, _sz, _i, _vp]),
+    "ngt_render_batch_bwd": (_i, [_vp, _vp, _vp, _sz, _i, _vp]),
+})
+
 SYMBOLS = tuple(_SIGNATURES)
 
 
