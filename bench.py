@@ -48,6 +48,10 @@ def parse():
     p.add_argument("--chain", action="store_true",
                    help="instead of the single step: the 30-step checkpointed chain at 96x160x96 (render every 5th "
                         "step, 5 views 240x135x96); prints its own JSON line (time, peak device memory)")
+    p.add_argument("--batch", action="store_true",
+                   help="instead of the single step: the uneven batch (32 sequences x 4 frames, grid edges "
+                        "48-160, 1-8 views, 3 image sizes) as batched launches (eager and HIP-graph replay) against "
+                        "per-instance launches; prints its own JSON line")
     p.add_argument("--traffic-probe", action="store_true",
                    help="run a 256 MiB copy and each op of the step once (for rocprofv3 --pmc FETCH_SIZE / "
                         "WRITE_SIZE runs) and exit")
@@ -371,8 +375,101 @@ def chain_bench(steps, warmup):
             "within_budget": res["checkpoint"]["peak_device_bytes"] < (8 << 30)}
 
 
+def batch_bench(steps, warmup):
+    """32 sequences (seed 5), each with a cubic grid edge drawn from {48, 64,
+    96, 128, 160}, 4 frames (densities), 1-8 ring views with an image size
+    drawn from {128x72, 256x144, 480x270} and edge-many depth slices.  Every
+    frame runs one MacCormack step (MC_SMOOTH, BORDER, MAC velocity) and the
+    fused render with L2 loss, forward and backward.  Batched: one persistent
+    launch per kernel over all 128 instances (eager and captured in a HIP
+    graph); per-instance: the same ops launched instance by instance."""
+    import torch
+    from ngt import synth, batch as B, ops, _lib
+    from ngt.renderer import Renderer
+    _lib.load()
+    rng = np.random.default_rng(5)
+    edges = (48, 64, 96, 128, 160)
+    images = ((72, 128), (144, 256), (270, 480))
+    dens, vels, cams, tgts = [], [], [], []
+    cells = ray_samples = 0
+    for q in range(32):
+        e = int(rng.choice(edges))
+        nv = int(rng.integers(1, 9))
+        hw = images[int(rng.integers(0, 3))]
+        grid_t, cs = synth.ring_setup((e, e, e), hw, e, nv, )
+        M, V, P, F = Renderer()._transform_params([grid_t], cs)
+        vel = synth.curl_noise_staggered_torch((e, e, e), seed=1000 + q, vmax=2.0)
+        for f in range(4):
+            dens.append(synth.blobs_torch((e, e, e), 16, seed=100 * q + f))
+            vels.append(vel)
+            cams.append((M, V, P, F, cs[0].depth_step, (e,) + hw))
+            tgts.append(torch.rand((nv,) + hw + (3,), device="cuda"))
+            cells += e ** 3
+            ray_samples += nv * hw[0] * hw[1] * e
+    torch.cuda.synchronize()
+    print("batch: %d instances generated" % len(dens), file=sys.stderr, flush=True)
+    step = B.TransportRenderBatch(dens, vels, cams, tgts)
+    print("batch: planned, workspace %.2f GB" % ((step.adv.workspace.numel() + step.ren.workspace.numel()) / 1e9),
+          file=sys.stderr, flush=True)
+    for _ in range(max(1, warmup)):
+        step.step()   # the first call autotunes the persistent grids
+    torch.cuda.synchronize()
+
+    def timed(fn, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n
+
+    print("batch: autotuned", file=sys.stderr, flush=True)
+    ms_eager = timed(step.step, steps)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step.step()
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step.step()
+    graph.replay()
+    torch.cuda.synchronize()
+    ms_graph = timed(graph.replay, steps)
+
+    def per_instance():
+        for i in range(len(dens)):
+            M, V, P, F, ds, out_dhw = cams[i]
+            d = dens[i][None].requires_grad_()
+            v = tuple(c[None].requires_grad_() for c in vels[i])
+            adv = ops.advect_grid_semi_langrange(d, v, -1.0, 2, "MC_SMOOTH", "BORDER", False)[:, 0]
+            img = ops.render_density_fused(adv, (1.0, 1.0, 1.0), ds, M, V, P, F, out_dhw)[0]
+            ((img[..., :3] - tgts[i]) ** 2).sum().backward()
+
+    print("batch: batched %.2f ms eager, %.2f ms graph" % (ms_eager, ms_graph), file=sys.stderr, flush=True)
+    per_instance()
+    torch.cuda.synchronize()
+    ms_plain = timed(per_instance, max(1, steps // 4))
+    best = min(ms_eager, ms_graph)
+    tuner = B.DEFAULT_TUNER
+    return {"workload": "uneven batch: 32 sequences x 4 frames (grid edge 48-160, 1-8 views, 128x72 / 256x144 / "
+                        "480x270 images, edge depth slices), MacCormack + fused render + L2, fwd+bwd",
+            "instances": len(dens), "voxel_updates_per_step": cells, "ray_samples_per_step": ray_samples,
+            "ms_per_step_batched_eager": ms_eager, "ms_per_step_batched_graph": ms_graph,
+            "ms_per_step_per_instance": ms_plain, "speedup_vs_per_instance": ms_plain / best,
+            "voxel_updates_per_s": cells / (best * 1e-3), "ray_samples_per_s": ray_samples / (best * 1e-3),
+            "workspace_bytes": int(step.adv.workspace.numel() + step.ren.workspace.numel()),
+            "autotuned_blocks_per_cu": {"/".join(str(x) for x in (k[0], k[-1])): v for k, v in tuner.best.items()},
+            "autotune_ms": {"/".join(str(x) for x in (k[0], k[-1])): {str(c): round(t, 3) for c, t in v.items()}
+                            for k, v in tuner.timings.items()}}
+
+
 def main():
     args = parse()
+    if args.batch:
+        print(json.dumps(batch_bench(max(1, args.steps), max(1, args.warmup))), flush=True)
+        return
     if args.chain:
         print(json.dumps(chain_bench(max(1, args.steps // 5), min(args.warmup, 1))), flush=True)
         return

@@ -80,7 +80,7 @@ NGT_DEVICE ThreadCell thread_cell(const AdvArgs& a, const VBlock& vb) {
 
 // A: semi-Lagrangian step (also the whole order-1 warp).
 template <int C, int BM, bool NEG>
-NGT_DEVICE void advect_sl_kernel_body(const VBlock& vb, AdvArgs a, const float* __restrict__ rho,
+NGT_DEVICE void advect_sl_kernel_body(const VBlock& vb, const AdvArgs& a, const float* __restrict__ rho,
                                                         float* __restrict__ out) {
   const ThreadCell t = thread_cell(a, vb);
   if (!t.ok) return;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void advect_sl_kernel(AdvArgs a, const float* 
 
 // B: MacCormack correction + clamp.
 template <int C, int BM, int CLAMPM>
-NGT_DEVICE void advect_mc_kernel_body(const VBlock& vb, AdvArgs a, const float* __restrict__ rho,
+NGT_DEVICE void advect_mc_kernel_body(const VBlock& vb, const AdvArgs& a, const float* __restrict__ rho,
                                                         const float* __restrict__ sl, float* __restrict__ out) {
   const ThreadCell t = thread_cell(a, vb);
   if (!t.ok) return;
@@ -464,7 +464,7 @@ NGT_DEVICE void add_vel_grad(float* gv, float gx, float gy, float gz, int mode) 
 // corrected value is kept (direct path to rho), the back-step adjoint into
 // the tile (overflow into ovf), and the back-step velocity gradient.
 template <int C, int BM, int CLAMPM>
-NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs& a, BrickGrid bg, const float* __restrict__ rho,
                                                                   const float* __restrict__ sl,
                                                                   const float* __restrict__ g_out,
                                                                   float* __restrict__ g_dir, float* __restrict__ r_dir,
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_brick_kernel(AdvArgs a, Bri
 // NEGATIVE mask (order 1); scatter into the tile (overflow straight into
 // g_rho, which starts at zero), SL velocity gradient.
 template <int C, int BM, bool NEG, bool ORDER2>
-NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs& a, BrickGrid bg, const float* __restrict__ rho,
                                                                   const float* __restrict__ g_in,
                                                                   const float* __restrict__ ovf1,
                                                                   const float* __restrict__ tiles1,
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(256) void advect_sl_bwd_brick_kernel(AdvArgs a, Bri
 
 // g_rho(n) += sum over the pairs of n of r_dir (order 2) + the B2 tiles.
 template <int C>
-NGT_DEVICE void advect_rho_gather_kernel_body(const VBlock& vb, AdvArgs a, BrickGrid bg, const float* __restrict__ r_dir,
+NGT_DEVICE void advect_rho_gather_kernel_body(const VBlock& vb, const AdvArgs& a, BrickGrid bg, const float* __restrict__ r_dir,
                                                                 const float* __restrict__ tiles2,
                                                                 float* __restrict__ g_rho) {
   const int x = vb.x * kBX + threadIdx.x;
@@ -747,12 +747,13 @@ struct AdvBatchView {
 
 NGT_DEVICE AdvBatchView adv_batch_view(const void* plan) {
   AdvBatchView v;
-  v.h = static_cast<const AdvBatchHeader*>(plan);
-  v.items = blob_at<AdvBatchItem>(plan, v.h->off_items);
-  v.cell = blob_at<int>(plan, v.h->off_cell);
-  v.brick = blob_at<int>(plan, v.h->off_brick);
-  v.face = blob_at<int>(plan, v.h->off_face);
-  v.fill = blob_at<int>(plan, v.h->off_fill);
+  const char* p = as_constant<char>(plan);
+  v.h = reinterpret_cast<const AdvBatchHeader*>(p);
+  v.items = reinterpret_cast<const AdvBatchItem*>(p + v.h->off_items);
+  v.cell = reinterpret_cast<const int*>(p + v.h->off_cell);
+  v.brick = reinterpret_cast<const int*>(p + v.h->off_brick);
+  v.face = reinterpret_cast<const int*>(p + v.h->off_face);
+  v.fill = reinterpret_cast<const int*>(p + v.h->off_fill);
   return v;
 }
 
@@ -761,31 +762,31 @@ NGT_DEVICE VBlock cell_tile(const AdvBatchItem& it, int local) {
 }
 
 template <int BM, bool NEG, bool TO_SL>
-__global__ __launch_bounds__(256) void advect_sl_batch_kernel(const void* plan) {
+__global__ __launch_bounds__(256) void advect_sl_batch_kernel(const void* __restrict__ plan) {
   const AdvBatchView v = adv_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
-    const int i = find_item(v.cell, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.cell, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
     advect_sl_kernel_body<1, BM, NEG>(cell_tile(it, t - v.cell[i]), it.a, it.input, TO_SL ? it.sl : it.output);
   }
 }
 
 template <int BM, int CLAMPM>
-__global__ __launch_bounds__(256) void advect_mc_batch_kernel(const void* plan) {
+__global__ __launch_bounds__(256) void advect_mc_batch_kernel(const void* __restrict__ plan) {
   const AdvBatchView v = adv_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
-    const int i = find_item(v.cell, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.cell, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
     advect_mc_kernel_body<1, BM, CLAMPM>(cell_tile(it, t - v.cell[i]), it.a, it.input, it.sl, it.output);
   }
 }
 
 // zero input_grad (range 0) and ovf1 (range 1, order 2) of every instance
-__global__ __launch_bounds__(256) void advect_fill_batch_kernel(const void* plan, float* __restrict__ ws) {
+__global__ __launch_bounds__(256) void advect_fill_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
   const int ranges = v.h->order == 2 ? 2 : 1;
   for (int t = blockIdx.x; t < v.h->tiles_fill; t += gridDim.x) {
-    const int i = find_item(v.fill, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.fill, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
     const int local = t - v.fill[i];
     const int chunks = static_cast<int>((it.cells + kFillChunk - 1) / kFillChunk);
@@ -798,10 +799,10 @@ __global__ __launch_bounds__(256) void advect_fill_batch_kernel(const void* plan
 }
 
 template <int BM, int CLAMPM>
-__global__ __launch_bounds__(256) void advect_mc_bwd_batch_kernel(const void* plan, float* __restrict__ ws) {
+__global__ __launch_bounds__(256) void advect_mc_bwd_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
-    const int i = find_item(v.brick, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.brick, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
     float* g_vc = v.h->staggered ? ws + it.ws_gvc : it.gvx;
     advect_mc_bwd_brick_kernel_body<1, BM, CLAMPM>(VBlock{t - v.brick[i], 0, 0}, it.a, it.bg, it.input, it.sl,
@@ -812,10 +813,10 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_batch_kernel(const void* pl
 }
 
 template <int BM, bool NEG, bool ORDER2>
-__global__ __launch_bounds__(256) void advect_sl_bwd_batch_kernel(const void* plan, float* __restrict__ ws) {
+__global__ __launch_bounds__(256) void advect_sl_bwd_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
-    const int i = find_item(v.brick, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.brick, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
     float* g_vc = v.h->staggered ? ws + it.ws_gvc : it.gvx;
     advect_sl_bwd_brick_kernel_body<1, BM, NEG, ORDER2>(VBlock{t - v.brick[i], 0, 0}, it.a, it.bg, it.input,
@@ -825,20 +826,20 @@ __global__ __launch_bounds__(256) void advect_sl_bwd_batch_kernel(const void* pl
   }
 }
 
-__global__ __launch_bounds__(256) void advect_rho_gather_batch_kernel(const void* plan, float* __restrict__ ws) {
+__global__ __launch_bounds__(256) void advect_rho_gather_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
-    const int i = find_item(v.cell, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.cell, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
     advect_rho_gather_kernel_body<1>(cell_tile(it, t - v.cell[i]), it.a, it.bg,
                                      v.h->order == 2 ? ws + it.ws_rdir : nullptr, ws + it.ws_t2, it.g_in);
   }
 }
 
-__global__ __launch_bounds__(256) void advect_faces_batch_kernel(const void* plan, float* __restrict__ ws) {
+__global__ __launch_bounds__(256) void advect_faces_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_face; t += gridDim.x) {
-    const int i = find_item(v.face, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.face, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
     faces_from_centered_grad_kernel_body(VBlock{t - v.face[i], 0, 0}, it.a.d, 1, ws + it.ws_gvc, it.gvx, it.gvy,
                                          it.gvz);

@@ -103,6 +103,15 @@ struct BlobLayout {
   }
 };
 
+// Read-only plan data seen through the constant address space, so uniform
+// reads of it become scalar loads (as kernel arguments are) and the kernel
+// bodies see it like a by-value argument.  Valid because a plan is never
+// written while a kernel runs.
+template <typename T>
+NGT_DEVICE const T* as_constant(const void* p) {
+  return (const T*)((const __attribute__((address_space(4))) char*)p);
+}
+
 template <typename T>
 NGT_HOST_DEVICE_INLINE T* blob_at(void* base, size_t off) {
   return reinterpret_cast<T*>(static_cast<char*>(base) + off);

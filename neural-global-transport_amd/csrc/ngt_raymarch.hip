@@ -185,7 +185,7 @@ NGT_DEVICE void scatter_cell(const RayArgs& a, const float* __restrict__ src, fl
 }
 
 template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
-NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, RayArgs a, CamBatch cb, float* __restrict__ out) {
+NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, float* __restrict__ out) {
   int x, y;
   pixel_of_thread(vb, x, y);
   if (x >= a.out_w || y >= a.out_h) return;
@@ -354,7 +354,7 @@ struct RayRec {
 };
 
 template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
-NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, RayArgs a, CamBatch cb, int pair_base,
+NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
                                                                    const float* __restrict__ out,
                                                                    const float* __restrict__ out_grad,
                                                                    float* __restrict__ G, RayRec* __restrict__ rays,
@@ -427,7 +427,7 @@ NGT_DEVICE int block_inclusive_scan(int v, int* s_wtot) {
 // stores; a sample on a brick boundary is visited by each brick it touches
 // and contributes only that brick's corners.
 template <int CG, int BX, int BY, int BZ>
-NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, RayArgs a, CamBatch cb, int pair_base,
+NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
                                                                  const RayRec* __restrict__ rays,
                                                                  const float* __restrict__ G, int kcap,
                                                                  float* __restrict__ in_grad, int accumulate) {
@@ -1016,11 +1016,12 @@ struct RenBatchView {
 
 NGT_DEVICE RenBatchView ren_batch_view(const void* plan) {
   RenBatchView v;
-  v.h = static_cast<const RenBatchHeader*>(plan);
-  v.items = blob_at<RenBatchItem>(plan, v.h->off_items);
-  v.ray = blob_at<int>(plan, v.h->off_ray);
-  v.brickify = blob_at<int>(plan, v.h->off_brickify);
-  v.brick = blob_at<int>(plan, v.h->off_brick);
+  const char* p = as_constant<char>(plan);
+  v.h = reinterpret_cast<const RenBatchHeader*>(p);
+  v.items = reinterpret_cast<const RenBatchItem*>(p + v.h->off_items);
+  v.ray = reinterpret_cast<const int*>(p + v.h->off_ray);
+  v.brickify = reinterpret_cast<const int*>(p + v.h->off_brickify);
+  v.brick = reinterpret_cast<const int*>(p + v.h->off_brick);
   return v;
 }
 
@@ -1030,10 +1031,10 @@ NGT_DEVICE VBlock ray_tile(const RenBatchItem& it, int local) {
   return VBlock{r % it.ntx, r / it.ntx, view};
 }
 
-__global__ __launch_bounds__(256) void render_brickify_batch_kernel(const void* plan, char* __restrict__ ws) {
+__global__ __launch_bounds__(256) void render_brickify_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws) {
   const RenBatchView v = ren_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_brickify; t += gridDim.x) {
-    const int i = find_item(v.brickify, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.brickify, v.h->n, t));
     const RenBatchItem& it = v.items[i];
     const int local = t - v.brickify[i];
     const VBlock vb{local % it.nbx64, (local / it.nbx64) % it.nby4, local / (it.nbx64 * it.nby4)};
@@ -1042,10 +1043,10 @@ __global__ __launch_bounds__(256) void render_brickify_batch_kernel(const void* 
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void render_fwd_batch_kernel(const void* plan, const char* __restrict__ ws) {
+__global__ __launch_bounds__(256) void render_fwd_batch_kernel(const void* __restrict__ plan, const char* __restrict__ ws) {
   const RenBatchView v = ren_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_ray; t += gridDim.x) {
-    const int i = find_item(v.ray, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.ray, v.h->n, t));
     const RenBatchItem& it = v.items[i];
     RayArgs a = it.a;
     a.bricked = reinterpret_cast<const float*>(ws + it.ws_bricked);
@@ -1054,10 +1055,10 @@ __global__ __launch_bounds__(256) void render_fwd_batch_kernel(const void* plan,
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void render_prepass_batch_kernel(const void* plan, char* __restrict__ ws) {
+__global__ __launch_bounds__(256) void render_prepass_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws) {
   const RenBatchView v = ren_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_ray; t += gridDim.x) {
-    const int i = find_item(v.ray, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.ray, v.h->n, t));
     const RenBatchItem& it = v.items[i];
     RayArgs a = it.a;
     a.bricked = reinterpret_cast<const float*>(ws + it.ws_bricked);
@@ -1067,12 +1068,13 @@ __global__ __launch_bounds__(256) void render_prepass_batch_kernel(const void* p
   }
 }
 
-__global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* plan, char* __restrict__ ws) {
+__global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws) {
   const RenBatchView v = ren_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
-    const int i = find_item(v.brick, v.h->n, t);
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.brick, v.h->n, t));
     const RenBatchItem& it = v.items[i];
-    raymarch_bwd_brick_kernel_body<1, 8, 8, 8>(VBlock{t - v.brick[i], 0, 0}, it.a, it.cb, 0,
+    const RayArgs a = it.a;
+    raymarch_bwd_brick_kernel_body<1, 8, 8, 8>(VBlock{t - v.brick[i], 0, 0}, a, it.cb, 0,
                                                reinterpret_cast<const RayRec*>(ws + it.ws_rays),
                                                reinterpret_cast<const float*>(ws + it.ws_G), it.kcap,
                                                it.density_grad, 0);

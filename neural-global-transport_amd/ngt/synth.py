@@ -82,3 +82,61 @@ def ring_setup(grid_dhw, image_hw=(144, 256), depth=128, views=5, fov=40.0, dist
     far = distance + np.sqrt(3.0) / 2.0
     cams = ring_cameras(views, distance, fov, image_hw, depth, near, far)
     return grid, cams
+
+
+# --- device-side generators (large batches) ----------------------------------------
+
+def blobs_torch(shape_dhw, count, seed, device="cuda", sigma_range=(4.0, 10.0), amp_range=(0.2, 1.0)):
+    """``blobs`` evaluated on the device (same parameter distribution, float32
+    arithmetic), [D,H,W,1]."""
+    import torch
+    rng = np.random.default_rng(seed)
+    D, H, W = shape_dhw
+    dims = np.array([W, H, D], dtype=np.float64)
+    z = torch.arange(D, device=device, dtype=torch.float32)[:, None, None]
+    y = torch.arange(H, device=device, dtype=torch.float32)[None, :, None]
+    x = torch.arange(W, device=device, dtype=torch.float32)[None, None, :]
+    out = torch.zeros((D, H, W), device=device)
+    for _ in range(count):
+        c = (0.2 + 0.6 * rng.uniform(0, 1, 3)) * dims
+        s = float(rng.uniform(*sigma_range))
+        a = float(rng.uniform(*amp_range))
+        k = -1.0 / (2 * s * s)
+        out += a * torch.exp(k * (z - c[2]) ** 2) * torch.exp(k * (y - c[1]) ** 2) * torch.exp(k * (x - c[0]) ** 2)
+    return out.clamp_(0.0, 1.0)[..., None]
+
+
+def curl_noise_staggered_torch(shape_dhw, seed, vmax=2.0, wavelength=24.0, modes=6, device="cuda"):
+    """MAC face velocities (x [D,H,W+1,1], y [D,H+1,W,1], z [D+1,H,W,1]) as the
+    discrete curl of edge potentials that are sums of random plane waves of
+    the given wavelength (cells): exactly divergence-free per cell, scaled to
+    max |v_centered| = vmax."""
+    import torch
+    rng = np.random.default_rng(seed)
+    D, H, W = shape_dhw
+
+    def potential(shape):
+        dz, dy, dx = shape
+        z = torch.arange(dz, device=device, dtype=torch.float32)[:, None, None]
+        y = torch.arange(dy, device=device, dtype=torch.float32)[None, :, None]
+        x = torch.arange(dx, device=device, dtype=torch.float32)[None, None, :]
+        out = torch.zeros(shape, device=device)
+        for _ in range(modes):
+            k = rng.normal(size=3)
+            k *= 2 * np.pi / wavelength / np.linalg.norm(k)
+            ph = float(rng.uniform(0, 2 * np.pi))
+            out += float(rng.normal()) * torch.sin(float(k[0]) * x + float(k[1]) * y + float(k[2]) * z + ph)
+        return out
+
+    Ax = potential((D + 1, H + 1, W))
+    Ay = potential((D + 1, H, W + 1))
+    Az = potential((D, H + 1, W + 1))
+    vx = (Az[:, 1:, :] - Az[:, :-1, :]) - (Ay[1:, :, :] - Ay[:-1, :, :])
+    vy = (Ax[1:, :, :] - Ax[:-1, :, :]) - (Az[:, :, 1:] - Az[:, :, :-1])
+    vz = (Ay[:, :, 1:] - Ay[:, :, :-1]) - (Ax[:, 1:, :] - Ax[:, :-1, :])
+    cx = 0.5 * (vx[:, :, 1:] + vx[:, :, :-1])
+    cy = 0.5 * (vy[:, 1:] + vy[:, :-1])
+    cz = 0.5 * (vz[1:] + vz[:-1])
+    s = vmax / float(torch.sqrt(cx * cx + cy * cy + cz * cz).max())
+    return tuple((c * s).contiguous()[..., None] for c in (vx, vy, vz))
+
