@@ -81,21 +81,38 @@ typedef void* ngt_stream_t; /* hipStream_t */
 const char* ngt_last_error(void);
 int ngt_version(void);
 
+/* ---- Mip pyramids (resample_grid.cu.cc:58-251, sampling.hpp:939-1010) -----
+ * With mipmapping NEAREST or LINEAR the sampling ops read a pyramid of
+ * num_mipmaps levels built from the input: level m+1 has floor(dims_m / 2)
+ * cells, each the mean of its 2x2x2 children.  A sample at position p (before
+ * the cell-centre offset) with level of detail lod reads level m at
+ * p / 2^m - cell_center_offset on a grid of dims >> m; lod = clamp(raw lod +
+ * mip_bias, 0, num_mipmaps); NEAREST reads level min(int(lod + .5), L),
+ * LINEAR lerps levels int(lod) and ceil(lod) by fract(lod).  The pyramid
+ * lives in the workspace; every level must keep at least one cell per axis
+ * (dims >> num_mipmaps >= 1), at most 16 levels. */
+
 /* ---- SampleGridLut (resample_grid.cc:94-120, 742-899) ----------------------
  * input [N,D,H,W,C] (C in 1,2,4), lookup [V,D',H',W',4] -> output
  * [N,V|1,D',H',W',C].  Position = lookup.xyz (x output dims if
- * normalized_coords) (+ output index if relative_coords) - cell_center_offset.
- * Only mipmapping NONE is implemented (NGT_ERR_UNIMPLEMENTED otherwise). */
+ * normalized_coords) (+ output index if relative_coords); raw lod =
+ * lookup.w; read at position - cell_center_offset (per mip level as above).
+ * The workspace holds the mip pyramid (0 bytes without mipmapping). */
+size_t ngt_sample_grid_lut_fwd_workspace_size(const int64_t* input_shape, int mipmapping, int num_mipmaps);
 int ngt_sample_grid_lut_fwd(const float* input, const int64_t* input_shape, const float* lookup,
                             const int64_t* lookup_shape, int interpolation, int boundary, int mipmapping,
                             int num_mipmaps, float mip_bias, float cell_center_offset, int separate_camera_batch,
-                            int relative_coords, int normalized_coords, float* output, ngt_stream_t stream);
+                            int relative_coords, int normalized_coords, float* output, void* workspace,
+                            size_t workspace_bytes, ngt_stream_t stream);
 
 /* ---- SampleGridLutGrad (resample_grid.cc:124-150, 925-1081) ----------------
  * output_grad [N,V|1,D',H',W',C] -> input_grad [N,D,H,W,C],
- * lookup_grad [V,D',H',W',4] (both fully overwritten). */
+ * lookup_grad [V,D',H',W',4] (both fully overwritten).  With mipmapping
+ * (NGT_GRAD_EXACT only: the reference gradient op accepts mipmapping NONE
+ * only, resample_grid.cc:135) the input gradient runs through the pyramid's
+ * adjoint and lookup_grad.w holds d/d(raw lod). */
 size_t ngt_sample_grid_lut_bwd_workspace_size(const int64_t* input_shape, const int64_t* lookup_shape,
-                                              int gradient_semantics);
+                                              int mipmapping, int num_mipmaps, int gradient_semantics);
 int ngt_sample_grid_lut_bwd(const float* input, const int64_t* input_shape, const float* output_grad,
                             const float* lookup, const int64_t* lookup_shape, int interpolation, int boundary,
                             int mipmapping, int num_mipmaps, float mip_bias, float cell_center_offset,
@@ -105,22 +122,42 @@ int ngt_sample_grid_lut_bwd(const float* input, const int64_t* input_shape, cons
 
 /* ---- SampleGridTransform (resample_grid.cc:23-65, 316-516) -----------------
  * input [N,D,H,W,C] -> output [N,V|1,out_dhw,C].  interpolation NEAREST or
- * LINEAR, boundary BORDER or CLAMP, cell_center_offset must be 0.5. */
+ * LINEAR, boundary BORDER or CLAMP, cell_center_offset must be 0.5.  With
+ * mipmapping the raw lod of an output cell is log2 of the largest sample
+ * displacement over one output cell (samplePos3D<CM, true>,
+ * sampling.hpp:151-228). */
+size_t ngt_sample_grid_transform_fwd_workspace_size(const int64_t* input_shape, int mipmapping, int num_mipmaps);
 int ngt_sample_grid_transform_fwd(const float* input, const int64_t* input_shape, const float* matrix_m,
                                   const float* matrix_v, const float* matrix_p, const float* frustum,
                                   int num_cameras, const int32_t* output_dhw, int interpolation, int boundary,
                                   int mipmapping, int num_mipmaps, float mip_bias, int coordinate_mode,
                                   float cell_center_offset, int separate_camera_batch, float* output,
-                                  ngt_stream_t stream);
+                                  void* workspace, size_t workspace_bytes, ngt_stream_t stream);
 
-/* ---- SampleGridTransformGrad (resample_grid.cc:68-92, 538-717) ------------ */
-size_t ngt_sample_grid_transform_bwd_workspace_size(const int64_t* input_shape, int gradient_semantics);
+/* ---- SampleGridTransformGrad (resample_grid.cc:68-92, 538-717) ------------
+ * Mipmapping with NGT_GRAD_EXACT only (the reference op accepts NONE only,
+ * resample_grid.cc:79). */
+size_t ngt_sample_grid_transform_bwd_workspace_size(const int64_t* input_shape, int mipmapping, int num_mipmaps,
+                                                    int gradient_semantics);
 int ngt_sample_grid_transform_bwd(const float* input, const int64_t* input_shape, const float* output_grad,
                                   const float* matrix_m, const float* matrix_v, const float* matrix_p,
                                   const float* frustum, int num_cameras, const int32_t* output_dhw,
-                                  int interpolation, int boundary, int coordinate_mode, float cell_center_offset,
-                                  int separate_camera_batch, int gradient_semantics, float* input_grad,
-                                  void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+                                  int interpolation, int boundary, int mipmapping, int num_mipmaps, float mip_bias,
+                                  int coordinate_mode, float cell_center_offset, int separate_camera_batch,
+                                  int gradient_semantics, float* input_grad, void* workspace, size_t workspace_bytes,
+                                  ngt_stream_t stream);
+
+/* ---- LodTransform (resample_grid.cc:152-172, 1089-1178) --------------------
+ * Lookup tables for SampleGridLut from camera transforms: output
+ * [V,D',H',W',4] = (sample position of each output cell centre, minus the
+ * integer output index if relative_coords; level of detail from central
+ * differences, calcLoD<CM> sampling.hpp:72-139).  matrix_mv / matrix_p /
+ * frustum hold V cameras (host pointers); input_dhw is the grid sampled
+ * FROM, output_dhw the grid sampled TO; inverse_transform and
+ * linearize_depth select the coordinate mode. */
+int ngt_lod_transform(const int32_t* input_dhw, const float* matrix_mv, const float* matrix_p, const float* frustum,
+                      int num_cameras, const int32_t* output_dhw, int inverse_transform, int linearize_depth,
+                      int relative_coords, float* output, ngt_stream_t stream);
 
 /* ---- ReduceGridBlend (+Grad) (reduce_blend.cc:24-55) -----------------------
  * input [N,D,H,W,C] -> output [N,H,W,C] or, keep_dims, [N,D,H,W,C]. */

@@ -177,6 +177,6 @@ extern "C" {
 
 const char* ngt_last_error(void) { return ngt::g_last_error.c_str(); }
 
-int ngt_version(void) { return 2; }
+int ngt_version(void) { return 3; }
 
 }  // extern "C"

@@ -27,15 +27,19 @@ _sz = ctypes.c_size_t
 _SIGNATURES = {
     "ngt_last_error": (ctypes.c_char_p, []),
     "ngt_version": (_i, []),
-    "ngt_sample_grid_lut_fwd": (_i, [_vp, _c_i64p, _vp, _c_i64p, _i, _i, _i, _i, _f, _f, _i, _i, _i, _vp, _vp]),
-    "ngt_sample_grid_lut_bwd_workspace_size": (_sz, [_c_i64p, _c_i64p, _i]),
+    "ngt_sample_grid_lut_fwd_workspace_size": (_sz, [_c_i64p, _i, _i]),
+    "ngt_sample_grid_lut_fwd": (_i, [_vp, _c_i64p, _vp, _c_i64p, _i, _i, _i, _i, _f, _f, _i, _i, _i, _vp, _vp, _sz,
+                                     _vp]),
+    "ngt_sample_grid_lut_bwd_workspace_size": (_sz, [_c_i64p, _c_i64p, _i, _i, _i]),
     "ngt_sample_grid_lut_bwd": (_i, [_vp, _c_i64p, _vp, _vp, _c_i64p, _i, _i, _i, _i, _f, _f, _i, _i, _i, _i, _vp, _vp,
                                      _vp, _sz, _vp]),
+    "ngt_sample_grid_transform_fwd_workspace_size": (_sz, [_c_i64p, _i, _i]),
     "ngt_sample_grid_transform_fwd": (_i, [_vp, _c_i64p, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i, _i, _i, _f,
-                                           _i, _f, _i, _vp, _vp]),
-    "ngt_sample_grid_transform_bwd_workspace_size": (_sz, [_c_i64p, _i]),
-    "ngt_sample_grid_transform_bwd": (_i, [_vp, _c_i64p, _vp, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i, _i, _f,
-                                           _i, _i, _vp, _vp, _sz, _vp]),
+                                           _i, _f, _i, _vp, _vp, _sz, _vp]),
+    "ngt_sample_grid_transform_bwd_workspace_size": (_sz, [_c_i64p, _i, _i, _i]),
+    "ngt_sample_grid_transform_bwd": (_i, [_vp, _c_i64p, _vp, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i, _i, _i,
+                                           _f, _i, _f, _i, _i, _vp, _vp, _sz, _vp]),
+    "ngt_lod_transform": (_i, [_c_i32p, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i, _i, _vp, _vp]),
     "ngt_reduce_grid_blend_fwd": (_i, [_vp, _c_i64p, _i, _i, _vp, _vp]),
     "ngt_reduce_grid_blend_bwd": (_i, [_vp, _vp, _vp, _c_i64p, _i, _i, _i, _vp, _vp]),
     "ngt_raymarch_grid_transform_fwd_workspace_size": (_sz, [_c_i64p, _i]),

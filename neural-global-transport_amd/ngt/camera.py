@@ -26,6 +26,9 @@ class Camera(object):
             self.set_fov(fov, aspect)
         self.static = static
         self.scissor_pad = None
+        # lookup tables of a static camera (grid -> frustum and back), built on demand by the Renderer
+        self.LuT = None
+        self.inverseLuT = None
 
     def view_transform_inverse(self, with_jitter=True):
         if self.jitter is not None and with_jitter:

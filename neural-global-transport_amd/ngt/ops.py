@@ -99,10 +99,14 @@ def _lut_fwd(inp, lut, a):
         raise ValueError("Invalid lut shape (NDHWC) with C=4: %s" % (list(lut.shape),))
     V = lut.shape[0] if a["separate_camera_batch"] else 1
     out = torch.empty((N, V) + tuple(lut.shape[1:4]) + (inp.shape[-1],), dtype=torch.float32, device=inp.device)
+    s_in = _lib.shape5(inp)
+    ws, nbytes = _lib.workspace(lib.ngt_sample_grid_lut_fwd_workspace_size(s_in, a["mipmapping"], a["num_mipmaps"]),
+                                inp.device)
     _lib.check(lib.ngt_sample_grid_lut_fwd(
-        _lib.ptr(inp), _lib.shape5(inp), _lib.ptr(lut), _lib.shape5(lut), a["interpolation"], a["boundary"],
+        _lib.ptr(inp), s_in, _lib.ptr(lut), _lib.shape5(lut), a["interpolation"], a["boundary"],
         a["mipmapping"], a["num_mipmaps"], a["mip_bias"], a["cell_center_offset"], int(a["separate_camera_batch"]),
-        int(a["relative_coords"]), int(a["normalized_coords"]), _lib.ptr(out), _lib.stream(inp.device)))
+        int(a["relative_coords"]), int(a["normalized_coords"]), _lib.ptr(out), _lib.ptr(ws), nbytes,
+        _lib.stream(inp.device)))
     return out
 
 
@@ -116,7 +120,8 @@ def _lut_bwd(inp, out_grad, lut, a, sem):
     in_grad = torch.empty_like(inp)
     lut_grad = torch.empty_like(lut)
     s_in, s_lut = _lib.shape5(inp), _lib.shape5(lut)
-    ws, nbytes = _lib.workspace(lib.ngt_sample_grid_lut_bwd_workspace_size(s_in, s_lut, sem), inp.device)
+    ws, nbytes = _lib.workspace(lib.ngt_sample_grid_lut_bwd_workspace_size(s_in, s_lut, a["mipmapping"],
+                                                                           a["num_mipmaps"], sem), inp.device)
     _lib.check(lib.ngt_sample_grid_lut_bwd(
         _lib.ptr(inp), s_in, _lib.ptr(out_grad), _lib.ptr(lut), s_lut, a["interpolation"], a["boundary"],
         a["mipmapping"], a["num_mipmaps"], a["mip_bias"], a["cell_center_offset"], int(a["separate_camera_batch"]),
@@ -195,10 +200,14 @@ def _xform_fwd(inp, cams, out_shape, a):
     V = cams.nv if a["separate_camera_batch"] else 1
     od = [int(v) for v in out_shape]
     out = torch.empty((inp.shape[0], V) + tuple(od) + (inp.shape[-1],), dtype=torch.float32, device=inp.device)
+    s_in = _lib.shape5(inp)
+    ws, nbytes = _lib.workspace(lib.ngt_sample_grid_transform_fwd_workspace_size(s_in, a["mipmapping"],
+                                                                                 a["num_mipmaps"]), inp.device)
     _lib.check(lib.ngt_sample_grid_transform_fwd(
-        _lib.ptr(inp), _lib.shape5(inp), cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(od), a["interpolation"],
+        _lib.ptr(inp), s_in, cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(od), a["interpolation"],
         a["boundary"], a["mipmapping"], a["num_mipmaps"], a["mip_bias"], a["coordinate_mode"],
-        a["cell_center_offset"], int(a["separate_camera_batch"]), _lib.ptr(out), _lib.stream(inp.device)))
+        a["cell_center_offset"], int(a["separate_camera_batch"]), _lib.ptr(out), _lib.ptr(ws), nbytes,
+        _lib.stream(inp.device)))
     return out
 
 
@@ -209,12 +218,13 @@ def _xform_bwd(inp, out_grad, cams, a, sem):
     cams.check_batch(inp.shape[0])
     in_grad = torch.empty_like(inp)
     s_in = _lib.shape5(inp)
-    ws, nbytes = _lib.workspace(lib.ngt_sample_grid_transform_bwd_workspace_size(s_in, sem), inp.device)
+    ws, nbytes = _lib.workspace(lib.ngt_sample_grid_transform_bwd_workspace_size(s_in, a["mipmapping"],
+                                                                                 a["num_mipmaps"], sem), inp.device)
     _lib.check(lib.ngt_sample_grid_transform_bwd(
         _lib.ptr(inp), s_in, _lib.ptr(out_grad), cams.m, cams.v, cams.p, cams.f, cams.nv,
-        _lib.dhw(out_grad.shape[2:5]), a["interpolation"], a["boundary"], a["coordinate_mode"],
-        a["cell_center_offset"], int(a["separate_camera_batch"]), sem, _lib.ptr(in_grad), _lib.ptr(ws), nbytes,
-        _lib.stream(inp.device)))
+        _lib.dhw(out_grad.shape[2:5]), a["interpolation"], a["boundary"], a["mipmapping"], a["num_mipmaps"],
+        a["mip_bias"], a["coordinate_mode"], a["cell_center_offset"], int(a["separate_camera_batch"]), sem,
+        _lib.ptr(in_grad), _lib.ptr(ws), nbytes, _lib.stream(inp.device)))
     return in_grad
 
 
@@ -250,6 +260,29 @@ def sample_grid_transform_grad(input, output_grad, matrix_m, matrix_v, matrix_p,
     a = _xform_attrs(interpolation, boundary, mipmapping, num_mipmaps, mip_bias, coordinate_mode, cell_center_offset,
                      separate_camera_batch)
     return _xform_bwd(input, output_grad, _Cams(matrix_m, matrix_v, matrix_p, frustum_params), a, _sem())
+
+
+def lod_transform(input_shape, matrix_mv, matrix_p, frustum_params, output_shape, inverse_transform=False,
+                  linearize_depth=True, relative_coords=False, device=None):
+    """LodTransform (resample_grid.cc:152-172): [V, D', H', W', 4] lookup
+    tables (sample position, level of detail) of V cameras, to be read by
+    ``sample_grid_lut`` with mipmapping.  ``input_shape`` is the DHW of the
+    grid sampled from, ``output_shape`` the DHW sampled to; matrices are host
+    arrays (model-view and projection, op layout)."""
+    lib = _lib.load()
+    mv, nmv, _kmv = _lib.host_floats(matrix_mv, 16)
+    p, npr, _kp = _lib.host_floats(matrix_p, 16)
+    f, nf, _kf = _lib.host_floats(frustum_params, 6)
+    if not (nmv == npr == nf):
+        raise ValueError("camera batch size mismatch")
+    ind = [int(v) for v in input_shape][-3:]
+    od = [int(v) for v in output_shape][-3:]
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    out = torch.empty((nmv,) + tuple(od) + (4,), dtype=torch.float32, device=dev)
+    _lib.check(lib.ngt_lod_transform(_lib.dhw(ind), mv, p, f, nmv, _lib.dhw(od), int(bool(inverse_transform)),
+                                     int(bool(linearize_depth)), int(bool(relative_coords)), _lib.ptr(out),
+                                     _lib.stream(dev)))
+    return out
 
 
 # --- ReduceGridBlend -------------------------------------------------------------

@@ -102,6 +102,73 @@ class Renderer(object):
                                            self.gradient_mip_bias_add)
         return ops.sample_grid_transform(data, M, V, P, F, out_shape, coordinate_mode=CM[inverse], **kw)
 
+    # --- lookup tables (LodTransform) ------------------------------------------------
+
+    def get_camera_LuT(self, grid_transform, cam, inverse=False):
+        """[1, D, H, W, 4] lookup table (position, LoD) of one camera
+        (renderer.py:123-135)."""
+        M_model = grid_transform.get_transform_matrix().transpose()
+        V, P, F = self._get_camera_params_batch([cam])
+        MV = [M_model @ v for v in V]
+        in_shape, out_shape = ((cam.transform.grid_size, grid_transform.grid_size) if inverse
+                               else (grid_transform.grid_size, cam.transform.grid_size))
+        return ops.lod_transform(in_shape, _mat_list(MV), _mat_list(P), np.stack(F), out_shape,
+                                 inverse_transform=inverse)
+
+    def get_transform_LuT(self, from_transformations, to_transformations, relative=False, inverse=False,
+                          fix_scale_center=False):
+        """Lookup tables between grid transforms, no perspective
+        (renderer.py:137-161)."""
+        M = [t.get_transform_matrix().transpose() for t in from_transformations]
+        if fix_scale_center:
+            V = [(GridTransform(t.grid_size, scale=[2, 2, 2], center=True, normalize="ALL").get_transform_matrix()
+                  @ t.get_inverse_transform()).transpose() for t in to_transformations]
+        else:
+            V = [t.get_inverse_transform().transpose() for t in to_transformations]
+        MV = [m @ v for m, v in zip(M, V)]
+        P = [np.eye(4, dtype=np.float32) for _ in to_transformations]
+        F = [np.asarray([-1, 1, -1, 1, 1, -1], dtype=np.float32) for _ in to_transformations]
+        in_shape, out_shape = ((to_transformations[0].grid_size, from_transformations[0].grid_size) if inverse
+                               else (from_transformations[0].grid_size, to_transformations[0].grid_size))
+        return ops.lod_transform(in_shape, _mat_list(MV), _mat_list(P), np.stack(F), out_shape,
+                                 linearize_depth=False, relative_coords=relative, inverse_transform=inverse)
+
+    def _setup_static_camera_LuT(self, grid_transform, sizes_cams, inverse=False):
+        """Builds and stores the lookup tables of static cameras
+        (renderer.py:163-181)."""
+        size, cams = sizes_cams
+        M_model = grid_transform.get_transform_matrix().transpose()
+        V, P, F = self._get_camera_params_batch(cams)
+        MV = [M_model @ v for v in V]
+        in_shape, out_shape = (size, grid_transform.grid_size) if inverse else (grid_transform.grid_size, size)
+        luts = ops.lod_transform(in_shape, _mat_list(MV), _mat_list(P), np.stack(F), out_shape,
+                                 inverse_transform=inverse)
+        for lut, cam in zip(luts.unbind(0), cams):
+            if inverse:
+                cam.inverseLuT = lut
+            else:
+                cam.LuT = lut
+
+    def _get_cam_luts(self, transformation, size, cams, inverse):
+        pending = [c for c in cams if (c.inverseLuT if inverse else c.LuT) is None]
+        if pending:
+            self._setup_static_camera_LuT(transformation, (size, pending), inverse)
+        return torch.stack([c.inverseLuT if inverse else c.LuT for c in cams])
+
+    def _sample_camera_LuT(self, data, transformation, cameras, inverse=False):
+        """Camera sampling through cached lookup tables (renderer.py:325-370)."""
+        kw = dict(interpolation=self.filter_mode, boundary=self.boundary_mode, mipmapping=self.mip_mode,
+                  num_mipmaps=self.num_mips, mip_bias=self.mip_bias, coordinate_mode="LOOKUP",
+                  relative_coords=False, normalized_coords=False)
+        size = cameras[0].transform.grid_size
+        luts = self._get_cam_luts(transformation, size, cameras, inverse)
+        if self.sample_gradients:
+            if data.shape[0] != 1:
+                raise NotImplementedError("camera lut rendering does not support data batches.")
+            inv = self._get_cam_luts(transformation, size, cameras, not inverse)
+            return _GatherGradLut.apply(data, luts, inv, kw, self.gradient_mip_bias_add)
+        return ops.sample_grid_lut(data, luts, **kw)
+
     def sample_camera(self, data, transformations, cameras, inverse=False, allow_static=True, force_static=False,
                       use_step_channel=None, squeeze_batch=None):
         if data.dim() != 5:
@@ -121,10 +188,24 @@ class Renderer(object):
             cameras = [cameras]
         if len(cameras) > 1 and not all(c.transform.grid_size == cameras[0].transform.grid_size for c in cameras[1:]):
             raise ValueError("all cameras must have the same resolution (DHW). (use Renderer._sort_cameras() for batching.)")
+        sample_lut = False
+        if allow_static and any(c.static is not None for c in cameras):
+            # static cameras sample through their cached lookup tables when they belong to this grid
+            # transform and there is no data batch (renderer.py:455-465)
+            if not all(c.static == transformations[0] for c in cameras) or n != 1:
+                if force_static:
+                    raise ValueError("Camera static setup does not match transformation or data is batched")
+                self.log.warning("Incorrect static camera setup, falling back to transform rendering for static "
+                                 "cameras.")
+            else:
+                sample_lut = True
         apply_step = None
         if use_step_channel is not None and use_step_channel != []:
             data, apply_step = self._premultiply_step(data, cameras, use_step_channel, allow_post=True)
-        sampled = self._sample_camera_transform(data, transformations, cameras, inverse)
+        if sample_lut:
+            sampled = self._sample_camera_LuT(data, transformations[0], cameras, inverse)
+        else:
+            sampled = self._sample_camera_transform(data, transformations, cameras, inverse)
         if apply_step is not None:
             sampled = sampled * apply_step
         return sampled.squeeze(0) if (squeeze_batch and n == 1) else sampled
@@ -378,3 +459,21 @@ class _GatherGradSample(torch.autograd.Function):
                                             coordinate_mode=ctx.cm_bwd, separate_camera_batch=False, **kw)
             dx.append(dxv.sum(0))
         return torch.cat(dx, 0), None, None, None, None, None, None
+
+
+class _GatherGradLut(torch.autograd.Function):
+    """``sample_gradients=True`` through lookup tables (renderer.py:330-352):
+    the backward gathers dy through the inverse lookup tables and sums the
+    views."""
+
+    @staticmethod
+    def forward(ctx, data, luts, inv_luts, kw, bias_add):
+        ctx.inv_luts, ctx.kw, ctx.bias_add = inv_luts, kw, bias_add
+        return ops.sample_grid_lut(data, luts, **kw)
+
+    @staticmethod
+    def backward(ctx, dy):
+        kw = dict(ctx.kw)
+        kw["mip_bias"] = kw["mip_bias"] + ctx.bias_add
+        dx = ops.sample_grid_lut(dy[0].contiguous(), ctx.inv_luts, separate_camera_batch=False, **kw)
+        return dx.sum(0), None, None, None, None
