@@ -6,7 +6,10 @@ Mirror of the warp / render parts of phitest/render/data_structures.py:
 ``_sampled_to_component_shape`` (:1655-1682), ``_warp_vel_component`` /
 ``warped`` / ``copy_warped`` (:1081-1177), ``DensityGrid.warped`` (:614-617),
 ``State.render_density`` / ``density_advected`` / ``velocity_advected``
-(:2260-2330).  Tensors are torch NDHWC on the HIP device.
+(:2260-2330), and the resolution changes ``DensityGrid.scaled`` (:548-564),
+``VelocityGrid.scaled`` / ``resample_velocity`` / ``_centered_to_staggered``
+/ ``_scalar_centered_to_staggered`` / ``_sampled_to_shape`` (:975-1058,
+1341-1434).  Tensors are torch NDHWC on the HIP device.
 
 ``VelocityGrid.warp`` runs the fused MacCormack kernels (one SL kernel + one
 correction/clamp kernel, velocity centred on the fly from the MAC faces).
@@ -21,11 +24,16 @@ import torch
 
 from . import ops
 from .renderer import Renderer
+from .transform import GridTransform
 
 
 def _default_warp_renderer(boundary="BORDER"):
     return Renderer(filter_mode="LINEAR", boundary_mode=boundary, mipmapping="NONE", blend_mode="ADDITIVE",
                     sample_gradients=False)
+
+
+# resampling between resolutions: LINEAR, no mips (the reference's scale renderer, reconstruct_sequence.py:602-609)
+_default_scale_renderer = _default_warp_renderer
 
 
 class DensityGrid:
@@ -36,7 +44,7 @@ class DensityGrid:
         self._d = d
         self.hull = hull
         self.inflow = inflow
-        self.scale_renderer = scale_renderer
+        self.scale_renderer = scale_renderer if scale_renderer is not None else _default_scale_renderer()
 
     @property
     def d(self):
@@ -55,6 +63,20 @@ class DensityGrid:
 
     def copy_warped(self, vel_grid, order=1, dt=1.0, clamp="NONE", **kwargs):
         return DensityGrid(self.shape, d=self.warped(vel_grid, order, dt, clamp), scale_renderer=self.scale_renderer)
+
+    def scaled(self, new_shape, with_inflow=False):
+        """The density resampled to ``new_shape`` (BORDER-aligned,
+        data_structures.py:548-564)."""
+        if not (isinstance(new_shape, (list, tuple)) and len(new_shape) == 3):
+            raise ValueError("Invalid shape")
+        density = self.with_inflow() if with_inflow else self.d
+        if list(new_shape) == self.shape:
+            return density
+        return self.scale_renderer.resample_grid3D_aligned(density, list(new_shape))
+
+    def copy_scaled(self, new_shape, **kwargs):
+        hull = None if self.hull is None else self.scale_renderer.resample_grid3D_aligned(self.hull, list(new_shape))
+        return DensityGrid(new_shape, d=self.scaled(new_shape), scale_renderer=self.scale_renderer, hull=hull)
 
 
 class VelocityGrid:
@@ -78,8 +100,122 @@ class VelocityGrid:
         self.y = y if y is not None else torch.randn([1] + self.y_shape + [1], device=dev) * std
         self.z = z if z is not None else torch.randn([1] + self.z_shape + [1], device=dev) * std
         self.boundary = boundary
-        self.scale_renderer = scale_renderer
+        self.scale_renderer = scale_renderer if scale_renderer is not None else _default_scale_renderer()
         self.warp_renderer = warp_renderer if warp_renderer is not None else _default_warp_renderer()
+
+    @staticmethod
+    def shape_centered_to_staggered(shape):
+        return [v + 1 for v in shape]
+
+    @property
+    def staggered_shape(self):
+        return self.shape_centered_to_staggered(self.centered_shape)
+
+    # --- resolution changes (data_structures.py:975-1058, 1341-1382, 1413-1434) ------
+
+    def scaled(self, centered_shape, scale_magnitude=True):
+        """MAC components resampled to a new centred resolution, each aligned
+        at the cell centres along its own axis; magnitudes scaled by the
+        per-axis resolution ratio."""
+        if not (isinstance(centered_shape, (list, tuple)) and len(centered_shape) == 3):
+            raise ValueError("Invalid shape")
+        centered_shape = [int(v) for v in centered_shape]
+        if centered_shape == self.centered_shape:
+            return self.x, self.y, self.z
+        xs, ys, zs = self.component_shapes(centered_shape)
+        r = self.scale_renderer
+        x = r.resample_grid3D_aligned(self.x, xs, align_x="CENTER")
+        y = r.resample_grid3D_aligned(self.y, ys, align_y="CENTER")
+        z = r.resample_grid3D_aligned(self.z, zs, align_z="CENTER")
+        if scale_magnitude:
+            s = [o / i for i, o in zip(self.centered_shape, centered_shape)]  # z, y, x
+            x, y, z = x * s[2], y * s[1], z * s[0]
+        return x, y, z
+
+    def copy_scaled(self, centered_shape, scale_magnitude=True, **kwargs):
+        x, y, z = self.scaled(centered_shape, scale_magnitude)
+        return VelocityGrid(centered_shape, x=x, y=y, z=z, boundary=self.boundary, scale_renderer=self.scale_renderer,
+                            warp_renderer=self.warp_renderer)
+
+    @staticmethod
+    def resample_velocity(scale_renderer, vel, scale=None, shape=None, is_staggered=True, scale_magnitude=True):
+        """Velocity (MAC component triple, or centred [N,D,H,W,3]) at a new
+        resolution given by ``scale`` or ``shape`` (data_structures.py:1000-1058)."""
+        if (scale is None) == (shape is None):
+            raise ValueError("You must provide 1 of scale or shape.")
+        if isinstance(vel, tuple):
+            centered = list(vel[0].shape)
+            centered[-2] -= 1
+        elif isinstance(vel, torch.Tensor):
+            centered = list(vel.shape)
+        else:
+            raise ValueError("Unknown velcity format '%s'" % (type(vel).__name__,))
+        if len(centered) != 5:
+            raise ValueError("velocity must be 5D")
+        if scale is not None:
+            shape = [int(round(v * scale)) for v in centered[1:-1]]
+        else:
+            if not (isinstance(shape, (list, tuple)) and len(shape) == 3):
+                raise ValueError("invalid shape")
+            scale = float(np.mean([float(t) / float(s) for s, t in zip(centered[1:-1], shape)]))
+        r = scale_renderer
+        if is_staggered:
+            if not isinstance(vel, tuple):
+                raise NotImplementedError("combined staggered velocity tensors are not supported")
+            xs, ys, zs = VelocityGrid.component_shapes(shape)
+            x, y, z = vel
+            comps = (r.resample_grid3D_aligned(x, xs, align_x="CENTER", align_y="BORDER", align_z="BORDER"),
+                     r.resample_grid3D_aligned(y, ys, align_x="BORDER", align_y="CENTER", align_z="BORDER"),
+                     r.resample_grid3D_aligned(z, zs, align_x="BORDER", align_y="BORDER", align_z="CENTER"))
+            return tuple(c * scale for c in comps) if scale_magnitude else comps
+        parts = torch.split(vel, [2, 1], dim=-1)
+        out = torch.cat([r.resample_grid3D_aligned(p.contiguous(), shape) for p in parts], dim=-1)
+        return out * scale if scale_magnitude else out
+
+    def _centered_to_staggered(self, centered, output_centered_shape=None, scale_magnitude=True):
+        """Centred [N,D,H,W,3] velocity -> MAC components of
+        ``output_centered_shape`` (data_structures.py:1341-1364)."""
+        if centered.dim() != 5 or centered.shape[-1] != 3:
+            raise ValueError("centered velocity must be [N,D,H,W,3]")
+        in_shape = [int(v) for v in centered.shape[1:4]]
+        if output_centered_shape is None:
+            output_centered_shape, factors = in_shape, None
+        else:
+            factors = [o / i for o, i in zip(output_centered_shape, in_shape)]
+        xs, ys, zs = self.component_shapes(list(output_centered_shape))
+        cx, cy, cz = (c.contiguous() for c in torch.split(centered, 1, dim=-1))
+        r = self.scale_renderer
+        x = r.resample_grid3D_aligned(cx, xs, align_x="CENTER", align_y="BORDER", align_z="BORDER")
+        y = r.resample_grid3D_aligned(cy, ys, align_x="BORDER", align_y="CENTER", align_z="BORDER")
+        z = r.resample_grid3D_aligned(cz, zs, align_x="BORDER", align_y="BORDER", align_z="CENTER")
+        if scale_magnitude and factors is not None:
+            x, y, z = x * factors[2], y * factors[1], z * factors[0]
+        return x, y, z
+
+    def _scalar_centered_to_staggered(self, centered, allow_split_channels=False):
+        """A centred scalar field on the staggered (+1) grid, its cell centres
+        on the input's cell borders (data_structures.py:1366-1382)."""
+        if centered.dim() != 5:
+            raise ValueError("centered field must be 5D")
+        staggered = self.shape_centered_to_staggered([int(v) for v in centered.shape[1:4]])
+        return self.scale_renderer.resample_grid3D_aligned(centered, staggered, align_x="STAGGER_OUTPUT",
+                                                           align_y="STAGGER_OUTPUT", align_z="STAGGER_OUTPUT",
+                                                           allow_split_channels=allow_split_channels)
+
+    def _sampled_to_shape(self, shape):
+        """The MAC components resampled onto a grid of ``shape`` with uniform
+        scaling (data_structures.py:1413-1434)."""
+        shape = [int(v) for v in shape]
+        scale = [2.0 / v for v in shape[::-1]]
+        target = GridTransform(shape)
+        out = []
+        for comp, cs in ((self.x, self.x_shape), (self.y, self.y_shape), (self.z, self.z_shape)):
+            if shape == cs:
+                out.append(comp)
+            else:
+                t = GridTransform(cs, scale=scale, center=True)
+                out.append(self.scale_renderer._sample_transform(comp, [t], [target]).squeeze(1))
+        return out
 
     def assign(self, x, y, z):
         self.x, self.y, self.z = x, y, z

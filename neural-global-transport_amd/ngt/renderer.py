@@ -4,7 +4,9 @@ Same constructor and methods as the reference ``Renderer`` for the render and
 warp paths: ``render_density`` (renderer.py:718-827), ``_render_cameras``
 (:669-682), ``raymarch_camera`` / ``_raymarch_camera_transform`` (:372-555),
 ``sample_camera`` / ``_sample_camera_transform`` (:290-504), ``_sample_LuT``
-(:559-574), ``_blend_grid`` (:577-640), ``_build_light_grid`` (:687-716),
+(:559-574), ``_sample_transform`` / ``resample_grid3D_aligned`` (:222-286,
+843-905), the lookup-table helpers (:123-196, 325-370), ``unproject``
+(:952-976), ``_blend_grid`` (:577-640), ``_build_light_grid`` (:687-716),
 ``resample_grid3D_offset`` (:922-936).  Data are NDHWC torch tensors on the
 HIP device; images are returned as a list of NHWC tensors (one per camera).
 
@@ -169,6 +171,71 @@ class Renderer(object):
             return _GatherGradLut.apply(data, luts, inv, kw, self.gradient_mip_bias_add)
         return ops.sample_grid_lut(data, luts, **kw)
 
+    # --- orthographic resampling between grids --------------------------------------
+
+    def _sample_transform(self, data, from_transformations, to_transformations, inverse=False,
+                          fix_scale_center=False, **kernelargs):
+        """Resampling between grid transforms through an orthographic identity
+        camera over the [-1, 1] cube of the target (renderer.py:222-286);
+        returns NVDHWC."""
+        CM = {False: "TRANSFORM", True: "TRANSFORM_REVERSE"}
+        M = [t.get_transform_matrix().transpose() for t in from_transformations]
+        if fix_scale_center:
+            V = [(GridTransform(t.grid_size, scale=[2, 2, 2], center=True, normalize="ALL").get_transform_matrix()
+                  @ t.get_inverse_transform()).transpose() for t in to_transformations]
+        else:
+            V = [t.get_inverse_transform().transpose() for t in to_transformations]
+        P = [np.eye(4, dtype=np.float32) for _ in to_transformations]
+        F = np.asarray([[-1, 1, -1, 1, 1, -1]] * len(to_transformations), dtype=np.float32)
+        cams = (_mat_list(M), _mat_list(V), _mat_list(P), F)
+        out_shape = tuple(int(v) for v in (from_transformations[0].grid_size if inverse
+                                           else to_transformations[0].grid_size))
+        kw = dict(interpolation=kernelargs.get("interpolation", self.filter_mode),
+                  boundary=kernelargs.get("boundary", self.boundary_mode),
+                  mipmapping=kernelargs.get("mipmapping", self.mip_mode), num_mipmaps=self.num_mips,
+                  mip_bias=self.mip_bias)
+        if self.sample_gradients:
+            return _GatherGradSample.apply(data, cams, out_shape, CM[inverse], CM[not inverse], kw,
+                                           self.gradient_mip_bias_add)
+        return ops.sample_grid_transform(data, *cams, out_shape, coordinate_mode=CM[inverse], **kw)
+
+    @staticmethod
+    def _aligned_scale(align, in_size, out_size):
+        a = align.upper()
+        if a == "BORDER":
+            return 2.0 / in_size
+        if a == "CENTER":  # outer cell centres meet
+            return 2.0 / out_size * (out_size - 1.0) / (in_size - 1.0)
+        if a == "STAGGER_INPUT":  # input centres on output borders
+            return 2.0 / (in_size - 1.0)
+        if a == "STAGGER_OUTPUT":  # input borders on output centres
+            return 2.0 / (in_size + 1.0)
+        raise ValueError("unknown alignment {}".format(align))
+
+    def resample_grid3D_aligned(self, data, target_shape, align_x="BORDER", align_y="BORDER", align_z="BORDER",
+                                allow_split_channels=False):
+        """[N, D, H, W, C] -> [N, target D, H, W, C] with per-axis alignment
+        BORDER / CENTER / STAGGER_INPUT / STAGGER_OUTPUT (renderer.py:843-905).
+        Channel counts other than 1, 2, 4 need ``allow_split_channels`` (the
+        channels are then resampled as separate 1-channel grids)."""
+        if data.dim() != 5:
+            raise ValueError("input shape must be NDHWC, is : {}".format(list(data.shape)))
+        if len(target_shape) != 3:
+            raise ValueError("target_shape must have 3 entries")
+        N, D, H, W, C = data.shape
+        if not allow_split_channels and C not in (1, 2, 4):
+            raise ValueError("resampling supports 1, 2 or 4 channels, is {}".format(C))
+        tz, ty, tx = (int(v) for v in target_shape)
+        scale = [self._aligned_scale(align_x, W, tx), self._aligned_scale(align_y, H, ty),
+                 self._aligned_scale(align_z, D, tz)]
+        in_transform = GridTransform([D, H, W], scale=scale, center=True)
+        target_transform = GridTransform([tz, ty, tx])
+        if C not in (1, 2, 4):
+            x = data.permute(0, 4, 1, 2, 3).reshape(N * C, D, H, W, 1)
+            y = self._sample_transform(x, [in_transform] * (N * C), [target_transform]).squeeze(1)
+            return y.reshape(N, C, tz, ty, tx).permute(0, 2, 3, 4, 1).contiguous()
+        return self._sample_transform(data, [in_transform] * N, [target_transform]).squeeze(1)
+
     def sample_camera(self, data, transformations, cameras, inverse=False, allow_static=True, force_static=False,
                       use_step_channel=None, squeeze_batch=None):
         if data.dim() != 5:
@@ -243,6 +310,20 @@ class Renderer(object):
             out_shape = [cameras[0].transform.grid_size[0], data.shape[2], data.shape[3]]
             return ops.raymarch_grid_transform_grad(dummy, data, data, M, V, P, F, out_shape, **kw)
         return ops.raymarch_grid_transform(data, M, V, P, F, cameras[0].transform.grid_size, **kw)
+
+    def unproject(self, grid_transform, targets, cameras, blend_func=torch.minimum):
+        """Volume estimate from per-camera target images [V, H, W, C]: each
+        target, extruded along its frustum depth, sampled back into the grid
+        and combined with ``blend_func`` (renderer.py:952-976, non-fused
+        branch)."""
+        C = targets.shape[-1]
+        out = torch.ones((1,) + tuple(int(v) for v in grid_transform.grid_size) + (C,), dtype=torch.float32,
+                         device=targets.device)
+        for i, cam in enumerate(cameras):
+            t = targets[i].reshape((1, 1) + tuple(targets[i].shape))
+            t = t.expand((1, int(cam.transform.grid_size[0])) + tuple(targets[i].shape)).contiguous()
+            out = blend_func(out, self.sample_camera(t, grid_transform, cam, inverse=True))
+        return out
 
     def raymarch_camera(self, data, transformations, cameras, use_step_channel=None, squeeze_batch=None,
                         **raymarch_kwargs):

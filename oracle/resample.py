@@ -320,3 +320,52 @@ def sample_grid_transform_grad(inp, out_grad, M, V, P, F, filter_mode=LINEAR, bo
             nz = count > 0
             in_grad[n][nz] = in_grad[n][nz] / count[nz][:, None]
     return in_grad
+
+
+# --- aligned resampling (orthographic transform sampling) ------------------------
+
+def aligned_scale(align, in_size, out_size):
+    """Per-axis input scale of resample_grid3D_aligned (renderer.py:864-876):
+    BORDER aligns the outer cell borders, CENTER the outer cell centres,
+    STAGGER_INPUT input centres to output borders, STAGGER_OUTPUT input
+    borders to output centres."""
+    a = align.upper()
+    if a == "BORDER":
+        return 2.0 / in_size
+    if a == "CENTER":
+        return 2.0 / out_size * (out_size - 1.0) / (in_size - 1.0)
+    if a == "STAGGER_INPUT":
+        return 2.0 / (in_size - 1.0)
+    if a == "STAGGER_OUTPUT":
+        return 2.0 / (in_size + 1.0)
+    raise ValueError("unknown alignment {}".format(align))
+
+
+def aligned_args(in_dhw, out_dhw, align_xyz):
+    """Op arguments (M, V, P, F) of resample_grid3D_aligned: the input grid,
+    centred and scaled per axis, seen by an orthographic identity camera over
+    [-1, 1]^3 whose frustum grid is the target (renderer.py:222-238,884-905)."""
+    sc = [aligned_scale(align_xyz[0], in_dhw[2], out_dhw[2]), aligned_scale(align_xyz[1], in_dhw[1], out_dhw[1]),
+          aligned_scale(align_xyz[2], in_dhw[0], out_dhw[0])]
+    M = G.grid_transform(in_dhw, scale_=sc, center=True)
+    V = np.linalg.inv(G.grid_transform(out_dhw))
+    return G.op_matrix(M), G.op_matrix(V), G.op_matrix(np.eye(4)), np.array([-1.0, 1.0, -1.0, 1.0, 1.0, -1.0])
+
+
+def resample_grid3D_aligned(data, out_dhw, align_xyz=("BORDER", "BORDER", "BORDER"), filter_mode=LINEAR,
+                            boundary=BORDER):
+    """[N, D, H, W, C] -> [N, D', H', W', C] (renderer.py:843-905)."""
+    data = np.asarray(data, dtype=np.float64)
+    M, V, P, F = aligned_args(data.shape[1:4], out_dhw, align_xyz)
+    N = data.shape[0]
+    return sample_grid_transform(data, [M] * N, [V], [P], [F], out_dhw, filter_mode, boundary, TRANSFORM)[:, 0]
+
+
+def resample_grid3D_aligned_grad(data_shape, out_grad, align_xyz=("BORDER", "BORDER", "BORDER"), filter_mode=LINEAR,
+                                 boundary=BORDER):
+    """Exact adjoint of ``resample_grid3D_aligned`` w.r.t. its data."""
+    out_grad = np.asarray(out_grad, dtype=np.float64)
+    M, V, P, F = aligned_args(data_shape[1:4], out_grad.shape[1:4], align_xyz)
+    N = data_shape[0]
+    return sample_grid_transform_grad(np.zeros(data_shape), out_grad[:, None], [M] * N, [V], [P], [F], filter_mode,
+                                      boundary, TRANSFORM)

@@ -190,3 +190,50 @@ def test_density_warped_staggered(ngt, fused):
     assert rel(host(rt.grad)[0], eg_rho) < GRAD_TOL
     for t, e in zip(ts, A.centered_from_staggered_adjoint(eg_v)):
         assert rel(host(t.grad)[0], e) < GRAD_TOL
+
+
+@pytest.mark.parametrize("semantics", ["exact", "reference"])
+def test_inverse_raymarch_and_its_gradient(ngt, semantics):
+    """Inverse raymarching (renderer.py:372-398): the ADDITIVE raymarch
+    gradient op applied to images lifts them into the grid; its gradient is
+    the ADDITIVE forward raymarch (ops_loader.py:98-118)."""
+    from ngt.renderer import Renderer
+    from ngt import ops
+    rng = np.random.default_rng(51)
+    dims, hw, depth = (12, 11, 13), (10, 14), 16
+    grid, cams = _scene(ngt, dims, hw, depth)
+    imgs = f32(rng.uniform(0.0, 1.0, (1, len(cams)) + hw + (2,)))
+    Ms, Vs, Ps, Fs = _op_args(ngt, grid, cams)
+    out_dhw = (depth,) + hw
+    r = Renderer(filter_mode="LINEAR", boundary_mode="BORDER", mipmapping="NONE", blend_mode="ADDITIVE")
+    it = dev(imgs).requires_grad_()
+    with ops.semantics(semantics):
+        vol = r.raymarch_camera(it, grid, cams, inverse=True, squeeze_batch=False)
+        exp = RM.raymarch_grid_transform_grad(np.zeros((1,) + dims + (2,)), imgs, imgs, Ms, Vs, Ps, Fs, out_dhw,
+                                              LINEAR, BORDER, B.ADDITIVE, semantics=semantics)
+        assert tuple(vol.shape) == exp.shape
+        assert rel(host(vol), exp) < GRAD_TOL
+        G = f32(rng.standard_normal(exp.shape))
+        vol.backward(dev(G))
+    eg = RM.raymarch_grid_transform(G, Ms, Vs, Ps, Fs, out_dhw, LINEAR, BORDER, B.ADDITIVE)
+    assert rel(host(it.grad), eg) < FWD_TOL
+
+
+def test_unproject_min_of_inverse_samples(ngt):
+    """Renderer.unproject: minimum over the cameras of each target extruded
+    along depth and sampled back into the grid (TRANSFORM_LINDEPTH_REVERSE)."""
+    from ngt.renderer import Renderer
+    rng = np.random.default_rng(52)
+    dims, hw, depth = (10, 9, 11), (8, 12), 6
+    grid, cams = _scene(ngt, dims, hw, depth, views=2)
+    targets = f32((rng.uniform(0.0, 1.0, (2,) + hw + (1,)) > 0.3).astype(np.float64))
+    r = Renderer(filter_mode="LINEAR", boundary_mode="BORDER", mipmapping="NONE")
+    got = host(r.unproject(grid, dev(targets), cams))
+    Ms, Vs, Ps, Fs = _op_args(ngt, grid, cams)
+    exp = np.ones((1,) + dims + (1,))
+    for v in range(2):
+        tar = np.broadcast_to(targets[v][None, None], (1, depth) + hw + (1,))
+        s = R.sample_grid_transform(tar, Ms, Vs[v:v + 1], Ps[v:v + 1], Fs[v:v + 1], dims, LINEAR, BORDER,
+                                    R.TRANSFORM_LINDEPTH_REVERSE)[:, 0]
+        exp = np.minimum(exp, s)
+    assert rel(got, exp) < FWD_TOL
