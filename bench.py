@@ -6,8 +6,9 @@ MAC-staggered curl-noise velocity (129x128x128 / 128x129x128 / 128x128x129,
 max |v| = 2 voxels/step); one MacCormack step (dt=1, trilinear, MC_SMOOTH
 revert clamp, BORDER) followed by the 5-view render (ring of 5 pinhole
 cameras, 256x144 pixels, 128 depth slices, BEER_LAMBERT, constant light);
-L2 loss against target images rendered from a perturbed density; full
-backward to the density and the three face arrays.  Inputs are resident in
+L2 loss against target images rendered from a perturbed density (reduced,
+and its image gradient formed, inside the render op); full backward to the
+density and the three face arrays.  Inputs are resident in
 HBM before timing starts.
 
 Prints one JSON line.  value = voxel-updates/s (128^3 cells advected per
@@ -66,7 +67,7 @@ OP_KERNELS = {
     "advect_fwd": ["advect_sl_kernel", "advect_mc_kernel"],
     "advect_bwd": ["advect_mc_bwd_brick_kernel", "advect_sl_bwd_brick_kernel", "advect_rho_gather_kernel",
                    "faces_from_centered_grad_kernel", "fill_kernel"],
-    "render_fwd": ["to_bricked_kernel", "raymarch_fwd_kernel"],
+    "render_fwd": ["to_bricked_kernel", "raymarch_fwd_kernel", "l2_partial_kernel", "l2_finish_kernel"],
     "render_bwd": ["to_bricked_kernel", "raymarch_bwd_prepass_kernel", "raymarch_bwd_brick_kernel"],
 }
 COPY_BYTES = 256 << 20
@@ -168,9 +169,8 @@ def fused_step(torch, ops, prob, leaves):
     rho, vx, vy, vz = leaves
     adv = ops.advect_grid_semi_langrange(rho, (vx, vy, vz), timestep=-1.0, order=2, clamp_extrema="MC_SMOOTH",
                                          boundary="BORDER", separate_velocity_batch=False)[:, 0]
-    img = ops.render_density_fused(adv, prob["light"], prob["ds"], prob["M"], prob["V"], prob["P"], prob["F"],
-                                   (DEPTH,) + IMAGE_HW)
-    loss = ((img[..., :3] - prob["target"]) ** 2).sum()
+    loss, _ = ops.render_density_fused_l2(adv, prob["target"], prob["light"], prob["ds"], prob["M"], prob["V"],
+                                          prob["P"], prob["F"], (DEPTH,) + IMAGE_HW)
     loss.backward()
     return loss
 
@@ -264,11 +264,14 @@ def op_calls(torch, prob):
     ws = torch.empty(wsb, dtype=torch.uint8, device=rho.device)
     g_out = torch.randn_like(adv)  # gradient of the advected field, [1,1,D,H,W,1]
     img = torch.empty((1, VIEWS) + IMAGE_HW + (4,), device=rho.device)
-    dimg = torch.randn_like(img)
+    tgt = prob["target"]
+    loss = torch.empty((), device=rho.device)
+    loss_grad = torch.ones((), device=rho.device)
     g_adv = torch.empty_like(rho)
-    rws_b = lib.ngt_render_density_bwd_workspace_size(s_rho, cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0)
+    rws_b = lib.ngt_render_density_l2_bwd_workspace_size(s_rho, cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1,
+                                                         0)
     rws = torch.empty(max(rws_b, 1), dtype=torch.uint8, device=rho.device)
-    fws_b = lib.ngt_render_density_fwd_workspace_size(s_rho, 1)
+    fws_b = lib.ngt_render_density_l2_fwd_workspace_size(s_rho, 1, cams.nv, out_dhw)
     fws = torch.empty(max(fws_b, 1), dtype=torch.uint8, device=rho.device)
     P = _lib.ptr
     calls = {
@@ -277,18 +280,19 @@ def op_calls(torch, prob):
         "advect_bwd": lambda: _lib.check(lib.ngt_advect_bwd(
             P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(g_out), P(sl), 0, P(g_in), P(gx), P(gy),
             P(gz), P(ws), wsb, st())),
-        "render_fwd": lambda: _lib.check(lib.ngt_render_density_fwd(
-            P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(img), P(fws),
-            fws_b, st())),
-        "render_bwd": lambda: _lib.check(lib.ngt_render_density_bwd(
-            P(rho), s_rho, lp, prob["ds"], P(img), P(dimg), cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0,
-            0, P(g_adv), P(rws), rws_b, st())),
+        "render_fwd": lambda: _lib.check(lib.ngt_render_density_l2_fwd(
+            P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(tgt), 1.0,
+            P(img), P(loss), P(fws), fws_b, st())),
+        "render_bwd": lambda: _lib.check(lib.ngt_render_density_l2_bwd(
+            P(rho), s_rho, lp, prob["ds"], P(img), P(tgt), 1.0, P(loss_grad), cams.m, cams.v, cams.p, cams.f,
+            cams.nv, out_dhw, 1, 0, 0, P(g_adv), P(rws), rws_b, st())),
     }
     # every raw buffer handed to the C ABI has the element count its entry
     # point reads (the ABI takes plain pointers and cannot check them)
     cells = int(np.prod(GRID))
     for t, n in ((adv, cells), (sl, cells), (g_out, cells), (g_in, cells), (g_adv, cells),
-                 (img, VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * 4), (dimg, VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * 4)):
+                 (img, VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * 4), (tgt, VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * 3),
+                 (loss, 1), (loss_grad, 1)):
         assert t.numel() == n, (tuple(t.shape), n)
     calls["render_fwd"]()
     return calls
@@ -538,16 +542,17 @@ def main():
     pixels = VIEWS * IMAGE_HW[0] * IMAGE_HW[1]
     # algorithmic bytes per call (compulsory interface traffic):
     #   advect fwd 20 B/voxel, advect bwd 36 B/voxel (staggered faces ~12 B/cell),
-    #   render fwd 4 B/voxel + 16 B/pixel, render bwd 8 B/voxel + 32 B/pixel
-    algo = {"advect_fwd": cells * 20, "advect_bwd": cells * 36, "render_fwd": cells * 4 + pixels * 16,
-            "render_bwd": cells * 8 + pixels * 32}
+    #   render + L2 fwd 4 B/voxel + 28 B/pixel (image out, RGB target in),
+    #   render + L2 bwd 8 B/voxel + 28 B/pixel (image and RGB target in)
+    algo = {"advect_fwd": cells * 20, "advect_bwd": cells * 36, "render_fwd": cells * 4 + pixels * 28,
+            "render_bwd": cells * 8 + pixels * 28}
     dominant = max(ops_ms, key=ops_ms.get)
     ms_kernel = ops_ms[dominant]
     kbytes = algo[dominant]
     achieved = kbytes / (ms_kernel * 1e-3) / 1e9
-    # whole-step compulsory bytes: advect fwd 20 + bwd 36 per voxel, render fwd
-    # 4/voxel + 16/pixel, render bwd 8/voxel + 32/pixel
-    step_bytes = cells * (20 + 36 + 4 + 8) + pixels * (16 + 32)
+    # whole-step compulsory bytes: advect fwd 20 + bwd 36 per voxel, render + L2
+    # fwd 4/voxel + 28/pixel, bwd 8/voxel + 28/pixel
+    step_bytes = cells * (20 + 36 + 4 + 8) + pixels * (28 + 28)
     best_ms = ms_graph if ms_graph is not None else ms
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_step128.json")
@@ -577,7 +582,7 @@ def main():
         "ms_per_step_graph": ms_graph,
         "ms_per_step_plain": ms_plain,
         "vs_plain": (ms_plain / best_ms) if ms_plain else None,
-        "roofline": {"kernel": "ngt_" + dominant.replace("render", "render_density") + " (" +
+        "roofline": {"kernel": "ngt_" + dominant.replace("render", "render_density_l2") + " (" +
                                " + ".join(OP_KERNELS[dominant]) + ")",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "ms_per_launch": ms_kernel,

@@ -222,6 +222,33 @@ int ngt_render_density_bwd(const float* density, const int64_t* density_shape, c
                            const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
                            float* density_grad, void* workspace, size_t workspace_bytes, ngt_stream_t stream);
 
+/* ---- Fused render + L2 image loss (optimization.py:478-526, the density
+ * target loss with loss_func = squared error) ---------------------------------
+ * _l2_fwd renders like ngt_render_density_fwd into output [N,V,H,W,4] and
+ * writes *loss (one device float) = loss_weight * sum over N,V,H,W and RGB of
+ * (output - target)^2, target [N,V,H,W,3]; the sum is reduced in a fixed
+ * order.  _l2_bwd writes density_grad for the upstream gradient *loss_grad
+ * (device float): the image gradient 2 * loss_weight * loss_grad *
+ * (output - target) is formed inside the render backward's gradient pass
+ * instead of being read from memory. */
+size_t ngt_render_density_l2_fwd_workspace_size(const int64_t* density_shape, int interpolation, int num_cameras,
+                                                const int32_t* output_dhw);
+int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
+                              float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
+                              const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
+                              int boundary, int blending_mode, const float* target, float loss_weight, float* output,
+                              float* loss, void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+size_t ngt_render_density_l2_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
+                                                const float* matrix_v, const float* matrix_p, const float* frustum,
+                                                int num_cameras, const int32_t* output_dhw, int interpolation,
+                                                int boundary);
+int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape, const float* light_rgb,
+                              float depth_step, const float* output, const float* target, float loss_weight,
+                              const float* loss_grad, const float* matrix_m, const float* matrix_v,
+                              const float* matrix_p, const float* frustum, int num_cameras, const int32_t* output_dhw,
+                              int interpolation, int boundary, int blending_mode, float* density_grad,
+                              void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+
 /* ---- AdvectGridSemiLangrange (advect.cc:21-71) with VelocityGrid.warp
  * semantics (data_structures.py:1690-1737) --------------------------------
  * input [N,D,H,W,C]; velocity either centered (vel_y = vel_z = NULL, vel_x =

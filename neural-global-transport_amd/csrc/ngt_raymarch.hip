@@ -37,6 +37,12 @@ struct RayArgs {
   const float* bricked;                         // bricked copy of input (kernels with BRICKED = true)
   BrickDims bd;
   int n_data;
+  // fused L2 image loss (render backward): when l2_target is set the image
+  // gradient is 2 * l2_scale * l2_loss_grad[0] * (image - target) on RGB and
+  // 0 on the optical depth, formed in the gradient pass instead of read
+  const float* l2_target;     // [N*V, H, W, 3]
+  const float* l2_loss_grad;  // one float on the device
+  float l2_scale;
 };
 
 NGT_DEVICE int kMaxRaySteps(const RayArgs& a) { return 16 * (a.din.x + a.din.y + a.din.z + a.out_d); }
@@ -353,6 +359,18 @@ struct RayRec {
   int pad;
 };
 
+// Image gradient of the fused L2 loss at one pixel (img: the pixel's C
+// channels, tpix: its first target value).
+template <int C>
+NGT_DEVICE Vec<C> l2_image_grad(const RayArgs& a, const float* __restrict__ img, int64_t tpix) {
+  const float s = 2.f * a.l2_scale * a.l2_loss_grad[0];
+  const Vec<C> v = vload<C>(img);
+  Vec<C> g = vzero<C>();
+#pragma unroll
+  for (int c = 0; c < (C < 3 ? C : 3); ++c) g.v[c] = s * (v.v[c] - a.l2_target[tpix + c]);
+  return g;
+}
+
 template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
 NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
                                                                    const float* __restrict__ out,
@@ -387,7 +405,8 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
       vstore<CG>(gp + (static_cast<int64_t>(k) * npix + pixel) * CG, fold_cell_grad<C, LIGHT>(a, gc));
     }
   } sink{a, G + static_cast<int64_t>(pair) * kcap * npix * CG, npix, pixel};
-  walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT, BRICKED>(a, src, r, vload<C>(out_grad + pix), out + pix, sink);
+  const Vec<C> g_img = a.l2_target ? l2_image_grad<C>(a, out + pix, pix / C * 3) : vload<C>(out_grad + pix);
+  walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT, BRICKED>(a, src, r, g_img, out + pix, sink);
 }
 
 template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
@@ -804,6 +823,9 @@ RayArgs make_ray_args(const float* input, const int64_t* shape, const int32_t* o
   a.inv_w = 1.f / a.out_w; a.inv_h = 1.f / a.out_h; a.inv_d = 1.f / a.out_d;
   a.light_r = a.light_g = a.light_b = 0.f;
   a.depth_step = 1.f;
+  a.l2_target = nullptr;
+  a.l2_loss_grad = nullptr;
+  a.l2_scale = 0.f;
   return a;
 }
 
@@ -985,6 +1007,48 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
 }
 
 // ---------------------------------------------------------------------------
+// Fused L2 image loss: sum over pixels and RGB of (image - target)^2, as
+// per-work-group partial sums and one fixed-order final sum (deterministic).
+
+__global__ __launch_bounds__(256) void l2_partial_kernel(const float* __restrict__ img, const float* __restrict__ tgt,
+                                                         int64_t npix, float* __restrict__ partial) {
+  __shared__ float s_w[4];
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  float v = 0.f;
+  if (p < npix) {
+    const float4 q = *reinterpret_cast<const float4*>(img + p * 4);
+    const float d0 = q.x - tgt[p * 3], d1 = q.y - tgt[p * 3 + 1], d2 = q.z - tgt[p * 3 + 2];
+    v = d0 * d0 + d1 * d1 + d2 * d2;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
+__global__ __launch_bounds__(256) void l2_finish_kernel(const float* __restrict__ partial, int n, float weight,
+                                                        float* __restrict__ loss) {
+  __shared__ double s_w[4];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += static_cast<double>(partial[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *loss = static_cast<float>(weight * ((s_w[0] + s_w[1]) + (s_w[2] + s_w[3])));
+}
+
+// Image gradient of the fused L2 loss for the backward paths that read an
+// image gradient buffer.
+__global__ __launch_bounds__(256) void l2_image_grad_kernel(RayArgs a, const float* __restrict__ img, int64_t npix,
+                                                            float* __restrict__ dimg) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (p >= npix) return;
+  vstore<4>(dimg + p * 4, l2_image_grad<4>(a, img + p * 4, p * 3));
+}
+
+// ---------------------------------------------------------------------------
 // Uneven batches of fused density renders: every instance (one density grid,
 // 1-8 views of one image size) in one persistent launch per kernel.
 struct RenBatchHeader {
@@ -1080,6 +1144,11 @@ __global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* __r
                                                it.density_grad, 0);
     __syncthreads();  // the brick's LDS arrays are rewritten by the next brick
   }
+}
+
+// Pixels of the N x V images of a fused render.
+int64_t render_pixels(const int64_t* density_shape, int num_cameras, const int32_t* output_dhw) {
+  return density_shape[0] * num_cameras * static_cast<int64_t>(output_dhw[1]) * output_dhw[2];
 }
 
 }  // namespace
@@ -1273,6 +1342,99 @@ int ngt_render_density_bwd(const float* density, const int64_t* density_shape, c
     });
   }
   return check_launch("ngt_render_density_bwd");
+}
+
+size_t ngt_render_density_l2_fwd_workspace_size(const int64_t* density_shape, int interpolation, int num_cameras,
+                                                const int32_t* output_dhw) {
+  if (density_shape == nullptr || output_dhw == nullptr || num_cameras <= 0) return 0;
+  const int64_t blocks = (render_pixels(density_shape, num_cameras, output_dhw) + 255) / 256;
+  return align_up(ngt_render_density_fwd_workspace_size(density_shape, interpolation)) +
+         align_up(static_cast<size_t>(blocks) * sizeof(float));
+}
+
+int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
+                              float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
+                              const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
+                              int boundary, int blending_mode, const float* target, float loss_weight, float* output,
+                              float* loss, void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
+  NGT_REQUIRE(target != nullptr && loss != nullptr, NGT_ERR_INVALID_ARGUMENT, "target and loss must be given");
+  NGT_REQUIRE(output_dhw != nullptr && num_cameras > 0 && density_shape != nullptr, NGT_ERR_INVALID_ARGUMENT,
+              "Invalid output_shape");
+  const size_t need = ngt_render_density_l2_fwd_workspace_size(density_shape, interpolation, num_cameras, output_dhw);
+  NGT_REQUIRE(workspace != nullptr && workspace_bytes >= need, NGT_ERR_WORKSPACE,
+              "workspace too small for ngt_render_density_l2_fwd");
+  const size_t bricked = align_up(ngt_render_density_fwd_workspace_size(density_shape, interpolation));
+  int rc = ngt_render_density_fwd(density, density_shape, light_rgb, depth_step, matrix_m, matrix_v, matrix_p, frustum,
+                                  num_cameras, output_dhw, interpolation, boundary, blending_mode, output,
+                                  bricked ? workspace : nullptr, bricked, stream);
+  if (rc) return rc;
+  const int64_t npix = render_pixels(density_shape, num_cameras, output_dhw);
+  const unsigned blocks = blocks_for(npix, 256);
+  float* partial = reinterpret_cast<float*>(static_cast<char*>(workspace) + bricked);
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(l2_partial_kernel, dim3(blocks), dim3(256), 0, s, output, target, npix, partial);
+  hipLaunchKernelGGL(l2_finish_kernel, dim3(1), dim3(256), 0, s, partial, static_cast<int>(blocks), loss_weight, loss);
+  return check_launch("ngt_render_density_l2_fwd");
+}
+
+size_t ngt_render_density_l2_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
+                                                const float* matrix_v, const float* matrix_p, const float* frustum,
+                                                int num_cameras, const int32_t* output_dhw, int interpolation,
+                                                int boundary) {
+  const size_t g = ngt_render_density_bwd_workspace_size(density_shape, matrix_m, matrix_v, matrix_p, frustum,
+                                                         num_cameras, output_dhw, interpolation, boundary);
+  if (g > 0 || density_shape == nullptr || output_dhw == nullptr || num_cameras <= 0) return g;
+  return static_cast<size_t>(render_pixels(density_shape, num_cameras, output_dhw)) * 4 * sizeof(float);
+}
+
+int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape, const float* light_rgb,
+                              float depth_step, const float* output, const float* target, float loss_weight,
+                              const float* loss_grad, const float* matrix_m, const float* matrix_v,
+                              const float* matrix_p, const float* frustum, int num_cameras, const int32_t* output_dhw,
+                              int interpolation, int boundary, int blending_mode, float* density_grad,
+                              void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
+  std::vector<Pair> pairs;
+  int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
+                          boundary, blending_mode, 1, pairs, true);
+  if (rc) return rc;
+  NGT_REQUIRE(light_rgb != nullptr, NGT_ERR_INVALID_ARGUMENT, "light_rgb must be given");
+  NGT_REQUIRE(output != nullptr && target != nullptr && loss_grad != nullptr, NGT_ERR_INVALID_ARGUMENT,
+              "output, target and loss_grad must be given");
+  // the gather adjoint when the workspace holds its buffers, else the image
+  // gradient buffer and the scatter adjoint
+  const int64_t npix = render_pixels(density_shape, num_cameras, output_dhw);
+  const size_t img_need = static_cast<size_t>(npix) * 4 * sizeof(float);
+  const size_t g_need = ngt_render_density_bwd_workspace_size(density_shape, matrix_m, matrix_v, matrix_p, frustum,
+                                                              num_cameras, output_dhw, interpolation, boundary);
+  const bool gather = g_need > 0 && workspace != nullptr && workspace_bytes >= g_need;
+  NGT_REQUIRE(gather || (workspace != nullptr && workspace_bytes >= img_need), NGT_ERR_WORKSPACE,
+              "workspace too small for ngt_render_density_l2_bwd");
+  RayArgs a = make_ray_args(density, density_shape, output_dhw);
+  a.light_r = light_rgb[0]; a.light_g = light_rgb[1]; a.light_b = light_rgb[2];
+  a.depth_step = depth_step;
+  a.l2_target = target;
+  a.l2_loss_grad = loss_grad;
+  a.l2_scale = loss_weight;
+  hipStream_t s = as_stream(stream);
+  GatherPlan gp;
+  if (gather && plan_gather(pairs, matrix_m, matrix_v, matrix_p, frustum, a, interpolation, boundary, 1, workspace,
+                            workspace_bytes, gp)) {
+    run_gather<true>(pairs, matrix_m, matrix_v, matrix_p, frustum, a, blending_mode, 4, output, nullptr, gp,
+                     density_grad, static_cast<int>(density_shape[0]), s);
+    return check_launch("ngt_render_density_l2_bwd");
+  }
+  NGT_REQUIRE(workspace_bytes >= img_need, NGT_ERR_WORKSPACE, "workspace too small for ngt_render_density_l2_bwd");
+  float* dimg = static_cast<float*>(workspace);
+  hipLaunchKernelGGL(l2_image_grad_kernel, dim3(blocks_for(npix, 256)), dim3(256), 0, s, a, output, npix, dimg);
+  a.l2_target = nullptr;
+  fill_zero(density_grad, shape5(density_shape).elems(), s);
+  for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
+    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+        hipLaunchKernelGGL((raymarch_bwd_exact_kernel<4, FF, BB, MM, true>), grid, dim3(256), 0, s, a, cb, output,
+                           dimg, density_grad))));
+  });
+  return check_launch("ngt_render_density_l2_bwd");
 }
 
 

@@ -510,6 +510,70 @@ def render_density_fused(density, light_rgb, depth_step, matrix_m, matrix_v, mat
                                 _enum(BLEND, blending_mode, "blend mode"))
 
 
+class _RenderDensityL2(torch.autograd.Function):
+    """Fused render + L2 image loss: returns (loss, images); only the loss
+    carries a gradient."""
+
+    @staticmethod
+    def forward(ctx, density, target, light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight):
+        lib = _lib.load()
+        density = _lib.device_tensor(density, "density")
+        target = _lib.device_tensor(target, "target")
+        cams.check_batch(density.shape[0])
+        img_shape = (density.shape[0], cams.nv, int(out_shape[1]), int(out_shape[2]))
+        if tuple(target.shape) != img_shape + (3,):
+            raise ValueError("target must be [N,V,H,W,3] = %s, is %s" % (list(img_shape + (3,)), list(target.shape)))
+        out = torch.empty(img_shape + (4,), dtype=torch.float32, device=density.device)
+        loss = torch.empty((), dtype=torch.float32, device=density.device)
+        lp, _, keep = _lib.host_floats(light_rgb, 3)
+        s_d = _lib.shape5(density)
+        ws, nbytes = _lib.workspace(lib.ngt_render_density_l2_fwd_workspace_size(s_d, interp, cams.nv,
+                                                                                 _lib.dhw(out_shape)), density.device)
+        _lib.check(lib.ngt_render_density_l2_fwd(
+            _lib.ptr(density), s_d, lp, float(depth_step), cams.m, cams.v, cams.p, cams.f, cams.nv,
+            _lib.dhw(out_shape), interp, boundary, mode, _lib.ptr(target), float(weight), _lib.ptr(out),
+            _lib.ptr(loss), _lib.ptr(ws), nbytes, _lib.stream(density.device)))
+        ctx.save_for_backward(density, target, out)
+        ctx.args = (light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight)
+        ctx.mark_non_differentiable(out)
+        return loss, out
+
+    @staticmethod
+    def backward(ctx, grad_loss, grad_out):
+        density, target, out = ctx.saved_tensors
+        light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight = ctx.args
+        lib = _lib.load()
+        gl = _lib.device_tensor(grad_loss.reshape(()).to(torch.float32), "loss_grad").contiguous()
+        g = torch.empty_like(density)
+        lp, _, keep = _lib.host_floats(light_rgb, 3)
+        s_d = _lib.shape5(density)
+        if use_gather_adjoint:
+            nb = lib.ngt_render_density_l2_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv,
+                                                              _lib.dhw(out_shape), interp, boundary)
+        else:
+            nb = out.numel() * 4
+        ws, nbytes = _lib.workspace(nb, density.device)
+        _lib.check(lib.ngt_render_density_l2_bwd(
+            _lib.ptr(density), s_d, lp, float(depth_step), _lib.ptr(out), _lib.ptr(target), float(weight),
+            _lib.ptr(gl), cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape), interp, boundary, mode,
+            _lib.ptr(g), _lib.ptr(ws), nbytes, _lib.stream(density.device)))
+        return g, None, None, None, None, None, None, None, None, None
+
+
+def render_density_fused_l2(density, target, light_rgb, depth_step, matrix_m, matrix_v, matrix_p, frustum_params,
+                            output_shape, loss_weight=1.0, interpolation="LINEAR", boundary="BORDER",
+                            blending_mode="BEER_LAMBERT"):
+    """``render_density_fused`` followed by the squared-error image loss
+    ``loss_weight * sum((images[..., :3] - target)^2)`` (the density target
+    loss, optimization.py:478-526) in one op: returns (loss, images).  The
+    backward forms the image gradient inside the render adjoint."""
+    interp = _enum(INTERPOLATION, interpolation, "filter mode")
+    cams = _Cams(matrix_m, matrix_v, matrix_p, frustum_params)
+    return _RenderDensityL2.apply(density, target, tuple(float(c) for c in light_rgb), float(depth_step), cams,
+                                  tuple(int(v) for v in output_shape), interp, _enum(BOUNDARY, boundary, "boundary"),
+                                  _enum(BLEND, blending_mode, "blend mode"), float(loss_weight))
+
+
 # --- AdvectGridSemiLangrange -------------------------------------------------------
 
 def _adv_layout(velocity):

@@ -48,6 +48,14 @@ class RenderSpec:
         return ops.render_density_fused(rho, self.light_rgb, self.depth_step, self.M, self.V, self.P, self.F,
                                         self.output_dhw, boundary=self.boundary, blending_mode=self.blending_mode)
 
+    def l2_loss(self, rho, target):
+        """sum((render(rho)[..., :3] - target)^2), rendered and reduced in one
+        fused op whose adjoint forms the image gradient in-kernel."""
+        loss, _ = ops.render_density_fused_l2(rho, target, self.light_rgb, self.depth_step, self.M, self.V, self.P,
+                                              self.F, self.output_dhw, boundary=self.boundary,
+                                              blending_mode=self.blending_mode)
+        return loss
+
     @classmethod
     def from_cameras(cls, grid_transform, cameras, light_rgb=(1.0, 1.0, 1.0), boundary="BORDER"):
         from .renderer import Renderer
@@ -58,9 +66,7 @@ class RenderSpec:
 def _segment(rho, vel, steps, spec, target, dt, clamp, boundary):
     for _ in range(steps):
         rho, vel = transport_step(rho, vel, dt, clamp, boundary)
-    img = spec.render(rho)
-    loss = ((img[..., :3] - target) ** 2).sum()
-    return rho, vel, loss
+    return rho, vel, spec.l2_loss(rho, target)
 
 
 def chain_loss(rho0, vel0, spec, targets, render_every=5, dt=1.0, clamp="MC_SMOOTH", boundary="BORDER",

@@ -1,0 +1,111 @@
+"""GPU parity of the fused render + L2 image loss (render_density_fused_l2):
+the loss and images against the FP64 oracle and the unfused composition,
+and the density gradient (image gradient formed inside the render adjoint)
+against the oracle's render adjoint of 2 * w * g * (image - target).
+
+Tolerances: loss and images 1e-5, gradients 1e-4, normwise relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import BORDER, CLAMP, LINEAR
+from oracle import blend as B
+from oracle import raymarch as RM
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ops():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from ngt import ops as o, _lib
+    _lib.load()
+    return o
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float32))).to("cuda:0")
+
+
+def host(t):
+    return t.detach().cpu().double().numpy()
+
+
+def rel(got, exp):
+    return float(np.max(np.abs(got - exp))) / max(float(np.max(np.abs(exp))), 1e-30)
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def _scene(dims=(12, 11, 13), hw=(10, 14), depth=16, views=3):
+    from ngt import synth
+    from ngt.renderer import Renderer
+    grid, cams = synth.ring_setup(dims, hw, depth, views)
+    grid.set_translation([0.0123, -0.0217, 0.0311])
+    M, V, P, F = Renderer()._transform_params([grid], cams)
+    return [np.asarray(a, np.float32) for a in (M, V, P, F)], float(cams[0].depth_step), (depth,) + hw
+
+
+@pytest.mark.parametrize("bound,gather", [(BORDER, True), (BORDER, False), (CLAMP, True)])
+@pytest.mark.parametrize("mode", [B.BEER_LAMBERT, B.ADDITIVE])
+def test_fused_l2_loss_and_gradient(ops, bound, gather, mode):
+    rng = np.random.default_rng(61 + bound + 2 * gather + mode)
+    dims = (12, 11, 13)
+    (M, V, P, F), ds, out_dhw = _scene(dims)
+    rgb = np.array([0.9, 0.7, 0.5])
+    rho = f32(rng.uniform(0.0, 0.5, (1,) + dims + (1,)))
+    tgt = f32(rng.uniform(0.0, 0.3, (1, 3) + out_dhw[1:] + (3,)))
+    w = 0.37
+    bname = {BORDER: "BORDER", CLAMP: "CLAMP"}[bound]
+    mname = {B.BEER_LAMBERT: "BEER_LAMBERT", B.ADDITIVE: "ADDITIVE"}[mode]
+    old = ops.use_gather_adjoint
+    ops.use_gather_adjoint = gather
+    try:
+        rt = dev(rho).requires_grad_()
+        loss, img = ops.render_density_fused_l2(rt, dev(tgt), rgb, ds, M, V, P, F, out_dhw, loss_weight=w,
+                                                boundary=bname, blending_mode=mname)
+        (loss * 3.0).backward()
+    finally:
+        ops.use_gather_adjoint = old
+    cat = np.concatenate([rho * rgb, rho], -1) * ds
+    eimg = RM.raymarch_grid_transform(cat, M, V, P, F, out_dhw, LINEAR, bound, mode)
+    assert rel(host(img), eimg) < FWD_TOL
+    eloss = w * float(np.sum((eimg[..., :3] - tgt) ** 2))
+    assert abs(float(loss) - eloss) / abs(eloss) < FWD_TOL
+    gimg = np.zeros_like(eimg)
+    gimg[..., :3] = 2.0 * w * 3.0 * (eimg[..., :3] - tgt)
+    g_cat = RM.raymarch_grid_transform_grad(cat, eimg, gimg, M, V, P, F, out_dhw, LINEAR, bound, mode)
+    eg = ds * (g_cat[..., :3] @ rgb + g_cat[..., 3])[..., None]
+    assert rel(host(rt.grad), eg) < GRAD_TOL
+
+
+def test_fused_l2_matches_unfused_composition(ops):
+    rng = np.random.default_rng(62)
+    dims = (12, 11, 13)
+    (M, V, P, F), ds, out_dhw = _scene(dims)
+    rho = f32(rng.uniform(0.0, 0.5, (1,) + dims + (1,)))
+    tgt = dev(rng.uniform(0.0, 0.3, (1, 3) + out_dhw[1:] + (3,)))
+    r1, r2 = dev(rho).requires_grad_(), dev(rho).requires_grad_()
+    loss, img = ops.render_density_fused_l2(r1, tgt, (1, 1, 1), ds, M, V, P, F, out_dhw)
+    loss.backward()
+    img2 = ops.render_density_fused(r2, (1, 1, 1), ds, M, V, P, F, out_dhw)
+    loss2 = ((img2[..., :3] - tgt) ** 2).sum()
+    loss2.backward()
+    assert torch.equal(img, img2)
+    assert abs(float(loss) - float(loss2)) <= 1e-5 * abs(float(loss2))
+    assert rel(host(r1.grad), host(r2.grad)) < 1e-5
+
+
+def test_fused_l2_rejects_bad_target(ops):
+    (M, V, P, F), ds, out_dhw = _scene()
+    rho = torch.zeros((1, 12, 11, 13, 1), device="cuda:0")
+    with pytest.raises(ValueError, match="target must be"):
+        ops.render_density_fused_l2(rho, torch.zeros((1, 3) + out_dhw[1:] + (4,), device="cuda:0"), (1, 1, 1), ds,
+                                    M, V, P, F, out_dhw)
