@@ -18,6 +18,7 @@ SampleGridLut launches on an explicit 4-channel LUT plus elementwise torch
 ops - the per-op "plain" path.
 """
 import copy
+import logging
 
 import numpy as np
 import torch
@@ -32,26 +33,101 @@ def _default_warp_renderer(boundary="BORDER"):
                     sample_gradients=False)
 
 
+_LOG = logging.getLogger("data_structures")
+
 # resampling between resolutions: LINEAR, no mips (the reference's scale renderer, reconstruct_sequence.py:602-609)
 _default_scale_renderer = _default_warp_renderer
 
 
+def _as_tensor(a, device):
+    if a is None or isinstance(a, torch.Tensor):
+        return a
+    return torch.as_tensor(np.asarray(a, dtype=np.float32), device=device)
+
+
+def _to_numpy(t):
+    return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+
+
 class DensityGrid:
-    def __init__(self, shape, d=None, scale_renderer=None, hull=None, inflow=None, device=None, **kwargs):
+    """Density [1, D, H, W, 1] with optional hull and inflow.  The inflow is a
+    sub-grid placed at ``inflow_offset`` (z, y, x) and optionally masked;
+    ``with_inflow`` = max(d + padded inflow, 0) (data_structures.py:380-462)."""
+
+    def __init__(self, shape, d=None, scale_renderer=None, hull=None, inflow=None, inflow_offset=None,
+                 inflow_mask=None, device=None, **kwargs):
         self.shape = [int(v) for v in shape]
+        dev = device or (d.device if isinstance(d, torch.Tensor) else "cuda")
         if d is None:
-            d = torch.zeros([1] + self.shape + [1], dtype=torch.float32, device=device or "cuda")
-        self._d = d
-        self.hull = hull
-        self.inflow = inflow
+            d = torch.zeros([1] + self.shape + [1], dtype=torch.float32, device=dev)
+        self._d = _as_tensor(d, dev)
+        self.hull = _as_tensor(hull, dev)
+        self._inflow = _as_tensor(inflow, dev)
+        self.inflow_mask = _as_tensor(inflow_mask, dev) if inflow is not None else None
+        self.inflow_offset = None
+        if self._inflow is not None:
+            self.inflow_offset = [int(v) for v in (inflow_offset if inflow_offset is not None else (0, 0, 0))]
+            ishape = list(self._inflow.shape[-4:-1])
+            for a in range(3):
+                if self.inflow_offset[a] < 0 or self.inflow_offset[a] + ishape[a] > self.shape[a]:
+                    raise ValueError("inflow of shape %s at offset %s does not fit the grid %s"
+                                     % (ishape, self.inflow_offset, self.shape))
         self.scale_renderer = scale_renderer if scale_renderer is not None else _default_scale_renderer()
 
     @property
     def d(self):
         return self._d
 
+    @property
+    def inflow(self):
+        """The (masked) inflow padded to the grid shape (zeros without inflow)."""
+        if self._inflow is None:
+            return torch.zeros_like(self._d)
+        f = self._inflow * self.inflow_mask if self.inflow_mask is not None else self._inflow
+        ish = f.shape[-4:-1]
+        off = self.inflow_offset
+        pad = []
+        for a in (2, 1, 0):  # F.pad takes the last axes first: x, y, z
+            pad += [off[a], self.shape[a] - off[a] - ish[a]]
+        return torch.nn.functional.pad(f, [0, 0] + pad)
+
     def with_inflow(self):
-        return self._d if self.inflow is None else self._d + self.inflow
+        return self._d if self._inflow is None else torch.clamp(self._d + self.inflow, min=0.0)
+
+    @classmethod
+    def from_file(cls, path, scale_renderer=None, hull=None, inflow=None, inflow_offset=None, inflow_mask=None,
+                  device=None, **kwargs):
+        """Density npz (data_structures.py:465-485): 'arr_0' [1, D, H, W, 1] and
+        optionally 'hull', 'inflow', 'inflow_mask', 'inflow_offset'.  Read
+        without pickle; None (and a warning) if the file cannot be read."""
+        try:
+            with np.load(path, allow_pickle=False) as f:
+                d = f["arr_0"]
+                if "hull" in f and hull is None:
+                    hull = f["hull"]
+                if "inflow" in f and inflow is None:
+                    inflow = f["inflow"]
+                    if "inflow_mask" in f and inflow_mask is None:
+                        inflow_mask = f["inflow_mask"]
+                    if "inflow_offset" in f and inflow_offset is None:
+                        inflow_offset = f["inflow_offset"].tolist()
+                return cls(list(d.shape[-4:-1]), d=d, scale_renderer=scale_renderer, hull=hull, inflow=inflow,
+                           inflow_offset=inflow_offset, inflow_mask=inflow_mask, device=device)
+        except Exception:
+            _LOG.warning("Failed to load density from '%s':", path, exc_info=True)
+            return None
+
+    def save(self, path):
+        """npz written as the reference writes it (data_structures.py:704-726)."""
+        extra = {}
+        if self.hull is not None:
+            extra["hull"] = _to_numpy(self.hull)
+        if self._inflow is not None:
+            extra["inflow"] = _to_numpy(self._inflow)
+            if self.inflow_mask is not None:
+                extra["inflow_mask"] = _to_numpy(self.inflow_mask)
+            extra["inflow_offset"] = np.asarray(self.inflow_offset)
+        np.savez_compressed(path, _to_numpy(self._d), **extra)
 
     def assign(self, d):
         self._d = d
@@ -96,9 +172,9 @@ class VelocityGrid:
         self.centered_shape = [int(v) for v in centered_shape]
         self.x_shape, self.y_shape, self.z_shape = self.component_shapes(self.centered_shape)
         dev = device or "cuda"
-        self.x = x if x is not None else torch.randn([1] + self.x_shape + [1], device=dev) * std
-        self.y = y if y is not None else torch.randn([1] + self.y_shape + [1], device=dev) * std
-        self.z = z if z is not None else torch.randn([1] + self.z_shape + [1], device=dev) * std
+        self.x = _as_tensor(x, dev) if x is not None else torch.randn([1] + self.x_shape + [1], device=dev) * std
+        self.y = _as_tensor(y, dev) if y is not None else torch.randn([1] + self.y_shape + [1], device=dev) * std
+        self.z = _as_tensor(z, dev) if z is not None else torch.randn([1] + self.z_shape + [1], device=dev) * std
         self.boundary = boundary
         self.scale_renderer = scale_renderer if scale_renderer is not None else _default_scale_renderer()
         self.warp_renderer = warp_renderer if warp_renderer is not None else _default_warp_renderer()
@@ -219,6 +295,33 @@ class VelocityGrid:
 
     def assign(self, x, y, z):
         self.x, self.y, self.z = x, y, z
+
+    @classmethod
+    def from_file(cls, path, boundary=None, scale_renderer=None, warp_renderer=None, device=None, **kwargs):
+        """Velocity npz (data_structures.py:909-925): 'vel_x', 'vel_y',
+        'vel_z' MAC components and 'centered_shape' (derived from vel_x in
+        legacy files).  Read without pickle; None (and a warning) if the file
+        cannot be read."""
+        try:
+            with np.load(path, allow_pickle=False) as v:
+                if "centered_shape" in v:
+                    shape = [int(s) for s in v["centered_shape"].tolist()]
+                else:
+                    shape = list(v["vel_x"].shape)
+                    shape[-2] -= 1
+                    shape = shape[1:-1]
+                dev = device or "cuda"
+                comps = [_as_tensor(v[k].astype(np.float32), dev) for k in ("vel_x", "vel_y", "vel_z")]
+            return cls(shape, x=comps[0], y=comps[1], z=comps[2], boundary=boundary, scale_renderer=scale_renderer,
+                       warp_renderer=warp_renderer, device=device)
+        except Exception:
+            _LOG.warning("Failed to load velocity from '%s':", path, exc_info=True)
+            return None
+
+    def save(self, path):
+        """npz written as the reference writes it (data_structures.py:1197-1198)."""
+        np.savez_compressed(path, centered_shape=self.centered_shape, vel_x=_to_numpy(self.x), vel_y=_to_numpy(self.y),
+                            vel_z=_to_numpy(self.z))
 
     def centered(self, pad_lod=False, concat=True):
         h = 0.5
