@@ -168,7 +168,7 @@ def build_problem(torch, dev):
 def fused_step(torch, ops, prob, leaves):
     rho, vx, vy, vz = leaves
     adv = ops.advect_grid_semi_langrange(rho, (vx, vy, vz), timestep=-1.0, order=2, clamp_extrema="MC_SMOOTH",
-                                         boundary="BORDER", separate_velocity_batch=False)[:, 0]
+                                         boundary="BORDER", separate_velocity_batch=False).squeeze(1)
     loss, _ = ops.render_density_fused_l2(adv, prob["target"], prob["light"], prob["ds"], prob["M"], prob["V"],
                                           prob["P"], prob["F"], (DEPTH,) + IMAGE_HW)
     loss.backward()
@@ -298,11 +298,11 @@ def op_calls(torch, prob):
     return calls
 
 
-def cpu_baseline():
+def _oracle_fp64_estimate():
     """FP64 NumPy oracle (single thread) on a bounded sample of the step: the
     MacCormack step fwd+bwd at 48^3 and the fused render fwd+bwd of one view at
-    64x36 pixels (128 slices) on the 128^3 grid; scaled per voxel-update and
-    per ray-sample to the full step."""
+    64x36 pixels (128 slices) on the 128^3 grid; seconds per full step, scaled
+    per voxel-update and per ray-sample."""
     from oracle import advect as A, raymarch as RM, geometry as G, BORDER, LINEAR
     from oracle import blend as B
     from ngt import synth
@@ -324,12 +324,59 @@ def cpu_baseline():
     img = RM.raymarch_grid_transform(cat, [M], [V], [P], [F], (DEPTH, 36, 64), LINEAR, BORDER, B.BEER_LAMBERT)
     RM.raymarch_grid_transform_grad(cat, img, img, [M], [V], [P], [F], (DEPTH, 36, 64), LINEAR, BORDER, B.BEER_LAMBERT)
     t_ren = (time.perf_counter() - t0) / (36 * 64 * DEPTH)
-    t_step = t_adv * np.prod(GRID) + t_ren * VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * DEPTH
-    return {"value": float(np.prod(GRID) / t_step), "unit": "voxel-updates/s", "cores": 1, "kind": "port",
-            "sample": "FP64 NumPy oracle, 1 thread: MacCormack+MC_SMOOTH fwd+bwd at 48^3 and 1-view 64x36x128 "
-                      "fused render fwd+bwd on the 128^3 grid, per-voxel / per-ray-sample times scaled to the "
-                      "full 128^3 + 5x256x144x128 step",
-            "seconds_per_step_est": float(t_step)}
+    return float(t_adv * np.prod(GRID) + t_ren * VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * DEPTH)
+
+
+def cpu_baseline(reps=5):
+    """The FP32 OpenMP CPU restatement (oracle/cpu_port.cpp) timed on this
+    host's cores on the full workload: the same 128^3 MacCormack step and
+    5-view 256x144x128 render + L2 loss, forward and backward, ``reps`` timed
+    steps after one warm-up (median).  The FP64 NumPy oracle's single-thread
+    estimate is reported beside it."""
+    from oracle import cpu_port
+    from ngt import synth
+    rho = synth.blobs(GRID, 16, seed=2)[..., 0]
+    vx, vy, vz = (a[..., 0] for a in synth.curl_noise_staggered(GRID, seed=4, vmax=2.0))
+    grid_t, cams = synth.ring_setup(GRID, IMAGE_HW, DEPTH, VIEWS)
+    from ngt.renderer import Renderer
+    M, V, P, F = Renderer()._transform_params([grid_t], cams)
+    ds = float(cams[0].depth_step)
+    light = np.ones(3, np.float32)
+    target = cpu_port.render_fwd(synth.blobs(GRID, 16, seed=3)[..., 0], M[0], V, P, (DEPTH,) + IMAGE_HW, light,
+                                 ds)[..., :3].copy()
+
+    def step():
+        adv, sl = cpu_port.advect_mc_fwd(rho, vx, vy, vz, -1.0)
+        img = cpu_port.render_fwd(adv, M[0], V, P, (DEPTH,) + IMAGE_HW, light, ds)
+        cpu_port.l2_loss(img, target)
+        g_adv = cpu_port.render_l2_bwd(adv, M[0], V, P, (DEPTH,) + IMAGE_HW, light, ds, img, target)
+        cpu_port.advect_mc_bwd(rho, vx, vy, vz, -1.0, sl, g_adv)
+
+    step()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        step()
+        times.append(time.perf_counter() - t0)
+    t_step = float(np.median(times))
+    return {"value": float(np.prod(GRID) / t_step), "unit": "voxel-updates/s", "cores": cpu_port.threads(),
+            "kind": "port",
+            "sample": "FP32 OpenMP C++ restatement (oracle/cpu_port.cpp, g++ -O3 -march=x86-64-v3 -fopenmp) of the "
+                      "full step: 128^3 MacCormack+MC_SMOOTH and 5-view 256x144x128 render + L2, fwd+bwd; median "
+                      "of %d steps after 1 warm-up" % reps,
+            "seconds_per_step": t_step, "cpu_model": _cpu_model(),
+            "oracle_fp64_1thread_seconds_per_step_est": _oracle_fp64_estimate()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def chain_bench(steps, warmup):
@@ -447,7 +494,7 @@ def batch_bench(steps, warmup):
             M, V, P, F, ds, out_dhw = cams[i]
             d = dens[i][None].requires_grad_()
             v = tuple(c[None].requires_grad_() for c in vels[i])
-            adv = ops.advect_grid_semi_langrange(d, v, -1.0, 2, "MC_SMOOTH", "BORDER", False)[:, 0]
+            adv = ops.advect_grid_semi_langrange(d, v, -1.0, 2, "MC_SMOOTH", "BORDER", False).squeeze(1)
             img = ops.render_density_fused(adv, (1.0, 1.0, 1.0), ds, M, V, P, F, out_dhw)[0]
             ((img[..., :3] - tgts[i]) ** 2).sum().backward()
 
