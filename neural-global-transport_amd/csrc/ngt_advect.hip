@@ -939,8 +939,10 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
         hipLaunchKernelGGL((advect_sl_kernel<CC, BB, false>), grid, block, 0, s, p.a, input, sl_buf)));
     slp = sl_buf;
   }
-  fill_zero(input_grad, p.a.n_data * p.cells * p.C, s);
-  if (!ref && p.C <= 2) {
+  const bool brick_path = !ref && p.C <= 2;
+  // the brick path's order-2 overflow buffer is cleared in the same launch
+  if (!(brick_path && order == 2)) fill_zero(input_grad, p.a.n_data * p.cells * p.C, s);
+  if (brick_path) {
     // brick-tiled exact adjoint: workspace after g_vc = [tiles2 | r_dir | ovf1 | tiles1]
     const BrickGrid bg = brick_grid(p.a.d);
     const int64_t tiles_n = static_cast<int64_t>(p.pairs) * bg.count() * kTileCells * p.C;
@@ -954,7 +956,7 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
     const dim3 bgrid(static_cast<unsigned>(bg.count() * p.pairs)), bblock(kBrickX, kBrickY);
     const bool neg = order == 1 && clamp == NGT_CLAMP_NEGATIVE;
     if (order == 2) {
-      fill_zero(ovf1, field, s);
+      fill_zero2(input_grad, p.a.n_data * p.cells * p.C, ovf1, field, s);
       NGT_SWITCH_C12(p.C, NGT_SWITCH_BM(boundary, NGT_SWITCH_CLAMP(clamp,
           hipLaunchKernelGGL((advect_mc_bwd_brick_kernel<CC, BB, KK>), bgrid, bblock, 0, s, p.a, bg, input, slp,
                              output_grad, g_sl, r_dir, ovf1, tiles1, g_vc, vel_mode);

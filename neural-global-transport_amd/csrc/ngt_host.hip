@@ -155,6 +155,21 @@ void fill_zero(float* p, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(fill_kernel, dim3(b), dim3(256), 0, s, p, n, 0.f);
 }
 
+// Two ranges in one launch: grid.y selects the range.
+__global__ void fill2_kernel(float* __restrict__ a, int64_t na, float* __restrict__ b, int64_t nb) {
+  float* p = blockIdx.y == 0 ? a : b;
+  const int64_t n = blockIdx.y == 0 ? na : nb;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.f;
+}
+
+void fill_zero2(float* a, int64_t na, float* b, int64_t nb, hipStream_t s) {
+  const int64_t n = std::max(na, nb);
+  if (n <= 0) return;
+  const unsigned g = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(fill2_kernel, dim3(g, 2), dim3(256), 0, s, a, na, b, nb);
+}
+
 void fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   const unsigned b = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 4096));

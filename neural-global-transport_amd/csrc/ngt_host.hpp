@@ -67,6 +67,7 @@ CamParams make_cam(const float* m16, const float* v16, const float* p16, const f
 
 // Zero-fill with a kernel (not hipMemsetAsync, so graph capture stays clean).
 void fill_zero(float* p, int64_t n, hipStream_t s);
+void fill_zero2(float* a, int64_t na, float* b, int64_t nb, hipStream_t s);
 void fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s);
 
 inline unsigned blocks_for(int64_t n, int threads) { return static_cast<unsigned>((n + threads - 1) / threads); }

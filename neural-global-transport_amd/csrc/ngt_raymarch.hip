@@ -523,9 +523,10 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   // box (slab test with a one-sample margin) and counts its samples there;
   // after a block-wide scan the chunk's samples are dealt out one per
   // work-item in sub-chunks of kItems.  A sub-chunk is counting-sorted in LDS
-  // by the sample's floor cell (bins = floor cells b0-1 .. b1 per axis; two
-  // integer LDS atomics per sample), and then every work-item sums, for each
-  // voxel it owns, the samples of the 8 bins whose trilinear footprint holds
+  // by the sample's floor cell (bins = floor cells b0-1 .. b1 per axis; one
+  // integer LDS atomic per sample gives its rank in the bin), and then every
+  // work-item sums, for each voxel it owns, the samples of the 8 bins whose
+  // trilinear footprint holds
   // that voxel, in registers: no floating-point atomics.  (A corner at
   // floor+1 where the reference uses ceil = floor at integer positions carries
   // weight exactly 0.)
@@ -618,11 +619,28 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
                         (ly <= b1y - fb0m1y) & (lz >= 0) & (lz <= b1z - fb0m1z);
         return ok ? (lz * (BY + 1) + ly) * (BX + 1) + lx : -1;
       };
-      for (int i = i0 + tid; i < i_end; i += 256) {
-        float wx, wy, wz;
-        int rec, k;
-        const int bin = locate(i, wx, wy, wz, rec, k);
-        if (bin >= 0) atomicAdd(&bin_cnt[bin], 1);
+      // each work-item locates its (at most kPer) samples once: bin, rank in
+      // the bin (from the counting atomic), weights and gradient stay in
+      // registers until they are placed
+      constexpr int kPer = kItems / 256;
+      int ibin[kPer], irank[kPer];
+      float iwx[kPer], iwy[kPer], iwz[kPer];
+      Vec<CG> ig[kPer];
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        const int i = i0 + tid + 256 * j;
+        ibin[j] = -1;
+        irank[j] = 0;
+        if (i < i_end) {
+          int rec, k;
+          const int bin = locate(i, iwx[j], iwy[j], iwz[j], rec, k);
+          if (bin >= 0) {
+            ig[j] = vload<CG>(G + (static_cast<int64_t>(k) * npix + (rec % npix) +
+                                   static_cast<int64_t>(rec / npix) * kcap * npix) * CG);
+            irank[j] = atomicAdd(&bin_cnt[bin], 1);
+            ibin[j] = bin;
+          }
+        }
       }
       __syncthreads();
       // exclusive scan of the bins: 3 per work-item, then across work-items
@@ -641,27 +659,20 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
           const int b = tid * kPer + j;
-          if (b < NB) {
-            bin_start[b] = run;
-            bin_cnt[b] = 0;  // becomes the placement cursor
-          }
+          if (b < NB) bin_start[b] = run;
           run += local[j];
         }
         if (tid == 255) bin_start[NB] = incl;
       }
       __syncthreads();
-      for (int i = i0 + tid; i < i_end; i += 256) {
-        float wx, wy, wz;
-        int rec, k;
-        const int bin = locate(i, wx, wy, wz, rec, k);
-        if (bin < 0) continue;
-        const int pos = bin_start[bin] + atomicAdd(&bin_cnt[bin], 1);
-        const Vec<CG> gs = vload<CG>(G + (static_cast<int64_t>(k) * npix + (rec % npix) +
-                                          static_cast<int64_t>(rec / npix) * kcap * npix) * CG);
-        srec[pos] = make_float4(wx, wy, wz, gs.v[0]);
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        if (ibin[j] < 0) continue;
+        const int pos = bin_start[ibin[j]] + irank[j];
+        srec[pos] = make_float4(iwx[j], iwy[j], iwz[j], ig[j].v[0]);
         if constexpr (CG > 1) {
 #pragma unroll
-          for (int ch = 0; ch < CG; ++ch) sg[pos * CG + ch] = gs.v[ch];
+          for (int ch = 0; ch < CG; ++ch) sg[pos * CG + ch] = ig[j].v[ch];
         }
       }
       __syncthreads();

@@ -188,6 +188,13 @@ def device_tensor(t, name):
     return t.contiguous()
 
 
+def empty_grad(t):
+    """Uninitialised float32 tensor of ``t``'s shape with standard contiguous
+    strides.  (``empty_like`` copies zero strides of size-1 dimensions, and
+    autograd clones such a gradient instead of taking it as the leaf's .grad.)"""
+    return torch.empty(t.shape, dtype=torch.float32, device=t.device)
+
+
 def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 

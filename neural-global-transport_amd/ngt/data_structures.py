@@ -356,7 +356,7 @@ class VelocityGrid:
                                                  clamp_extrema=clamp if clamp in ("MC_SMOOTH", "NEGATIVE") else "NONE",
                                                  boundary=self.warp_renderer.boundary_mode,
                                                  separate_velocity_batch=False)
-            return out[:, 0]
+            return out.squeeze(1)  # a view: its backward launches nothing
         return self._warp_plain(data, order, dt, clamp, centered_velocity)
 
     def _warp_plain(self, data, order, dt, clamp, centered_velocity=None):
@@ -413,7 +413,7 @@ class VelocityGrid:
                                                  clamp_extrema=clamp if clamp in ("MC_SMOOTH", "NEGATIVE") else "NONE",
                                                  boundary=self.warp_renderer.boundary_mode,
                                                  separate_velocity_batch=False)
-            return out[:, 0]
+            return out.squeeze(1)  # a view: its backward launches nothing
         r = self.warp_renderer
         warped = r._sample_LuT(data, lut, True, relative=True)
         if order == 2:

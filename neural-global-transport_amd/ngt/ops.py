@@ -117,8 +117,8 @@ def _lut_bwd(inp, out_grad, lut, a, sem):
     out_grad = _lib.device_tensor(out_grad, "output_grad")
     if out_grad.dim() != 6:
         raise ValueError("Invalid output_grads shape (NVDHWC): %s" % (list(out_grad.shape),))
-    in_grad = torch.empty_like(inp)
-    lut_grad = torch.empty_like(lut)
+    in_grad = _lib.empty_grad(inp)
+    lut_grad = _lib.empty_grad(lut)
     s_in, s_lut = _lib.shape5(inp), _lib.shape5(lut)
     ws, nbytes = _lib.workspace(lib.ngt_sample_grid_lut_bwd_workspace_size(s_in, s_lut, a["mipmapping"],
                                                                            a["num_mipmaps"], sem), inp.device)
@@ -216,7 +216,7 @@ def _xform_bwd(inp, out_grad, cams, a, sem):
     inp = _lib.device_tensor(inp, "input")
     out_grad = _lib.device_tensor(out_grad, "output_grad")
     cams.check_batch(inp.shape[0])
-    in_grad = torch.empty_like(inp)
+    in_grad = _lib.empty_grad(inp)
     s_in = _lib.shape5(inp)
     ws, nbytes = _lib.workspace(lib.ngt_sample_grid_transform_bwd_workspace_size(s_in, a["mipmapping"],
                                                                                  a["num_mipmaps"], sem), inp.device)
@@ -304,7 +304,7 @@ def _blend_bwd(out_grad, out, inp, mode, keep_dims, sem):
     inp = _lib.device_tensor(inp, "input")
     out = _lib.device_tensor(out, "output")
     out_grad = _lib.device_tensor(out_grad, "output_grad")
-    in_grad = torch.empty_like(inp)
+    in_grad = _lib.empty_grad(inp)
     _lib.check(lib.ngt_reduce_grid_blend_bwd(_lib.ptr(out_grad), _lib.ptr(out), _lib.ptr(inp), _lib.shape5(inp), mode,
                                              int(keep_dims), sem, _lib.ptr(in_grad), _lib.stream(inp.device)))
     return in_grad
@@ -373,7 +373,7 @@ def _ray_bwd(inp, out, out_grad, cams, out_shape, a, sem):
     out = _lib.device_tensor(out, "output").reshape(out_shape5)
     out_grad = _lib.device_tensor(out_grad, "output_grad").reshape(out_shape5)
     cams.check_batch(inp.shape[0])
-    in_grad = torch.empty_like(inp)
+    in_grad = _lib.empty_grad(inp)
     s_in = _lib.shape5(inp)
     nb = lib.ngt_raymarch_grid_transform_bwd_workspace_size(
         s_in, cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape), a["interpolation"], a["boundary"],
@@ -470,7 +470,7 @@ def _render_bwd(density, light_rgb, depth_step, out, out_grad, cams, out_shape, 
     density = _lib.device_tensor(density, "density")
     out = _lib.device_tensor(out, "output")
     out_grad = _lib.device_tensor(out_grad, "output_grad")
-    g = torch.empty_like(density)
+    g = _lib.empty_grad(density)
     lp, _, keep = _lib.host_floats(light_rgb, 3)
     s_d = _lib.shape5(density)
     nb = lib.ngt_render_density_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv,
@@ -536,6 +536,7 @@ class _RenderDensityL2(torch.autograd.Function):
         ctx.save_for_backward(density, target, out)
         ctx.args = (light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight)
         ctx.mark_non_differentiable(out)
+        ctx.set_materialize_grads(False)  # no zero-filled gradient for the images
         return loss, out
 
     @staticmethod
@@ -543,8 +544,10 @@ class _RenderDensityL2(torch.autograd.Function):
         density, target, out = ctx.saved_tensors
         light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight = ctx.args
         lib = _lib.load()
+        if grad_loss is None:
+            return None, None, None, None, None, None, None, None, None, None
         gl = _lib.device_tensor(grad_loss.reshape(()).to(torch.float32), "loss_grad").contiguous()
-        g = torch.empty_like(density)
+        g = _lib.empty_grad(density)
         lp, _, keep = _lib.host_floats(light_rgb, 3)
         s_d = _lib.shape5(density)
         if use_gather_adjoint:
@@ -624,10 +627,10 @@ def _adv_bwd(inp, vel, g, sl, ts, order, clamp, boundary, separate, sem):
     vx, vy, vz, V = _adv_layout(vel)
     g = _lib.device_tensor(g, "output_grad")
     s_in = _lib.shape5(inp)
-    gi = torch.empty_like(inp)
-    gvx = torch.empty_like(vx)
-    gvy = torch.empty_like(vy) if vy is not None else None
-    gvz = torch.empty_like(vz) if vz is not None else None
+    gi = _lib.empty_grad(inp)
+    gvx = _lib.empty_grad(vx)
+    gvy = _lib.empty_grad(vy) if vy is not None else None
+    gvz = _lib.empty_grad(vz) if vz is not None else None
     ws, nbytes = _lib.workspace(lib.ngt_advect_workspace_size(s_in, V, int(separate), order, 1), inp.device)
     _lib.check(lib.ngt_advect_bwd(
         _lib.ptr(inp), s_in, _lib.ptr(vx), _lib.ptr(vy), _lib.ptr(vz), V, float(ts), order, clamp, boundary,

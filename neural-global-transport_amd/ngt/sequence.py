@@ -25,8 +25,8 @@ def transport_step(rho, vel, dt=1.0, clamp="MC_SMOOTH", boundary="BORDER"):
     """One MacCormack step of the density [1,D,H,W,1] and the centered
     velocity [1,D,H,W,3] (both advected by the velocity at the start of the
     step)."""
-    rho1 = ops.advect_grid_semi_langrange(rho, vel, -dt, 2, clamp, boundary, False)[:, 0]
-    vel1 = ops.advect_grid_semi_langrange(vel, vel, -dt, 2, clamp, boundary, False)[:, 0]
+    rho1 = ops.advect_grid_semi_langrange(rho, vel, -dt, 2, clamp, boundary, False).squeeze(1)
+    vel1 = ops.advect_grid_semi_langrange(vel, vel, -dt, 2, clamp, boundary, False).squeeze(1)
     return rho1, vel1
 
 
