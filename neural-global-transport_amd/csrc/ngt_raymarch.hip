@@ -540,7 +540,8 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   __shared__ float4 srec[kItems];
   __shared__ float sg[CG > 1 ? kItems * CG : 1];
   __shared__ int s_wtot[4];
-  // owned voxels: local index tid and tid + 256 (NV == 512)
+  __shared__ unsigned char s_ray[kItems];
+  // owned voxels: the x-pair of work-item tid (NV == 512)
   float accv[2][CG];
 #pragma unroll
   for (int o = 0; o < 2; ++o)
@@ -593,14 +594,14 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
     for (int i0 = 0; i0 < items; i0 += kItems) {
       const int i_end = min(i0 + kItems, items);
       for (int b = tid; b < NB; b += 256) bin_cnt[b] = 0;
+      {  // the ray of every sample of this sub-chunk: each ray marks its own range
+        const int r0 = max(s_off[tid], i0), r1 = min(s_off[tid + 1], i_end);
+        for (int i = r0; i < r1; ++i) s_ray[i - i0] = static_cast<unsigned char>(tid);
+      }
       __syncthreads();
       // sample i -> (bin, weights, gradient); bin < 0 when it does not touch the brick
       auto locate = [&](int i, float& wx, float& wy, float& wz, int& rec, int& k) -> int {
-        int lo_j = 0, hi_j = 256;
-        while (hi_j - lo_j > 1) {
-          const int mid = (lo_j + hi_j) >> 1;
-          if (s_off[mid] <= i) lo_j = mid; else hi_j = mid;
-        }
+        const int lo_j = s_ray[i - i0];
         k = s_k0[lo_j] + (i - s_off[lo_j]);
         rec = s_rec[lo_j];
         const RayRec rr = rays[rec];
@@ -676,35 +677,55 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
         }
       }
       __syncthreads();
-      // owners: voxel v is corner d of the samples whose floor cell is v - d
+      // owners: work-item tid holds the x-pair (ox, ox + 1) of row (oy, oz);
+      // voxel v is corner d of the samples whose floor cell is v - d, so per
+      // (dy, dz) the pair reads the three consecutive bins ox .. ox + 2 (bin x
+      // index = floor x - b0x + 1): bin ox feeds voxel ox as its ceil corner,
+      // bin ox + 1 feeds ox as floor and ox + 1 as ceil corner, bin ox + 2
+      // feeds ox + 1 as floor corner
+      {
+        const int ox = 2 * (tid % (BX / 2)), oy = (tid / (BX / 2)) % BY, oz = tid / (BX / 2 * BY);
 #pragma unroll
-      for (int o = 0; o < 2; ++o) {
-        const int v = tid + o * 256;
-        const int vx = v % BX, vy = (v / BX) % BY, vz = v / (BX * BY);
-#pragma unroll
-        for (int d = 0; d < 8; ++d) {
-          const int dx = d & 1, dy = (d >> 1) & 1, dz = (d >> 2) & 1;
-          const int bin = ((vz + 1 - dz) * (BY + 1) + (vy + 1 - dy)) * (BX + 1) + (vx + 1 - dx);
-          const int e = bin_start[bin + 1];
-          for (int q = bin_start[bin]; q < e; ++q) {
+        for (int dyz = 0; dyz < 4; ++dyz) {
+          const int dy = dyz & 1, dz = dyz >> 1;
+          const int row = ((oz + 1 - dz) * (BY + 1) + (oy + 1 - dy)) * (BX + 1) + ox;
+          const int s0 = bin_start[row], s1 = bin_start[row + 1], s2 = bin_start[row + 2], s3 = bin_start[row + 3];
+          // sample gradient times its y / z weight for this (dy, dz)
+          auto tyz = [&](int q, const float4& rq, int ch) {
+            const float g = CG == 1 ? rq.w : sg[q * CG + ch];
+            return g * ((dy ? rq.y : 1.f - rq.y) * (dz ? rq.z : 1.f - rq.z));
+          };
+          for (int q = s0; q < s1; ++q) {  // voxel ox as ceil corner
             const float4 rq = srec[q];
-            const float w = (dx ? rq.x : 1.f - rq.x) * (dy ? rq.y : 1.f - rq.y) * (dz ? rq.z : 1.f - rq.z);
-            if constexpr (CG == 1) {
-              accv[o][0] += rq.w * w;
-            } else {
 #pragma unroll
-              for (int ch = 0; ch < CG; ++ch) accv[o][ch] += sg[q * CG + ch] * w;
+            for (int ch = 0; ch < CG; ++ch) accv[0][ch] += tyz(q, rq, ch) * rq.x;
+          }
+          for (int q = s1; q < s2; ++q) {  // ox as floor, ox + 1 as ceil corner
+            const float4 rq = srec[q];
+#pragma unroll
+            for (int ch = 0; ch < CG; ++ch) {
+              const float t = tyz(q, rq, ch);
+              accv[0][ch] += t * (1.f - rq.x);
+              accv[1][ch] += t * rq.x;
             }
+          }
+          for (int q = s2; q < s3; ++q) {  // ox + 1 as floor corner
+            const float4 rq = srec[q];
+#pragma unroll
+            for (int ch = 0; ch < CG; ++ch) accv[1][ch] += tyz(q, rq, ch) * (1.f - rq.x);
           }
         }
       }
       __syncthreads();  // bins and records are rewritten by the next sub-chunk
     }
   }
+  {
+    const int ox = 2 * (tid % (BX / 2)), oy = (tid / (BX / 2)) % BY, oz = tid / (BX / 2 * BY);
 #pragma unroll
-  for (int o = 0; o < 2; ++o)
+    for (int o = 0; o < 2; ++o)
 #pragma unroll
-    for (int ch = 0; ch < CG; ++ch) acc[(tid + o * 256) * CG + ch] = accv[o][ch];
+      for (int ch = 0; ch < CG; ++ch) acc[((oz * BY + oy) * BX + ox + o) * CG + ch] = accv[o][ch];
+  }
   __syncthreads();
   const int64_t base = static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z;
   for (int i = tid; i < NV; i += blockDim.x) {
