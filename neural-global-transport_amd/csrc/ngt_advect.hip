@@ -129,8 +129,8 @@ NGT_DEVICE void advect_mc_kernel_body(const VBlock& vb, const AdvArgs& a, const 
 #pragma unroll
   for (int i = 0; i < C; ++i) r.v[i] = fwd.v[i] + 0.5f * (cur.v[i] - back.v[i]);
   if constexpr (CLAMPM == NGT_CLAMP_MC_SMOOTH) {
-    const Vec<C> mn = read_minmax<C, BM, false>(src, a.d, fx + tx, fy + ty, fz + tz);
-    const Vec<C> mx = read_minmax<C, BM, true>(src, a.d, fx + tx, fy + ty, fz + tz);
+    Vec<C> mn, mx;
+    read_minmax2<C, BM>(src, a.d, fx + tx, fy + ty, fz + tz, mn, mx);
 #pragma unroll
     for (int i = 0; i < C; ++i)
       if (r.v[i] < mn.v[i] || r.v[i] > mx.v[i]) r.v[i] = fwd.v[i];
@@ -172,15 +172,16 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_kernel(AdvArgs a, const flo
   const float bx = fx + (-tx), by = fy + (-ty), bz = fz + (-tz);
   Vec<C> g = vload<C>(g_out + (t.pair * cells + cell) * C);
   const Vec<C> fwd = vload<C>(slp + cell * C);
-  const Vec<C> back = read_linear<C, BM>(slp, a.d, bx, by, bz);
+  Vec<C> dx, dy, dz;
+  const Vec<C> back = read_linear_gradient<C, BM>(slp, a.d, bx, by, bz, dx, dy, dz);
   const Vec<C> cur = vload<C>(src + cell * C);
   Vec<C> corr;
 #pragma unroll
   for (int i = 0; i < C; ++i) corr.v[i] = fwd.v[i] + 0.5f * (cur.v[i] - back.v[i]);
   Vec<C> keep = vfill<C>(1.f);  // 1 where the corrected value is kept
   if constexpr (CLAMPM == NGT_CLAMP_MC_SMOOTH) {
-    const Vec<C> mn = read_minmax<C, BM, false>(src, a.d, fx + tx, fy + ty, fz + tz);
-    const Vec<C> mx = read_minmax<C, BM, true>(src, a.d, fx + tx, fy + ty, fz + tz);
+    Vec<C> mn, mx;
+    read_minmax2<C, BM>(src, a.d, fx + tx, fy + ty, fz + tz, mn, mx);
 #pragma unroll
     for (int i = 0; i < C; ++i)
       if (corr.v[i] < mn.v[i] || corr.v[i] > mx.v[i]) keep.v[i] = 0.f;
@@ -203,8 +204,6 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_kernel(AdvArgs a, const flo
   } else {
     scatter_linear<C, BM, false>(gsl, a.d, bx, by, bz, gback);
   }
-  Vec<C> dx, dy, dz;
-  read_gradient<C, BM>(slp, a.d, bx, by, bz, dx, dy, dz);
   float* gv = g_vc + (static_cast<int64_t>(v) * cells + cell) * 3;
   atomic_add(gv + 0, -a.ts * vdot(gback, dx));
   atomic_add(gv + 1, -a.ts * vdot(gback, dy));
@@ -340,6 +339,10 @@ __global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, i
 // a fixed order: no global atomics in the common case and a deterministic
 // sum.
 constexpr int kBrickX = 32, kBrickY = 8, kBrickZ = 8, kHalo = 3;
+// a work-group is kBrickX x kBrickY x kBrickZT work-items; each walks
+// kBrickZ / kBrickZT cells of its (x, y) column
+constexpr int kBrickZT = 2, kBrickZPer = kBrickZ / kBrickZT;
+constexpr int kBrickThreads = kBrickX * kBrickY * kBrickZT;
 constexpr int kTileX = kBrickX + 2 * kHalo, kTileY = kBrickY + 2 * kHalo, kTileZ = kBrickZ + 2 * kHalo;
 constexpr int kTileCells = kTileX * kTileY * kTileZ;
 
@@ -434,17 +437,17 @@ NGT_DEVICE BrickCell brick_cell(const BrickGrid& bg, const VBlock& vb) {
 
 template <int C>
 NGT_DEVICE void tile_clear(float* tile) {
-  const int tid = threadIdx.y * kBrickX + threadIdx.x;
-  for (int i = tid; i < kTileCells * C; i += kBrickX * kBrickY) tile[i] = 0.f;
+  const int tid = (threadIdx.z * kBrickY + threadIdx.y) * kBrickX + threadIdx.x;
+  for (int i = tid; i < kTileCells * C; i += kBrickThreads) tile[i] = 0.f;
   __syncthreads();
 }
 
 template <int C>
 NGT_DEVICE void tile_store(const float* tile, float* __restrict__ tiles, const BrickGrid& bg, const BrickCell& b) {
   __syncthreads();
-  const int tid = threadIdx.y * kBrickX + threadIdx.x;
+  const int tid = (threadIdx.z * kBrickY + threadIdx.y) * kBrickX + threadIdx.x;
   float* dst = tiles + (static_cast<int64_t>(b.pair) * bg.count() + b.brick) * kTileCells * C;
-  for (int i = tid; i < kTileCells * C; i += kBrickX * kBrickY) dst[i] = tile[i];
+  for (int i = tid; i < kTileCells * C; i += kBrickThreads) dst[i] = tile[i];
 }
 
 // Velocity-gradient contribution of one cell: plain store / read-add-store
@@ -480,9 +483,10 @@ NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
   const float* slp = sl + b.pair * cells * C;
   float* ovp = ovf + b.pair * cells * C;
   if (b.x < a.d.x && b.y < a.d.y) {
-    const int z1 = min(b.z0 + kBrickZ, a.d.z);
+    const int zb = b.z0 + static_cast<int>(threadIdx.z) * kBrickZPer;
+    const int z1 = min(zb + kBrickZPer, a.d.z);
 #pragma unroll 2
-    for (int z = b.z0; z < z1; ++z) {
+    for (int z = zb; z < z1; ++z) {
       float ux, uy, uz;
       load_velocity(a, v, b.x, b.y, z, ux, uy, uz);
       const int64_t cell = cell_offset(a.d, b.x, b.y, z);
@@ -491,15 +495,17 @@ NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
       const float bx = fx + (-tx), by = fy + (-ty), bz = fz + (-tz);
       Vec<C> g = vload<C>(g_out + (b.pair * cells + cell) * C);
       const Vec<C> fwd = vload<C>(slp + cell * C);
-      const Vec<C> back = read_linear<C, BM>(slp, a.d, bx, by, bz);
+      // the back-step read and its position gradient share one corner fetch
+      Vec<C> dx, dy, dz;
+      const Vec<C> back = read_linear_gradient<C, BM>(slp, a.d, bx, by, bz, dx, dy, dz);
       const Vec<C> cur = vload<C>(src + cell * C);
       Vec<C> corr;
 #pragma unroll
       for (int i = 0; i < C; ++i) corr.v[i] = fwd.v[i] + 0.5f * (cur.v[i] - back.v[i]);
       Vec<C> keep = vfill<C>(1.f);
       if constexpr (CLAMPM == NGT_CLAMP_MC_SMOOTH) {
-        const Vec<C> mn = read_minmax<C, BM, false>(src, a.d, fx + tx, fy + ty, fz + tz);
-        const Vec<C> mx = read_minmax<C, BM, true>(src, a.d, fx + tx, fy + ty, fz + tz);
+        Vec<C> mn, mx;
+        read_minmax2<C, BM>(src, a.d, fx + tx, fy + ty, fz + tz, mn, mx);
 #pragma unroll
         for (int i = 0; i < C; ++i)
           if (corr.v[i] < mn.v[i] || corr.v[i] > mx.v[i]) keep.v[i] = 0.f;
@@ -515,11 +521,9 @@ NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
         rd.v[i] = 0.5f * keep.v[i] * g.v[i];
         gback.v[i] = -rd.v[i];
       }
-      vstore<C>(g_dir + (b.pair * cells + cell) * C, g);
+      if (g_dir) vstore<C>(g_dir + (b.pair * cells + cell) * C, g);
       vstore<C>(r_dir + (b.pair * cells + cell) * C, rd);
       scatter_linear_tile<C, BM>(tile, b.ox, b.oy, b.oz, ovp, a.d, bx, by, bz, gback);
-      Vec<C> dx, dy, dz;
-      read_gradient<C, BM>(slp, a.d, bx, by, bz, dx, dy, dz);
       add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, -a.ts * vdot(gback, dx),
                    -a.ts * vdot(gback, dy), -a.ts * vdot(gback, dz), vel_mode == 1 ? 0 : vel_mode);
     }
@@ -528,7 +532,7 @@ NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
 }
 
 template <int C, int BM, int CLAMPM>
-__global__ __launch_bounds__(256) void advect_mc_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+__global__ __launch_bounds__(kBrickThreads) void advect_mc_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
                                                                   const float* __restrict__ sl,
                                                                   const float* __restrict__ g_out,
                                                                   float* __restrict__ g_dir, float* __restrict__ r_dir,
@@ -556,9 +560,10 @@ NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
   const float* src = rho + n * cells * C;
   float* ovp = g_rho + n * cells * C;
   if (b.x < a.d.x && b.y < a.d.y) {
-    const int z1 = min(b.z0 + kBrickZ, a.d.z);
+    const int zb = b.z0 + static_cast<int>(threadIdx.z) * kBrickZPer;
+    const int z1 = min(zb + kBrickZPer, a.d.z);
 #pragma unroll 2
-    for (int z = b.z0; z < z1; ++z) {
+    for (int z = zb; z < z1; ++z) {
       float ux, uy, uz;
       load_velocity(a, v, b.x, b.y, z, ux, uy, uz);
       const int64_t cell = cell_offset(a.d, b.x, b.y, z);
@@ -589,7 +594,7 @@ NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
 }
 
 template <int C, int BM, bool NEG, bool ORDER2>
-__global__ __launch_bounds__(256) void advect_sl_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
+__global__ __launch_bounds__(kBrickThreads) void advect_sl_bwd_brick_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ rho,
                                                                   const float* __restrict__ g_in,
                                                                   const float* __restrict__ ovf1,
                                                                   const float* __restrict__ tiles1,
@@ -799,7 +804,7 @@ __global__ __launch_bounds__(256) void advect_fill_batch_kernel(const void* __re
 }
 
 template <int BM, int CLAMPM>
-__global__ __launch_bounds__(256) void advect_mc_bwd_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
+__global__ __launch_bounds__(kBrickThreads) void advect_mc_bwd_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.brick, v.h->n, t));
@@ -813,7 +818,7 @@ __global__ __launch_bounds__(256) void advect_mc_bwd_batch_kernel(const void* __
 }
 
 template <int BM, bool NEG, bool ORDER2>
-__global__ __launch_bounds__(256) void advect_sl_bwd_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
+__global__ __launch_bounds__(kBrickThreads) void advect_sl_bwd_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
   for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.brick, v.h->n, t));
@@ -940,7 +945,7 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
     slp = sl_buf;
   }
   const bool brick_path = !ref && p.C <= 2;
-  // the brick path's order-2 overflow buffer is cleared in the same launch
+  // the brick path clears (or has B1 store) input_grad in its order-2 branch
   if (!(brick_path && order == 2)) fill_zero(input_grad, p.a.n_data * p.cells * p.C, s);
   if (brick_path) {
     // brick-tiled exact adjoint: workspace after g_vc = [tiles2 | r_dir | ovf1 | tiles1]
@@ -953,15 +958,25 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
     const bool shared_v = p.a.separate && p.a.n_data > 1;
     const int vel_mode = shared_v ? 2 : (order == 2 ? 1 : 0);
     if (shared_v) fill_zero(g_vc, static_cast<int64_t>(p.a.n_vel) * p.cells * 3, s);
-    const dim3 bgrid(static_cast<unsigned>(bg.count() * p.pairs)), bblock(kBrickX, kBrickY);
+    const dim3 bgrid(static_cast<unsigned>(bg.count() * p.pairs)), bblock(kBrickX, kBrickY, kBrickZT);
     const bool neg = order == 1 && clamp == NGT_CLAMP_NEGATIVE;
+    // one (data, velocity) pair per data entry: B1 stores the direct rho term
+    // straight into input_grad (each cell once, before B2's overflow atomics
+    // add to it); without the NEGATIVE mask B1's output gradient is g_out
+    // itself and B2 reads that
+    const bool direct = !p.a.separate || p.a.n_vel == 1;
+    const bool pass_g = clamp != NGT_CLAMP_NEGATIVE;
     if (order == 2) {
-      fill_zero2(input_grad, p.a.n_data * p.cells * p.C, ovf1, field, s);
+      if (direct) fill_zero(ovf1, field, s);
+      else fill_zero2(input_grad, p.a.n_data * p.cells * p.C, ovf1, field, s);
+      float* g_dir = pass_g ? nullptr : g_sl;
+      float* rd = direct ? input_grad : r_dir;
+      const float* g_b2 = pass_g ? output_grad : g_sl;
       NGT_SWITCH_C12(p.C, NGT_SWITCH_BM(boundary, NGT_SWITCH_CLAMP(clamp,
           hipLaunchKernelGGL((advect_mc_bwd_brick_kernel<CC, BB, KK>), bgrid, bblock, 0, s, p.a, bg, input, slp,
-                             output_grad, g_sl, r_dir, ovf1, tiles1, g_vc, vel_mode);
+                             output_grad, g_dir, rd, ovf1, tiles1, g_vc, vel_mode);
           hipLaunchKernelGGL((advect_sl_bwd_brick_kernel<CC, BB, false, true>), bgrid, bblock, 0, s, p.a, bg, input,
-                             g_sl, ovf1, tiles1, input_grad, tiles2, g_vc, vel_mode))));
+                             g_b2, ovf1, tiles1, input_grad, tiles2, g_vc, vel_mode))));
     } else {
       NGT_SWITCH_C12(p.C, NGT_SWITCH_BM(boundary,
           if (neg) hipLaunchKernelGGL((advect_sl_bwd_brick_kernel<CC, BB, true, false>), bgrid, bblock, 0, s, p.a, bg,
@@ -971,7 +986,7 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
     }
     const dim3 dgrid(blocks_for(p.a.d.x, kBX), blocks_for(p.a.d.y, kBY), static_cast<unsigned>(p.a.d.z * p.a.n_data));
     NGT_SWITCH_C12(p.C, hipLaunchKernelGGL((advect_rho_gather_kernel<CC>), dgrid, block, 0, s, p.a, bg,
-                                           order == 2 ? r_dir : nullptr, tiles2, input_grad));
+                                           order == 2 && !direct ? r_dir : nullptr, tiles2, input_grad));
     if (p.a.staggered) launch_faces(p, g_vc, vel_x_grad, vel_y_grad, vel_z_grad, s);
     return check_launch("ngt_advect_bwd");
   }
@@ -1144,7 +1159,7 @@ int ngt_advect_batch_bwd(const void* plan_host, const void* plan_device, void* w
   const dim3 b(kBX, kBY);
   hipLaunchKernelGGL(advect_fill_batch_kernel, dim3(persistent_grid(h.tiles_fill, blocks_per_cu)), dim3(256), 0, s,
                      plan_device, ws);
-  const dim3 gb(persistent_grid(h.tiles_brick, blocks_per_cu)), bb(kBrickX, kBrickY);
+  const dim3 gb(persistent_grid(h.tiles_brick, blocks_per_cu)), bb(kBrickX, kBrickY, kBrickZT);
   if (h.order == 2) {
     NGT_SWITCH_BM(h.boundary, NGT_SWITCH_CLAMP(h.clamp,
         hipLaunchKernelGGL((advect_mc_bwd_batch_kernel<BB, KK>), gb, bb, 0, s, plan_device, ws);

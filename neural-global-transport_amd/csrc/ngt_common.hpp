@@ -371,6 +371,36 @@ NGT_DEVICE Vec<C> read_minmax(const float* __restrict__ g, const Dims& d, float 
   return acc;
 }
 
+// Min and max over the 8 corners from one fetch (read_minmax semantics).
+template <int C, int BM>
+NGT_DEVICE void read_minmax2(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz, Vec<C>& mn,
+                             Vec<C>& mx) {
+  const Corners c = make_corners<BM>(d, qx, qy, qz);
+  mn = vfill<C>(kFloatMax);
+  mx = vfill<C>(-kFloatMax);
+  if constexpr (BM == BORDER) {
+    if (!(c.vx0 && c.vx1 && c.vy0 && c.vy1 && c.vz0 && c.vz1)) {
+      mn = vzero<C>();
+      mx = vzero<C>();
+    }
+  }
+  const Corner8<C> k = fetch8<C, BM>(g, d, c);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      mn.v[i] = (k.ok[j] && k.v[j].v[i] < mn.v[i]) ? k.v[j].v[i] : mn.v[i];
+      mx.v[i] = (k.ok[j] && k.v[j].v[i] > mx.v[i]) ? k.v[j].v[i] : mx.v[i];
+    }
+  }
+  if constexpr (BM == BORDER) {
+    if (!c.inside) {
+      mn = vzero<C>();
+      mx = vzero<C>();
+    }
+  }
+}
+
 template <int C, int FM, int BM>
 NGT_DEVICE Vec<C> read_filtered(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz) {
   if constexpr (FM == LINEAR) return read_linear<C, BM>(g, d, qx, qy, qz);
@@ -400,6 +430,29 @@ NGT_DEVICE void read_gradient(const float* __restrict__ g, const Dims& d, float 
       dz.v[i] = c.inside ? dz.v[i] : 0.f;
     }
   }
+}
+
+// read_linear and read_gradient of one position from one fetch of the 8
+// corners (identical results).
+template <int C, int BM>
+NGT_DEVICE Vec<C> read_linear_gradient(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz,
+                                       Vec<C>& dx, Vec<C>& dy, Vec<C>& dz) {
+  const Corners c = make_corners<BM>(d, qx, qy, qz);
+  Vec<C> k[8];
+  fetch8_raw<C, BM>(g, d, c, k);
+  mask8<C, BM>(c, k);
+  dx = vlerp(vlerp(vsub(k[1], k[0]), vsub(k[3], k[2]), c.wy), vlerp(vsub(k[5], k[4]), vsub(k[7], k[6]), c.wy), c.wz);
+  dy = vlerp(vlerp(vsub(k[2], k[0]), vsub(k[3], k[1]), c.wx), vlerp(vsub(k[6], k[4]), vsub(k[7], k[5]), c.wx), c.wz);
+  dz = vlerp(vlerp(vsub(k[4], k[0]), vsub(k[5], k[1]), c.wx), vlerp(vsub(k[6], k[2]), vsub(k[7], k[3]), c.wx), c.wy);
+  if constexpr (BM == BORDER) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      dx.v[i] = c.inside ? dx.v[i] : 0.f;
+      dy.v[i] = c.inside ? dy.v[i] : 0.f;
+      dz.v[i] = c.inside ? dz.v[i] : 0.f;
+    }
+  }
+  return lerp8<C, BM>(k, c);
 }
 
 // Scatter ``val`` with trilinear weights to the 8 corners.
