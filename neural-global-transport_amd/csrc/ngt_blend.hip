@@ -48,7 +48,7 @@ NGT_DEVICE Vec<C> blend_step_grad(const Vec<C>& prev, const Vec<C>& cell, Vec<C>
     for (int i = 0; i < C - 1; ++i) lg += cell.v[i] * g.v[i];
     Vec<C> gc;
     if constexpr (MODE == BEER_LAMBERT) {
-      const float e = expf(-fmaxf(prev.v[C - 1] + cell.v[C - 1], 0.f));
+      const float e = exp_neg(fmaxf(prev.v[C - 1] + cell.v[C - 1], 0.f));
       const float gd = g.v[C - 1] - e * lg;
 #pragma unroll
       for (int i = 0; i < C - 1; ++i) gc.v[i] = e * g.v[i];
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void blend_bwd_exact_kernel(BlendArgs a, const
       Vec<C> gc;
       if constexpr (MODE == BEER_LAMBERT) {
         dacc = fmaxf(dacc + cell.v[C - 1], 0.f);
-        const float e = expf(-dacc);
+        const float e = exp_neg(dacc);
 #pragma unroll
         for (int i = 0; i < C - 1; ++i) gc.v[i] = e * g.v[i];
         gc.v[C - 1] = g.v[C - 1] - (total - prefix);

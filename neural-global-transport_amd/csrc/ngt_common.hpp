@@ -202,45 +202,63 @@ NGT_DEVICE void fetch8_raw(const float* __restrict__ g, const Dims& d, const Cor
   for (int k = 0; k < 8; ++k) v[k] = vload<C>(g + (rows[k >> 1] + xs[k & 1]) * static_cast<uint32_t>(C));
 }
 
-// Bricked layout of a grid slice: 4x4x4-cell bricks (x fastest inside a
-// brick, bricks in x, y, z order), so a trilinear footprint and the 8x8-ray
-// footprint of a wavefront touch few cache lines whatever the view direction.
-// Cell (x, y, z) sits at  (x>>2)*64 + (x&3)  +  (y>>2)*sy + (y&3)*4
-//                       + (z>>2)*sz + (z&3)*16,  sy = 64*nbx, sz = 64*nbx*nby.
+// Haloed brick layout of a grid slice: the grid is cut into 4x4x4-cell
+// bricks, each stored with a one-cell halo as a 5x5x5 block (x fastest)
+// padded to 128 floats.  Brick (bx, by, bz) holds cells 4*b - 1 .. 4*b + 3 of
+// each axis, so every trilinear footprint whose floor corner x0 lies in
+// [-1, d.x - 1] sits inside the brick with bx = (x0 + 1) >> 2, its 8 corners
+// at the fixed offsets {0, 1, 5, 6, 25, 26, 30, 31} from the floor corner: one
+// address per sample (the rest are immediate offsets of the loads), and the
+// footprint spans 31 consecutive floats.  Halo cells outside the grid hold
+// what the boundary mode reads there: 0 (BORDER, so out-of-grid corners need
+// no mask), the clamped edge cell (CLAMP) or the wrapped cell (WRAP).
 struct BrickDims {
-  uint32_t sy, sz;       // strides of one brick step in y / z (cells)
-  uint32_t slice_cells;  // cells of one bricked slice (bricks * 64)
+  uint32_t nbx, nbxy;    // bricks per row / per plane
+  uint32_t slice_cells;  // floats per channel of one slice (bricks * 128)
 };
 
+constexpr uint32_t kBrickFloats = 128;  // 5^3 = 125 halo cells, padded
+
 inline BrickDims brick_dims(const Dims& d) {
-  const uint32_t nbx = (d.x + 3) / 4, nby = (d.y + 3) / 4, nbz = (d.z + 3) / 4;
-  return BrickDims{64u * nbx, 64u * nbx * nby, 64u * nbx * nby * nbz};
+  const uint32_t nbx = d.x / 4 + 1, nby = d.y / 4 + 1, nbz = d.z / 4 + 1;
+  const uint64_t cells = static_cast<uint64_t>(nbx) * nby * nbz * kBrickFloats;
+  return BrickDims{nbx, nbx * nby, cells < (uint64_t(1) << 32) ? static_cast<uint32_t>(cells) : 0u};
 }
 
-NGT_DEVICE uint32_t brick_offset(const BrickDims& b, int x, int y, int z) {
-  const uint32_t ux = static_cast<uint32_t>(x), uy = static_cast<uint32_t>(y), uz = static_cast<uint32_t>(z);
-  return (ux >> 2) * 64u + (ux & 3u) + (uy >> 2) * b.sy + ((uy & 3u) << 2) + (uz >> 2) * b.sz + ((uz & 3u) << 4);
+// Whether a C-channel slice of the haloed layout is addressable with 32-bit
+// offsets (the bricked kernels are used only then).
+inline bool bricked_fits(const Dims& d, int C) {
+  const BrickDims b = brick_dims(d);
+  return b.slice_cells != 0 && static_cast<uint64_t>(b.slice_cells) * C < (uint64_t(1) << 32);
 }
 
-// fetch8_raw on a bricked slice.
-template <int C, int BM>
-NGT_DEVICE void fetch8_raw_bricked(const float* __restrict__ g, const Dims& d, const BrickDims& b, const Corners& c,
-                                   Vec<C> (&v)[8]) {
-  int x0 = c.x0, x1 = c.x1, y0 = c.y0, y1 = c.y1, z0 = c.z0, z1 = c.z1;
-  if constexpr (BM == BORDER) {
-    x0 = min(max(x0, 0), d.x - 1); x1 = min(max(x1, 0), d.x - 1);
-    y0 = min(max(y0, 0), d.y - 1); y1 = min(max(y1, 0), d.y - 1);
-    z0 = min(max(z0, 0), d.z - 1); z1 = min(max(z1, 0), d.z - 1);
+// Offset (cells) of the floor corner of a footprint in the haloed layout.
+// x0 etc. are floor(q); BORDER and CLAMP clamp them into [-1, d - 1] (for
+// BORDER that changes only positions outside the forward validity, whose
+// value is zeroed anyway; for CLAMP both corners of a clamped axis then read
+// the same edge value, as the clamped reads do), WRAP wraps them.
+template <int BM>
+NGT_DEVICE uint32_t brick_corner(const Dims& d, const BrickDims& b, int x0, int y0, int z0) {
+  if constexpr (BM == WRAP) {
+    x0 = wrap_index(x0, d.x); y0 = wrap_index(y0, d.y); z0 = wrap_index(z0, d.z);
+  } else {
+    x0 = min(max(x0, -1), d.x - 1); y0 = min(max(y0, -1), d.y - 1); z0 = min(max(z0, -1), d.z - 1);
   }
-  const uint32_t X[2] = {(static_cast<uint32_t>(x0) >> 2) * 64u + (static_cast<uint32_t>(x0) & 3u),
-                         (static_cast<uint32_t>(x1) >> 2) * 64u + (static_cast<uint32_t>(x1) & 3u)};
-  const uint32_t Y[2] = {(static_cast<uint32_t>(y0) >> 2) * b.sy + ((static_cast<uint32_t>(y0) & 3u) << 2),
-                         (static_cast<uint32_t>(y1) >> 2) * b.sy + ((static_cast<uint32_t>(y1) & 3u) << 2)};
-  const uint32_t Z[2] = {(static_cast<uint32_t>(z0) >> 2) * b.sz + ((static_cast<uint32_t>(z0) & 3u) << 4),
-                         (static_cast<uint32_t>(z1) >> 2) * b.sz + ((static_cast<uint32_t>(z1) & 3u) << 4)};
-#pragma unroll
-  for (int k = 0; k < 8; ++k)
-    v[k] = vload<C>(g + (X[k & 1] + Y[(k >> 1) & 1] + Z[k >> 2]) * static_cast<uint32_t>(C));
+  const uint32_t xp = static_cast<uint32_t>(x0 + 1), yp = static_cast<uint32_t>(y0 + 1),
+                 zp = static_cast<uint32_t>(z0 + 1);
+  const uint32_t brick = (zp >> 2) * b.nbxy + (yp >> 2) * b.nbx + (xp >> 2);
+  return brick * kBrickFloats + (xp & 3u) + (yp & 3u) * 5u + (zp & 3u) * 25u;
+}
+
+// Corner k (bit0 = x-ceil, bit1 = y-ceil, bit2 = z-ceil) relative to the floor corner.
+NGT_DEVICE constexpr uint32_t brick_corner_step(int k) { return (k & 1) + ((k >> 1) & 1) * 5 + (k >> 2) * 25; }
+
+// Cell (x, y, z) of the grid, x etc. in [-1, d], of brick-local slot l.
+NGT_DEVICE void brick_slot_cell(const BrickDims& b, uint32_t brick, uint32_t l, int& x, int& y, int& z) {
+  const uint32_t bz = brick / b.nbxy, r = brick - bz * b.nbxy;
+  const uint32_t by = r / b.nbx, bx = r - by * b.nbx;
+  const uint32_t k = l / 25u, j = (l / 5u) % 5u, i = l % 5u;
+  x = static_cast<int>(4u * bx + i) - 1; y = static_cast<int>(4u * by + j) - 1; z = static_cast<int>(4u * bz + k) - 1;
 }
 
 NGT_DEVICE bool corner_ok(const Corners& c, int k) {
@@ -292,23 +310,50 @@ NGT_DEVICE Vec<C> read_linear(const float* __restrict__ g, const Dims& d, float 
 }
 
 // U trilinear reads whose 8*U loads are all issued before any is consumed;
-// BRICKED selects the bricked slice layout (BrickDims b).
+// BRICKED selects the haloed brick layout (BrickDims b).
 template <int C, int BM, int U, bool BRICKED>
 NGT_DEVICE void read_linear_batch(const float* __restrict__ g, const Dims& d, const BrickDims& b, const float (&qx)[U],
                                   const float (&qy)[U], const float (&qz)[U], Vec<C> (&out)[U]) {
-  Corners c[U];
   Vec<C> v[U][8];
+  if constexpr (BRICKED) {
+    // haloed bricks: one address per sample, no per-corner masks
+    float wx[U], wy[U], wz[U];
+    bool inside[U];
 #pragma unroll
-  for (int u = 0; u < U; ++u) c[u] = make_corners<BM>(d, qx[u], qy[u], qz[u]);
+    for (int u = 0; u < U; ++u) {
+      const float fx = floorf(qx[u]), fy = floorf(qy[u]), fz = floorf(qz[u]);
+      wx[u] = qx[u] - fx; wy[u] = qy[u] - fy; wz[u] = qz[u] - fz;
+      inside[u] = true;
+      if constexpr (BM == BORDER)
+        inside[u] = (qx[u] + 0.5f >= 0.f) & (qx[u] + 0.5f < static_cast<float>(d.x)) & (qy[u] + 0.5f >= 0.f) &
+                    (qy[u] + 0.5f < static_cast<float>(d.y)) & (qz[u] + 0.5f >= 0.f) &
+                    (qz[u] + 0.5f < static_cast<float>(d.z));
+      const uint32_t base = brick_corner<BM>(d, b, static_cast<int>(fx), static_cast<int>(fy), static_cast<int>(fz));
+      const float* p = g + base * static_cast<uint32_t>(C);
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    if constexpr (BRICKED) fetch8_raw_bricked<C, BM>(g, d, b, c[u], v[u]);
-    else fetch8_raw<C, BM>(g, d, c[u], v[u]);
-  }
+      for (int k = 0; k < 8; ++k) v[u][k] = vload<C>(p + brick_corner_step(k) * C);
+    }
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    mask8<C, BM>(c[u], v[u]);
-    out[u] = lerp8<C, BM>(v[u], c[u]);
+    for (int u = 0; u < U; ++u) {
+      const Vec<C> v0 = vlerp(vlerp(v[u][0], v[u][1], wx[u]), vlerp(v[u][2], v[u][3], wx[u]), wy[u]);
+      const Vec<C> v1 = vlerp(vlerp(v[u][4], v[u][5], wx[u]), vlerp(v[u][6], v[u][7], wx[u]), wy[u]);
+      out[u] = vlerp(v0, v1, wz[u]);
+      if constexpr (BM == BORDER) {
+#pragma unroll
+        for (int i = 0; i < C; ++i) out[u].v[i] = inside[u] ? out[u].v[i] : 0.f;
+      }
+    }
+  } else {
+    Corners c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = make_corners<BM>(d, qx[u], qy[u], qz[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) fetch8_raw<C, BM>(g, d, c[u], v[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      mask8<C, BM>(c[u], v[u]);
+      out[u] = lerp8<C, BM>(v[u], c[u]);
+    }
   }
 }
 
@@ -613,6 +658,13 @@ NGT_DEVICE void os_to_idx_lindepth(const CamParams& cp, float x, float y, float 
 
 // ------------------------------------------------------------- blending -----
 
+// e^-d (d >= 0, an optical depth) as 2^(-d log2 e) on the hardware exp2
+// (v_exp_f32, about 1 ulp): rounding the product adds a relative error of
+// about d * 2^-24, i.e. < 1e-6 for the depths whose transmittance still
+// matters at the 1e-5 forward tolerance (e^-16 ~ 1e-7).  Forward and adjoint
+// walks share it, so the adjoint differentiates the forward actually computed.
+NGT_DEVICE float exp_neg(float d) { return __builtin_amdgcn_exp2f(d * -1.44269504088896341f); }
+
 // Forward blend functors (blending.hpp:9-243).  C == 1 and ADDITIVE are plain
 // sums; for C in {2,4} the last channel is density / alpha.
 template <int C, int MODE>
@@ -622,7 +674,7 @@ NGT_DEVICE void blend_step(Vec<C>& acc, const Vec<C>& cell) {
     for (int i = 0; i < C; ++i) acc.v[i] += cell.v[i];
   } else if constexpr (MODE == BEER_LAMBERT) {
     const float dd = fmaxf(acc.v[C - 1] + cell.v[C - 1], 0.f);
-    const float t = expf(-dd);
+    const float t = exp_neg(dd);
 #pragma unroll
     for (int i = 0; i < C - 1; ++i) acc.v[i] += t * cell.v[i];
     acc.v[C - 1] = dd;
@@ -653,7 +705,7 @@ NGT_DEVICE Vec<C> blend_grad_reference(Vec<C>& grads, const Vec<C>& cell, Vec<C>
     for (int i = 0; i < C - 1; ++i) lg += cell.v[i] * grads.v[i];
     Vec<C> gc;
     if constexpr (MODE == BEER_LAMBERT) {
-      const float e = expf(-cell_out.v[C - 1]);
+      const float e = exp_neg(cell_out.v[C - 1]);
       cell_out.v[C - 1] = fmaxf(cell_out.v[C - 1] - cell.v[C - 1], 0.f);
       const float dd = grads.v[C - 1] - e * lg;
 #pragma unroll

@@ -202,8 +202,9 @@ NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, const RayArgs& a, con
   Vec<C> acc = vzero<C>();
   // Samples are fetched kUnroll at a time so their gathers are in flight
   // together; the blend recurrence then consumes them in order.  The last
-  // round's positions past the ray end are fetched (in-grid addresses) but
-  // not blended.
+  // round's positions past the ray end are fetched (in-grid addresses) and
+  // blended as empty cells, which leaves the accumulator unchanged in every
+  // blend mode (no branch per sample, so no load is sunk into one).
   for (int k = 0; k < r.steps; k += kUnroll) {
     float qx[kUnroll], qy[kUnroll], qz[kUnroll];
     Vec<C> cells[kUnroll];
@@ -214,8 +215,12 @@ NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, const RayArgs& a, con
     }
     sample_cells<C, FM, BM, LIGHT, BRICKED>(a, src, qx, qy, qz, cells);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
-      if (k + u < r.steps) blend_step<C, MODE>(acc, cells[u]);
+    for (int u = 0; u < kUnroll; ++u) {
+      const bool live = k + u < r.steps;
+#pragma unroll
+      for (int i = 0; i < C; ++i) cells[u].v[i] = live ? cells[u].v[i] : 0.f;
+      blend_step<C, MODE>(acc, cells[u]);
+    }
   }
   vstore<C>(out + ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C, acc);
 }
@@ -268,7 +273,7 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
         for (int i = 0; i < C - 1; ++i) lg += cell.v[i] * g.v[i];
         if constexpr (MODE == BEER_LAMBERT) {
           dacc = fmaxf(dacc + cell.v[C - 1], 0.f);
-          const float e = expf(-dacc);
+          const float e = exp_neg(dacc);
 #pragma unroll
           for (int i = 0; i < C - 1; ++i) gc.v[i] = e * g.v[i];
           gc.v[C - 1] = g.v[C - 1] - (total - prefix);
@@ -293,9 +298,16 @@ NGT_DEVICE void walk_ray_gradient(const RayArgs& a, const float* __restrict__ sr
         cells[u] = vzero<C>();
       }
       if constexpr (kNeedCell) sample_cells<C, FM, BM, LIGHT, BRICKED>(a, src, qx, qy, qz, cells);
+      // samples past the ray end step as empty cells (state unchanged); only
+      // the live ones reach the sink
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u)
-        if (k + u < r.steps) sink.one(k + u, qx[u], qy[u], qz[u], step(cells[u]));
+      for (int u = 0; u < kUnroll; ++u) {
+        const bool live = k + u < r.steps;
+#pragma unroll
+        for (int i = 0; i < C; ++i) cells[u].v[i] = live ? cells[u].v[i] : 0.f;
+        const Vec<C> gc = step(cells[u]);
+        if (live) sink.one(k + u, qx[u], qy[u], qz[u], gc);
+      }
     }
   }
 }
@@ -875,35 +887,53 @@ void for_cam_batches(const std::vector<Pair>& pairs, const float* M, const float
   }
 }
 
-// Copy of an NDHWC grid in the bricked slice layout (one work-item per cell,
-// coalesced reads; a brick's 64 cells are written as 16-byte runs).
-template <int C>
+// Copy of an NDHWC grid in the haloed brick layout: one work-item per stored
+// slot (two bricks per work-group), so the writes are coalesced; halo slots
+// read the boundary mode's value (0 for BORDER, the clamped or wrapped cell
+// otherwise); the 3 padding slots of a brick are written as 0.
+template <int C, int BM>
 NGT_DEVICE void to_bricked_kernel_body(const VBlock& vb, const float* __restrict__ in, Dims d, BrickDims b,
-                                                         float* __restrict__ out) {
-  const int x = vb.x * 64 + threadIdx.x;
-  const int y = vb.y * 4 + threadIdx.y;
-  const int z = vb.z % d.z;
-  const int n = vb.z / d.z;
-  if (x >= d.x || y >= d.y) return;
-  const int64_t cells = static_cast<int64_t>(d.x) * d.y * d.z;
-  const Vec<C> v = vload<C>(in + (n * cells + cell_offset(d, x, y, z)) * C);
-  vstore<C>(out + (static_cast<int64_t>(n) * b.slice_cells + brick_offset(b, x, y, z)) * C, v);
+                                       float* __restrict__ out) {
+  const uint32_t t = static_cast<uint32_t>(vb.x) * 256u + threadIdx.x;
+  if (t >= b.slice_cells) return;
+  const int n = vb.y;
+  Vec<C> v = vzero<C>();
+  const uint32_t l = t % kBrickFloats;
+  if (l < 125u) {
+    int x, y, z;
+    brick_slot_cell(b, t / kBrickFloats, l, x, y, z);
+    bool ok = true;
+    if constexpr (BM == BORDER) {
+      ok = x >= 0 && x < d.x && y >= 0 && y < d.y && z >= 0 && z < d.z;
+    } else if constexpr (BM == CLAMP) {
+      x = min(max(x, 0), d.x - 1); y = min(max(y, 0), d.y - 1); z = min(max(z, 0), d.z - 1);
+    } else {
+      x = wrap_index(x, d.x); y = wrap_index(y, d.y); z = wrap_index(z, d.z);
+    }
+    const int64_t cells = static_cast<int64_t>(d.x) * d.y * d.z;
+    if (ok) v = vload<C>(in + (n * cells + cell_offset(d, x, y, z)) * C);
+  }
+  vstore<C>(out + (static_cast<int64_t>(n) * b.slice_cells + t) * C, v);
 }
 
-template <int C>
+template <int C, int BM>
 __global__ __launch_bounds__(256) void to_bricked_kernel(const float* __restrict__ in, Dims d, BrickDims b,
                                                          float* __restrict__ out) {
-  to_bricked_kernel_body<C>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, in, d, b, out);
+  to_bricked_kernel_body<C, BM>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), 0}, in, d, b, out);
 }
 
-void launch_to_bricked(const float* in, const Dims& d, const BrickDims& b, int n_data, int C, float* out,
+void launch_to_bricked(const float* in, const Dims& d, const BrickDims& b, int n_data, int C, int boundary, float* out,
                        hipStream_t s) {
-  const dim3 grid(blocks_for(d.x, 64), blocks_for(d.y, 4), static_cast<unsigned>(d.z * n_data));
-  NGT_SWITCH_C(C, hipLaunchKernelGGL((to_bricked_kernel<CC>), grid, dim3(64, 4), 0, s, in, d, b, out));
+  const dim3 grid(blocks_for(b.slice_cells, 256), static_cast<unsigned>(n_data));
+  NGT_SWITCH_C(C, NGT_SWITCH_BM(boundary,
+      hipLaunchKernelGGL((to_bricked_kernel<CC, BB>), grid, dim3(256), 0, s, in, d, b, out)));
 }
 
+// Bytes of the haloed brick copy of a grid; 0 when its slices would not be
+// addressable with 32-bit offsets (callers then use the plain kernels).
 size_t bricked_bytes(const int64_t* shape, int channels) {
   const Dims d{static_cast<int>(shape[3]), static_cast<int>(shape[2]), static_cast<int>(shape[1])};
+  if (!bricked_fits(d, channels)) return 0;
   return static_cast<size_t>(shape[0]) * brick_dims(d).slice_cells * channels * sizeof(float);
 }
 
@@ -976,7 +1006,7 @@ size_t gather_workspace_bytes(const int64_t* shape, const float* M, const float*
                               int num_cameras, const int32_t* out_dhw, int interpolation, int boundary, int separate,
                               int grad_channels) {
   if (!shape || !M || !V || !P || !F || !out_dhw || num_cameras <= 0) return 0;
-  if (interpolation != LINEAR || boundary != BORDER) return 0;
+  if (interpolation != LINEAR || boundary != BORDER || bricked_bytes(shape, grad_channels) == 0) return 0;
   std::vector<Pair> pairs;
   if (make_pairs(static_cast<int>(shape[0]), num_cameras, separate != 0, pairs) != NGT_OK) return 0;
   const RayArgs a = make_ray_args(nullptr, shape, out_dhw);
@@ -990,7 +1020,8 @@ size_t gather_workspace_bytes(const int64_t* shape, const float* M, const float*
 bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* F,
                  const RayArgs& a, int interpolation, int boundary, int grad_channels, void* workspace,
                  size_t workspace_bytes, GatherPlan& gp) {
-  if (interpolation != LINEAR || boundary != BORDER || workspace == nullptr) return false;
+  if (interpolation != LINEAR || boundary != BORDER || workspace == nullptr || !bricked_fits(a.din, grad_channels))
+    return false;
   gp.kcap = ray_step_cap(pairs, M, V, P, F, a);
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   const size_t rays_bytes = align_up(pairs.size() * npix * sizeof(RayRec));
@@ -1008,7 +1039,7 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
                 const RayArgs& a_in, int blending_mode, int C, const float* output, const float* output_grad,
                 const GatherPlan& gp, float* in_grad, int n_data, hipStream_t s) {
   RayArgs a = a_in;
-  launch_to_bricked(a.input, a.din, a.bd, n_data, LIGHT ? 1 : C, gp.bricked, s);
+  launch_to_bricked(a.input, a.din, a.bd, n_data, LIGHT ? 1 : C, BORDER, gp.bricked, s);
   a.bricked = gp.bricked;
   for (size_t b = 0; b < pairs.size(); b += kMaxCamsPerLaunch) {
     CamBatch cb;
@@ -1095,7 +1126,7 @@ struct RenBatchItem {
   CamBatch cb;
   int ntx, nty;        // 16 x 16 pixel tiles per view
   int kcap;
-  int nbx64, nby4;     // tiles of the bricked copy
+  int brick_tiles;     // 256-slot tiles of the bricked copy
   float* images;
   const float* images_grad;
   float* density_grad;
@@ -1133,8 +1164,8 @@ __global__ __launch_bounds__(256) void render_brickify_batch_kernel(const void* 
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.brickify, v.h->n, t));
     const RenBatchItem& it = v.items[i];
     const int local = t - v.brickify[i];
-    const VBlock vb{local % it.nbx64, (local / it.nbx64) % it.nby4, local / (it.nbx64 * it.nby4)};
-    to_bricked_kernel_body<1>(vb, it.a.input, it.a.din, it.a.bd, reinterpret_cast<float*>(ws + it.ws_bricked));
+    to_bricked_kernel_body<1, BORDER>(VBlock{local, 0, 0}, it.a.input, it.a.din, it.a.bd,
+                                      reinterpret_cast<float*>(ws + it.ws_bricked));
   }
 }
 
@@ -1210,7 +1241,7 @@ int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_sha
   const size_t need = ngt_raymarch_grid_transform_fwd_workspace_size(input_shape, interpolation);
   const bool bricked = need > 0 && workspace != nullptr && workspace_bytes >= need;
   if (bricked) {
-    launch_to_bricked(input, a.din, a.bd, a.n_data, C, static_cast<float*>(workspace), s);
+    launch_to_bricked(input, a.din, a.bd, a.n_data, C, boundary, static_cast<float*>(workspace), s);
     a.bricked = static_cast<const float*>(workspace);
   }
   for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
@@ -1319,7 +1350,7 @@ int ngt_render_density_fwd(const float* density, const int64_t* density_shape, c
   const size_t need = ngt_render_density_fwd_workspace_size(density_shape, interpolation);
   const bool bricked = need > 0 && workspace != nullptr && workspace_bytes >= need;
   if (bricked) {
-    launch_to_bricked(density, a.din, a.bd, a.n_data, 1, static_cast<float*>(workspace), s);
+    launch_to_bricked(density, a.din, a.bd, a.n_data, 1, boundary, static_cast<float*>(workspace), s);
     a.bricked = static_cast<const float*>(workspace);
   }
   for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
@@ -1522,8 +1553,8 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     it.kcap = ray_step_cap(pairs, q.matrix_m, q.matrix_v, q.matrix_p, q.frustum, it.a);
     it.ntx = static_cast<int>(blocks_for(it.a.out_w, 16));
     it.nty = static_cast<int>(blocks_for(it.a.out_h, 16));
-    it.nbx64 = static_cast<int>(blocks_for(it.a.din.x, 64));
-    it.nby4 = static_cast<int>(blocks_for(it.a.din.y, 4));
+    NGT_REQUIRE(bricked_fits(it.a.din, 1), NGT_ERR_INVALID_ARGUMENT, "batch density grid too large");
+    it.brick_tiles = static_cast<int>(blocks_for(it.a.bd.slice_cells, 256));
     it.images = q.images; it.images_grad = q.images_grad; it.density_grad = q.density_grad;
     const size_t npix = static_cast<size_t>(it.a.out_w) * it.a.out_h;
     it.ws_bricked = static_cast<int64_t>(ws);
@@ -1533,7 +1564,7 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     it.ws_G = static_cast<int64_t>(ws);
     ws += align_up(q.num_views * static_cast<size_t>(it.kcap) * npix * sizeof(float));
     tr += static_cast<int64_t>(q.num_views) * it.ntx * it.nty;
-    tz += static_cast<int64_t>(it.nbx64) * it.nby4 * it.a.din.z;
+    tz += it.brick_tiles;
     tb += static_cast<int64_t>(blocks_for(it.a.din.x, 8)) * blocks_for(it.a.din.y, 8) * blocks_for(it.a.din.z, 8);
     NGT_REQUIRE(tr < (int64_t(1) << 31) && tz < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT, "batch too large");
     ray[i + 1] = static_cast<int>(tr); brickify[i + 1] = static_cast<int>(tz); brick[i + 1] = static_cast<int>(tb);
@@ -1557,7 +1588,7 @@ int ngt_render_batch_fwd(const void* plan_host, const void* plan_device, void* w
   hipStream_t s = as_stream(stream);
   char* ws = static_cast<char*>(workspace);
   hipLaunchKernelGGL(render_brickify_batch_kernel, dim3(persistent_grid(h.tiles_brickify, blocks_per_cu)),
-                     dim3(64, 4), 0, s, plan_device, ws);
+                     dim3(256), 0, s, plan_device, ws);
   const dim3 g(persistent_grid(h.tiles_ray, blocks_per_cu));
   NGT_SWITCH_MODE(h.mode, hipLaunchKernelGGL((render_fwd_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws));
   return check_launch("ngt_render_batch_fwd");
@@ -1576,7 +1607,7 @@ int ngt_render_batch_bwd(const void* plan_host, const void* plan_device, void* w
   hipStream_t s = as_stream(stream);
   char* ws = static_cast<char*>(workspace);
   hipLaunchKernelGGL(render_brickify_batch_kernel, dim3(persistent_grid(h.tiles_brickify, blocks_per_cu)),
-                     dim3(64, 4), 0, s, plan_device, ws);
+                     dim3(256), 0, s, plan_device, ws);
   const dim3 g(persistent_grid(h.tiles_ray, blocks_per_cu));
   NGT_SWITCH_MODE(h.mode,
       hipLaunchKernelGGL((render_prepass_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws));
