@@ -273,51 +273,53 @@ __global__ __launch_bounds__(256) void add_normalized_kernel(AdvArgs a, const fl
 }
 
 // B3: centered-velocity gradient -> MAC face gradients (adjoint of the face
-// average).  One work-item per face of each component.
+// average).  A work-group covers 64 x 4 (x, y) columns of one velocity over
+// kFaceZ planes; each work-item walks its column in z, so g_vc is read once:
+// the z - 1 term is the previous plane's value kept in a register, the x - 1
+// and y - 1 terms hit the lines the neighbouring work-items just read.  The
+// last cell of an axis also writes the closing face (x = d.x, y = d.y,
+// z = d.z), which has only the left cell.
+constexpr int kFaceZ = 8;
+
+inline int64_t face_tiles(const Dims& d) {
+  return static_cast<int64_t>((d.x + 63) / 64) * ((d.y + 3) / 4) * ((d.z + kFaceZ - 1) / kFaceZ);
+}
+
 NGT_DEVICE void faces_from_centered_grad_kernel_body(const VBlock& vb, Dims d, int n_vel, const float* __restrict__ g_vc,
-                                                                      float* __restrict__ gx, float* __restrict__ gy,
-                                                                      float* __restrict__ gz) {
+                                                     float* __restrict__ gx, float* __restrict__ gy,
+                                                     float* __restrict__ gz) {
+  const int ntx = (d.x + 63) / 64, nty = (d.y + 3) / 4, nzc = (d.z + kFaceZ - 1) / kFaceZ;
+  const int per_v = ntx * nty * nzc;
+  const int v = vb.x / per_v;
+  const int t = vb.x - v * per_v;
+  const int x = (t % ntx) * 64 + static_cast<int>(threadIdx.x & 63u);
+  const int y = ((t / ntx) % nty) * 4 + static_cast<int>(threadIdx.x >> 6);
+  const int z0 = (t / (ntx * nty)) * kFaceZ, z1 = min(z0 + kFaceZ, d.z);
+  if (v >= n_vel || x >= d.x || y >= d.y) return;
   const int64_t nx = static_cast<int64_t>(d.z) * d.y * (d.x + 1);
   const int64_t ny = static_cast<int64_t>(d.z) * (d.y + 1) * d.x;
   const int64_t nz = static_cast<int64_t>(d.z + 1) * d.y * d.x;
-  const int64_t per_v = nx + ny + nz;
-  const int64_t i = static_cast<int64_t>(vb.x) * blockDim.x + threadIdx.x;
-  if (i >= per_v * n_vel) return;
-  const int v = static_cast<int>(i / per_v);
-  int64_t j = i % per_v;
   const int64_t cells = static_cast<int64_t>(d.x) * d.y * d.z;
   const float* g = g_vc + static_cast<int64_t>(v) * cells * 3;
-  if (j < nx) {
-    const int x = static_cast<int>(j % (d.x + 1));
-    const int64_t r = j / (d.x + 1);
-    const int y = static_cast<int>(r % d.y), z = static_cast<int>(r / d.y);
-    float s = 0.f;
-    if (x > 0) s += g[cell_offset(d, x - 1, y, z) * 3 + 0];
-    if (x < d.x) s += g[cell_offset(d, x, y, z) * 3 + 0];
-    gx[static_cast<int64_t>(v) * nx + j] = 0.5f * s;
-    return;
+  float* ox = gx + static_cast<int64_t>(v) * nx;
+  float* oy = gy + static_cast<int64_t>(v) * ny;
+  float* oz = gz + static_cast<int64_t>(v) * nz;
+  float prev = z0 > 0 ? g[cell_offset(d, x, y, z0 - 1) * 3 + 2] : 0.f;
+  for (int z = z0; z < z1; ++z) {
+    const int64_t c = cell_offset(d, x, y, z);
+    const float g0 = g[c * 3 + 0], g1 = g[c * 3 + 1], g2 = g[c * 3 + 2];
+    const float l0 = x > 0 ? g[(c - 1) * 3 + 0] : 0.f;
+    const float l1 = y > 0 ? g[(c - d.x) * 3 + 1] : 0.f;
+    const int64_t fx = (static_cast<int64_t>(z) * d.y + y) * (d.x + 1) + x;
+    ox[fx] = 0.5f * (l0 + g0);
+    if (x == d.x - 1) ox[fx + 1] = 0.5f * g0;
+    const int64_t fy = (static_cast<int64_t>(z) * (d.y + 1) + y) * d.x + x;
+    oy[fy] = 0.5f * (l1 + g1);
+    if (y == d.y - 1) oy[fy + d.x] = 0.5f * g1;
+    oz[c] = 0.5f * (prev + g2);
+    prev = g2;
   }
-  j -= nx;
-  if (j < ny) {
-    const int x = static_cast<int>(j % d.x);
-    const int64_t r = j / d.x;
-    const int y = static_cast<int>(r % (d.y + 1)), z = static_cast<int>(r / (d.y + 1));
-    float s = 0.f;
-    if (y > 0) s += g[cell_offset(d, x, y - 1, z) * 3 + 1];
-    if (y < d.y) s += g[cell_offset(d, x, y, z) * 3 + 1];
-    gy[static_cast<int64_t>(v) * ny + j] = 0.5f * s;
-    return;
-  }
-  j -= ny;
-  {
-    const int x = static_cast<int>(j % d.x);
-    const int64_t r = j / d.x;
-    const int y = static_cast<int>(r % d.y), z = static_cast<int>(r / d.y);
-    float s = 0.f;
-    if (z > 0) s += g[cell_offset(d, x, y, z - 1) * 3 + 2];
-    if (z < d.z) s += g[cell_offset(d, x, y, z) * 3 + 2];
-    gz[static_cast<int64_t>(v) * nz + j] = 0.5f * s;
-  }
+  if (z1 == d.z) oz[cells + static_cast<int64_t>(y) * d.x + x] = 0.5f * prev;
 }
 
 __global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, int n_vel, const float* __restrict__ g_vc,
@@ -705,10 +707,8 @@ dim3 adv_grid(const AdvPlan& p) {
 
 void launch_faces(const AdvPlan& p, const float* g_vc, float* gx, float* gy, float* gz, hipStream_t s) {
   const Dims d = p.a.d;
-  const int64_t faces = (static_cast<int64_t>(d.z) * d.y * (d.x + 1) + static_cast<int64_t>(d.z) * (d.y + 1) * d.x +
-                         static_cast<int64_t>(d.z + 1) * d.y * d.x) * p.a.n_vel;
-  hipLaunchKernelGGL(faces_from_centered_grad_kernel, dim3(blocks_for(faces, 256)), dim3(256), 0, s, d, p.a.n_vel,
-                     g_vc, gx, gy, gz);
+  hipLaunchKernelGGL(faces_from_centered_grad_kernel, dim3(static_cast<unsigned>(face_tiles(d) * p.a.n_vel)), dim3(256),
+                     0, s, d, p.a.n_vel, g_vc, gx, gy, gz);
 }
 
 // ---------------------------------------------------------------------------
@@ -1101,13 +1101,11 @@ int ngt_advect_batch_plan(const ngt_advect_instance* inst, int n, int order, int
     tc += static_cast<int64_t>(it.nbx) * it.nby * q.dhw[0];
     tb += it.bg.count();
     if (staggered) {
-      const Dims& d = it.a.d;
-      const int64_t faces = static_cast<int64_t>(d.z) * d.y * (d.x + 1) + static_cast<int64_t>(d.z) * (d.y + 1) * d.x +
-                            static_cast<int64_t>(d.z + 1) * d.y * d.x;
-      tf += blocks_for(faces, 256);
+      tf += face_tiles(it.a.d);
     }
     tl += static_cast<int64_t>(order == 2 ? 2 : 1) * ((it.cells + kFillChunk - 1) / kFillChunk);
-    NGT_REQUIRE(tc < (int64_t(1) << 31) && tl < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT, "batch too large");
+    NGT_REQUIRE(tc < (int64_t(1) << 31) && tl < (int64_t(1) << 31) && tf < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT,
+                "batch too large");
     cell[i + 1] = static_cast<int>(tc); brick[i + 1] = static_cast<int>(tb);
     face[i + 1] = static_cast<int>(tf); fill[i + 1] = static_cast<int>(tl);
   }

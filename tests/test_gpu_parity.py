@@ -193,7 +193,7 @@ def test_blend(ngt_ops, mode, C, keep):
 
 @pytest.mark.parametrize("mode", [B.BEER_LAMBERT, B.ALPHA, B.ALPHA_ADDITIVE, B.ADDITIVE])
 @pytest.mark.parametrize("C", [1, 4])
-@pytest.mark.parametrize("bound", [BORDER, CLAMP])
+@pytest.mark.parametrize("bound", [BORDER, CLAMP, WRAP])
 def test_raymarch(ngt_ops, mode, C, bound):
     rng = np.random.default_rng(500 + mode * 10 + C + bound)
     dims = (12, 11, 13)
@@ -352,11 +352,12 @@ def test_advect_overflow_channels_batches(ngt_ops, vmax, C, separate):
 
 
 @pytest.mark.parametrize("C,bound,mode", [(1, BORDER, B.BEER_LAMBERT), (2, CLAMP, B.ALPHA_ADDITIVE),
-                                          (4, BORDER, B.ALPHA), (4, CLAMP, B.ADDITIVE)])
+                                          (4, BORDER, B.ALPHA), (4, CLAMP, B.ADDITIVE), (1, WRAP, B.BEER_LAMBERT),
+                                          (2, WRAP, B.ADDITIVE)])
 def test_raymarch_bricked_layout_is_bit_identical(ngt_ops, C, bound, mode):
-    """The forward over the 4x4x4-bricked copy (dims not multiples of 4)
-    equals the direct NDHWC read bit for bit, and so do the gather-adjoint
-    gradients that read it."""
+    """The forward over the haloed 4x4x4-brick copy (dims not multiples of 4;
+    halo cells hold the boundary mode's values) equals the direct NDHWC read
+    bit for bit, for the generic raymarch and the fused density renderer."""
     rng = np.random.default_rng(800 + C)
     dims = (13, 10, 11)
     out_dhw = (17, 9, 12)
