@@ -383,7 +383,7 @@ NGT_DEVICE void scatter_linear_tile(float* tile, int ox, int oy, int oz, float* 
         static_cast<unsigned>(lz) < static_cast<unsigned>(kTileZ)) {
       float* t = tile + ((lz * kTileY + ly) * kTileX + lx) * C;
 #pragma unroll
-      for (int i = 0; i < C; ++i) atomicAdd(t + i, val.v[i] * w);
+      for (int i = 0; i < C; ++i) lds_add(t + i, val.v[i] * w);
     } else {
       const int64_t o = cell_offset(d, ix, iy, iz);
 #pragma unroll

@@ -110,6 +110,14 @@ NGT_DEVICE float vdot(const Vec<C>& a, const Vec<C>& b) {
 // Hardware fp32 atomic add (global_atomic_add_f32, no CAS loop).
 NGT_DEVICE void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 
+// fp32 atomic add into LDS (ds_add_f32).  p must point into __shared__
+// memory: a plain float* reaching an atomic is otherwise compiled as a flat
+// atomic, which goes through the vector-memory path.
+typedef __attribute__((address_space(3))) float lds_float;
+NGT_DEVICE void lds_add(float* p, float v) {
+  __hip_atomic_fetch_add((lds_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Block coordinates of a work-group's tile.  Kernel bodies take them as an
 // argument so the same body runs under a plain grid launch (blockIdx) and
 // under the persistent work-list launches of uneven batches.
