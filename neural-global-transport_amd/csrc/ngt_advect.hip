@@ -332,15 +332,17 @@ __global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, i
 // Brick-tiled exact backward (C <= 2).  A work-group owns a brick of
 // kBrickX x kBrickY x kBrickZ cells and an LDS tile of the brick grown by
 // kHalo cells on every side.  The trilinear scatter of the brick's samples
-// lands in the tile whenever the displacement |ts*v| stays within kHalo
-// cells (corners of x +- s lie in [x - kHalo, x + kHalo]); rarer corners
-// further out (large velocities, WRAP across the domain edge) go to an
-// overflow field with global fp32 atomics.  Each tile is stored whole to a
-// per-brick partial buffer with plain coalesced stores, and the consumer
-// gathers, for every cell, the (at most 2 per axis) tiles that cover it in
-// a fixed order: no global atomics in the common case and a deterministic
-// sum.
-constexpr int kBrickX = 32, kBrickY = 8, kBrickZ = 8, kHalo = 3;
+// lands in the tile whenever the displacement s = ts*v per axis satisfies
+// -kHalo < s <= kHalo (the corners floor(x - s), floor(x - s) + 1 then lie in
+// [x - kHalo, x + kHalo]), i.e. for displacements of up to 2 cells per step;
+// rarer corners further out (faster flows, WRAP across the domain edge) go
+// to an overflow field with global fp32 atomics.  Each tile is stored whole
+// to a per-brick partial buffer with plain coalesced stores, and the
+// consumer gathers, for every cell, the (at most 2 per axis) tiles that
+// cover it in a fixed order: no global atomics in the common case and a
+// deterministic sum.  The halo sets the partial-buffer traffic (tile cells
+// per brick cell: 2.53 with kHalo = 2, 3.64 with 3).
+constexpr int kBrickX = 32, kBrickY = 8, kBrickZ = 8, kHalo = 2;
 // a work-group is kBrickX x kBrickY x kBrickZT work-items; each walks
 // kBrickZ / kBrickZT cells of its (x, y) column
 constexpr int kBrickZT = 2, kBrickZPer = kBrickZ / kBrickZT;
