@@ -67,8 +67,8 @@ OP_KERNELS = {
     "advect_fwd": ["advect_sl_kernel", "advect_mc_kernel"],
     "advect_bwd": ["advect_mc_bwd_brick_kernel", "advect_sl_bwd_brick_kernel", "advect_rho_gather_kernel",
                    "faces_from_centered_grad_kernel", "fill_kernel"],
-    "render_fwd": ["to_bricked_kernel", "raymarch_fwd_kernel", "l2_partial_kernel", "l2_finish_kernel"],
-    "render_bwd": ["to_bricked_kernel", "raymarch_bwd_prepass_kernel", "raymarch_bwd_brick_kernel"],
+    "render_fwd": ["to_bricked_kernel", "raymarch_fwd_kernel", "l2_finish_kernel"],
+    "render_bwd": ["raymarch_bwd_prepass_kernel", "raymarch_bwd_brick_kernel"],
 }
 COPY_BYTES = 256 << 20
 
@@ -273,19 +273,26 @@ def op_calls(torch, prob):
     rws = torch.empty(max(rws_b, 1), dtype=torch.uint8, device=rho.device)
     fws_b = lib.ngt_render_density_l2_fwd_workspace_size(s_rho, 1, cams.nv, out_dhw)
     fws = torch.empty(max(fws_b, 1), dtype=torch.uint8, device=rho.device)
+    brk_b = lib.ngt_bricked_grid_bytes(s_rho)
+    brk = torch.empty(brk_b, dtype=torch.uint8, device=rho.device)  # the forward's brick copy, reused backward
     P = _lib.ptr
+
+    def render_fwd():
+        _lib.check(lib.ngt_bricked_grid(P(rho), s_rho, 0, P(brk), st()))
+        _lib.check(lib.ngt_render_density_l2_fwd(
+            P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(tgt), 1.0,
+            P(brk), P(img), P(loss), P(fws), fws_b, st()))
+
     calls = {
         "advect_fwd": lambda: _lib.check(lib.ngt_advect_fwd(
             P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(adv), P(sl), None, 0, st())),
         "advect_bwd": lambda: _lib.check(lib.ngt_advect_bwd(
             P(rho), s_rho, P(vx), P(vy), P(vz), 1, -1.0, 2, 1, 0, 0, P(g_out), P(sl), 0, P(g_in), P(gx), P(gy),
             P(gz), P(ws), wsb, st())),
-        "render_fwd": lambda: _lib.check(lib.ngt_render_density_l2_fwd(
-            P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(tgt), 1.0,
-            P(img), P(loss), P(fws), fws_b, st())),
+        "render_fwd": render_fwd,
         "render_bwd": lambda: _lib.check(lib.ngt_render_density_l2_bwd(
             P(rho), s_rho, lp, prob["ds"], P(img), P(tgt), 1.0, P(loss_grad), cams.m, cams.v, cams.p, cams.f,
-            cams.nv, out_dhw, 1, 0, 0, P(g_adv), P(rws), rws_b, st())),
+            cams.nv, out_dhw, 1, 0, 0, P(brk), P(g_adv), P(rws), rws_b, st())),
     }
     # every raw buffer handed to the C ABI has the element count its entry
     # point reads (the ABI takes plain pointers and cannot check them)

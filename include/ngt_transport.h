@@ -222,6 +222,15 @@ int ngt_render_density_bwd(const float* density, const int64_t* density_shape, c
                            const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
                            float* density_grad, void* workspace, size_t workspace_bytes, ngt_stream_t stream);
 
+/* ---- Haloed brick copy of a grid (no reference counterpart: the layout
+ * the LINEAR ray kernels read, exposed so that a caller can make it once and
+ * hand it to both the forward and the backward of the fused L2 render) -------
+ * 4x4x4-cell bricks stored with a one-cell halo (5x5x5 padded to 128 floats),
+ * halo cells holding the boundary mode's values.  _bytes returns 0 when the
+ * grid's slices are too large for the layout's 32-bit offsets. */
+size_t ngt_bricked_grid_bytes(const int64_t* shape);
+int ngt_bricked_grid(const float* grid, const int64_t* shape, int boundary, float* bricked, ngt_stream_t stream);
+
 /* ---- Fused render + L2 image loss (optimization.py:478-526, the density
  * target loss with loss_func = squared error) ---------------------------------
  * _l2_fwd renders like ngt_render_density_fwd into output [N,V,H,W,4] and
@@ -230,14 +239,18 @@ int ngt_render_density_bwd(const float* density, const int64_t* density_shape, c
  * order.  _l2_bwd writes density_grad for the upstream gradient *loss_grad
  * (device float): the image gradient 2 * loss_weight * loss_grad *
  * (output - target) is formed inside the render backward's gradient pass
- * instead of being read from memory. */
+ * instead of being read from memory.  `bricked` (nullable): the haloed brick
+ * copy of density made by ngt_bricked_grid with the same boundary; when
+ * given, neither call makes its own copy (the workspace sizes still assume
+ * NULL). */
 size_t ngt_render_density_l2_fwd_workspace_size(const int64_t* density_shape, int interpolation, int num_cameras,
                                                 const int32_t* output_dhw);
 int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
                               float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
                               const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
-                              int boundary, int blending_mode, const float* target, float loss_weight, float* output,
-                              float* loss, void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+                              int boundary, int blending_mode, const float* target, float loss_weight,
+                              const float* bricked, float* output, float* loss, void* workspace,
+                              size_t workspace_bytes, ngt_stream_t stream);
 size_t ngt_render_density_l2_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
                                                 const float* matrix_v, const float* matrix_p, const float* frustum,
                                                 int num_cameras, const int32_t* output_dhw, int interpolation,
@@ -246,8 +259,8 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
                               float depth_step, const float* output, const float* target, float loss_weight,
                               const float* loss_grad, const float* matrix_m, const float* matrix_v,
                               const float* matrix_p, const float* frustum, int num_cameras, const int32_t* output_dhw,
-                              int interpolation, int boundary, int blending_mode, float* density_grad,
-                              void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+                              int interpolation, int boundary, int blending_mode, const float* bricked,
+                              float* density_grad, void* workspace, size_t workspace_bytes, ngt_stream_t stream);
 
 /* ---- AdvectGridSemiLangrange (advect.cc:21-71) with VelocityGrid.warp
  * semantics (data_structures.py:1690-1737) --------------------------------

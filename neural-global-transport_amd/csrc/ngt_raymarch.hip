@@ -43,6 +43,10 @@ struct RayArgs {
   const float* l2_target;     // [N*V, H, W, 3]
   const float* l2_loss_grad;  // one float on the device
   float l2_scale;
+  // fused L2 image loss (render forward): when set, each work-group of the
+  // forward writes the sum over its pixels and RGB of (image - target)^2 to
+  // l2_partial[linear block index of the launch]
+  float* l2_partial;
 };
 
 NGT_DEVICE int kMaxRaySteps(const RayArgs& a) { return 16 * (a.din.x + a.din.y + a.din.z + a.out_d); }
@@ -191,10 +195,8 @@ NGT_DEVICE void scatter_cell(const RayArgs& a, const float* __restrict__ src, fl
 }
 
 template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
-NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, float* __restrict__ out) {
-  int x, y;
-  pixel_of_thread(vb, x, y);
-  if (x >= a.out_w || y >= a.out_h) return;
+NGT_DEVICE void raymarch_fwd_pixel(const RayArgs& a, const CamBatch& cb, const VBlock& vb, int x, int y,
+                                   float* __restrict__ out, float& l2) {
   const CamParams& cp = cb.cam[vb.z];
   constexpr int CIN = LIGHT ? 1 : C;
   const float* src = ray_source<CIN, BRICKED>(a, cp.data_index);
@@ -222,7 +224,33 @@ NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, const RayArgs& a, con
       blend_step<C, MODE>(acc, cells[u]);
     }
   }
-  vstore<C>(out + ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C, acc);
+  const int64_t pix = (static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x;
+  vstore<C>(out + pix * C, acc);
+  if constexpr (C >= 3) {
+    if (a.l2_partial) {
+      const float* t = a.l2_target + pix * 3;
+      const float d0 = acc.v[0] - t[0], d1 = acc.v[1] - t[1], d2 = acc.v[2] - t[2];
+      l2 = d0 * d0 + d1 * d1 + d2 * d2;
+    }
+  }
+}
+
+template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
+NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, float* __restrict__ out) {
+  int x, y;
+  pixel_of_thread(vb, x, y);
+  float l2 = 0.f;
+  if (x < a.out_w && y < a.out_h) raymarch_fwd_pixel<C, FM, BM, MODE, LIGHT, BRICKED>(a, cb, vb, x, y, out, l2);
+  if (a.l2_partial) {  // uniform: the work-group's partial of the fused L2 loss
+    __shared__ float s_w[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) l2 += __shfl_down(l2, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = l2;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      a.l2_partial[(static_cast<int64_t>(vb.z) * gridDim.y + vb.y) * gridDim.x + vb.x] =
+          (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+  }
 }
 
 template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
@@ -870,6 +898,7 @@ RayArgs make_ray_args(const float* input, const int64_t* shape, const int32_t* o
   a.l2_target = nullptr;
   a.l2_loss_grad = nullptr;
   a.l2_scale = 0.f;
+  a.l2_partial = nullptr;
   return a;
 }
 
@@ -1034,13 +1063,19 @@ bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V,
   return true;
 }
 
+// bricked_in: a haloed brick copy of the input made by the caller (BORDER
+// halo), or nullptr to make one in the plan's workspace.
 template <bool LIGHT>
 void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* F,
                 const RayArgs& a_in, int blending_mode, int C, const float* output, const float* output_grad,
-                const GatherPlan& gp, float* in_grad, int n_data, hipStream_t s) {
+                const GatherPlan& gp, float* in_grad, int n_data, hipStream_t s, const float* bricked_in = nullptr) {
   RayArgs a = a_in;
-  launch_to_bricked(a.input, a.din, a.bd, n_data, LIGHT ? 1 : C, BORDER, gp.bricked, s);
-  a.bricked = gp.bricked;
+  if (bricked_in) {
+    a.bricked = bricked_in;
+  } else {
+    launch_to_bricked(a.input, a.din, a.bd, n_data, LIGHT ? 1 : C, BORDER, gp.bricked, s);
+    a.bricked = gp.bricked;
+  }
   for (size_t b = 0; b < pairs.size(); b += kMaxCamsPerLaunch) {
     CamBatch cb;
     cb.count = static_cast<int>(std::min<size_t>(kMaxCamsPerLaunch, pairs.size() - b));
@@ -1071,24 +1106,9 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
 
 // ---------------------------------------------------------------------------
 // Fused L2 image loss: sum over pixels and RGB of (image - target)^2, as
-// per-work-group partial sums and one fixed-order final sum (deterministic).
-
-__global__ __launch_bounds__(256) void l2_partial_kernel(const float* __restrict__ img, const float* __restrict__ tgt,
-                                                         int64_t npix, float* __restrict__ partial) {
-  __shared__ float s_w[4];
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  float v = 0.f;
-  if (p < npix) {
-    const float4 q = *reinterpret_cast<const float4*>(img + p * 4);
-    const float d0 = q.x - tgt[p * 3], d1 = q.y - tgt[p * 3 + 1], d2 = q.z - tgt[p * 3 + 2];
-    v = d0 * d0 + d1 * d1 + d2 * d2;
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) partial[blockIdx.x] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
-}
+// per-work-group partial sums written by the forward kernel (a shuffle tree
+// per wavefront, then the 4 wavefronts in order) and one fixed-order final
+// sum in double (deterministic).
 
 __global__ __launch_bounds__(256) void l2_finish_kernel(const float* __restrict__ partial, int n, float weight,
                                                         float* __restrict__ loss) {
@@ -1212,6 +1232,51 @@ __global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* __r
 // Pixels of the N x V images of a fused render.
 int64_t render_pixels(const int64_t* density_shape, int num_cameras, const int32_t* output_dhw) {
   return density_shape[0] * num_cameras * static_cast<int64_t>(output_dhw[1]) * output_dhw[2];
+}
+
+// The fused density render forward.  bricked_in: a haloed brick copy of
+// density made by the caller with this boundary (nullptr: made here in the
+// workspace when it is large enough).  l2_target / l2_partial: the fused L2
+// loss's target and one partial per forward work-group (both or neither).
+int render_density_fwd_impl(const float* density, const int64_t* density_shape, const float* light_rgb,
+                            float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
+                            const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
+                            int boundary, int blending_mode, float* output, void* workspace, size_t workspace_bytes,
+                            const float* bricked_in, const float* l2_target, float* l2_partial, ngt_stream_t stream) {
+  std::vector<Pair> pairs;
+  int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
+                          boundary, blending_mode, 1, pairs, true);
+  if (rc) return rc;
+  NGT_REQUIRE(light_rgb != nullptr, NGT_ERR_INVALID_ARGUMENT, "light_rgb must be given");
+  RayArgs a = make_ray_args(density, density_shape, output_dhw);
+  a.light_r = light_rgb[0]; a.light_g = light_rgb[1]; a.light_b = light_rgb[2];
+  a.depth_step = depth_step;
+  hipStream_t s = as_stream(stream);
+  const size_t need = ngt_render_density_fwd_workspace_size(density_shape, interpolation);
+  const bool bricked = bricked_in != nullptr || (need > 0 && workspace != nullptr && workspace_bytes >= need);
+  if (bricked_in) {
+    a.bricked = bricked_in;
+  } else if (bricked) {
+    launch_to_bricked(density, a.din, a.bd, a.n_data, 1, boundary, static_cast<float*>(workspace), s);
+    a.bricked = static_cast<const float*>(workspace);
+  }
+  a.l2_target = l2_target;
+  int64_t partial_base = 0;
+  for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
+    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    a.l2_partial = l2_partial ? l2_partial + partial_base : nullptr;
+    partial_base += static_cast<int64_t>(grid.x) * grid.y * grid.z;
+    if (bricked) {
+      NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<4, LINEAR, BB, MM, true, true>), grid, dim3(256), 0, s, a, cb,
+                             output)));
+    } else {
+      NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<4, FF, BB, MM, true, false>), grid, dim3(256), 0, s, a, cb,
+                             output))));
+    }
+  });
+  return check_launch("ngt_render_density_fwd");
 }
 
 }  // namespace
@@ -1338,34 +1403,9 @@ int ngt_render_density_fwd(const float* density, const int64_t* density_shape, c
                            const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
                            int boundary, int blending_mode, float* output, void* workspace, size_t workspace_bytes,
                            ngt_stream_t stream) {
-  std::vector<Pair> pairs;
-  int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
-                          boundary, blending_mode, 1, pairs, true);
-  if (rc) return rc;
-  NGT_REQUIRE(light_rgb != nullptr, NGT_ERR_INVALID_ARGUMENT, "light_rgb must be given");
-  RayArgs a = make_ray_args(density, density_shape, output_dhw);
-  a.light_r = light_rgb[0]; a.light_g = light_rgb[1]; a.light_b = light_rgb[2];
-  a.depth_step = depth_step;
-  hipStream_t s = as_stream(stream);
-  const size_t need = ngt_render_density_fwd_workspace_size(density_shape, interpolation);
-  const bool bricked = need > 0 && workspace != nullptr && workspace_bytes >= need;
-  if (bricked) {
-    launch_to_bricked(density, a.din, a.bd, a.n_data, 1, boundary, static_cast<float*>(workspace), s);
-    a.bricked = static_cast<const float*>(workspace);
-  }
-  for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
-    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
-    if (bricked) {
-      NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_fwd_kernel<4, LINEAR, BB, MM, true, true>), grid, dim3(256), 0, s, a, cb,
-                             output)));
-    } else {
-      NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_fwd_kernel<4, FF, BB, MM, true, false>), grid, dim3(256), 0, s, a, cb,
-                             output))));
-    }
-  });
-  return check_launch("ngt_render_density_fwd");
+  return render_density_fwd_impl(density, density_shape, light_rgb, depth_step, matrix_m, matrix_v, matrix_p, frustum,
+                                 num_cameras, output_dhw, interpolation, boundary, blending_mode, output, workspace,
+                                 workspace_bytes, nullptr, nullptr, nullptr, stream);
 }
 
 size_t ngt_render_density_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
@@ -1407,36 +1447,59 @@ int ngt_render_density_bwd(const float* density, const int64_t* density_shape, c
   return check_launch("ngt_render_density_bwd");
 }
 
+size_t ngt_bricked_grid_bytes(const int64_t* shape) {
+  if (shape == nullptr || check_grid_shape(shape, "grid") != NGT_OK) return 0;
+  return bricked_bytes(shape, static_cast<int>(shape[4]));
+}
+
+int ngt_bricked_grid(const float* grid, const int64_t* shape, int boundary, float* bricked, ngt_stream_t stream) {
+  int rc = check_grid_shape(shape, "grid");
+  if (rc) return rc;
+  NGT_REQUIRE(grid != nullptr && bricked != nullptr, NGT_ERR_INVALID_ARGUMENT, "grid and bricked must be given");
+  NGT_REQUIRE(boundary >= 0 && boundary <= 2, NGT_ERR_INVALID_ARGUMENT, "invalid boundary mode");
+  const int C = static_cast<int>(shape[4]);
+  NGT_REQUIRE(C == 1 || C == 2 || C == 4, NGT_ERR_UNIMPLEMENTED, "Only 1,2 and 4 Channel supported.");
+  NGT_REQUIRE(bricked_bytes(shape, C) > 0, NGT_ERR_INVALID_ARGUMENT,
+              "grid too large for the bricked layout (a slice must hold fewer than 2^32 values)");
+  const Dims d = shape5(shape).dims();
+  launch_to_bricked(grid, d, brick_dims(d), static_cast<int>(shape[0]), C, boundary, bricked, as_stream(stream));
+  return check_launch("ngt_bricked_grid");
+}
+
+// partial sums of the fused L2 loss: one per forward work-group (16 x 16 pixels)
+static int64_t l2_partials(const int64_t* density_shape, int num_cameras, const int32_t* output_dhw) {
+  return density_shape[0] * num_cameras * static_cast<int64_t>((output_dhw[2] + 15) / 16) * ((output_dhw[1] + 15) / 16);
+}
+
 size_t ngt_render_density_l2_fwd_workspace_size(const int64_t* density_shape, int interpolation, int num_cameras,
                                                 const int32_t* output_dhw) {
   if (density_shape == nullptr || output_dhw == nullptr || num_cameras <= 0) return 0;
-  const int64_t blocks = (render_pixels(density_shape, num_cameras, output_dhw) + 255) / 256;
   return align_up(ngt_render_density_fwd_workspace_size(density_shape, interpolation)) +
-         align_up(static_cast<size_t>(blocks) * sizeof(float));
+         align_up(static_cast<size_t>(l2_partials(density_shape, num_cameras, output_dhw)) * sizeof(float));
 }
 
 int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
                               float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
                               const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
-                              int boundary, int blending_mode, const float* target, float loss_weight, float* output,
-                              float* loss, void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
+                              int boundary, int blending_mode, const float* target, float loss_weight,
+                              const float* bricked, float* output, float* loss, void* workspace,
+                              size_t workspace_bytes, ngt_stream_t stream) {
   NGT_REQUIRE(target != nullptr && loss != nullptr, NGT_ERR_INVALID_ARGUMENT, "target and loss must be given");
   NGT_REQUIRE(output_dhw != nullptr && num_cameras > 0 && density_shape != nullptr, NGT_ERR_INVALID_ARGUMENT,
               "Invalid output_shape");
   const size_t need = ngt_render_density_l2_fwd_workspace_size(density_shape, interpolation, num_cameras, output_dhw);
   NGT_REQUIRE(workspace != nullptr && workspace_bytes >= need, NGT_ERR_WORKSPACE,
               "workspace too small for ngt_render_density_l2_fwd");
-  const size_t bricked = align_up(ngt_render_density_fwd_workspace_size(density_shape, interpolation));
-  int rc = ngt_render_density_fwd(density, density_shape, light_rgb, depth_step, matrix_m, matrix_v, matrix_p, frustum,
-                                  num_cameras, output_dhw, interpolation, boundary, blending_mode, output,
-                                  bricked ? workspace : nullptr, bricked, stream);
+  NGT_REQUIRE(bricked == nullptr || interpolation == LINEAR, NGT_ERR_INVALID_ARGUMENT,
+              "a bricked copy serves LINEAR interpolation only");
+  const size_t brick_ws = align_up(ngt_render_density_fwd_workspace_size(density_shape, interpolation));
+  float* partial = reinterpret_cast<float*>(static_cast<char*>(workspace) + brick_ws);
+  int rc = render_density_fwd_impl(density, density_shape, light_rgb, depth_step, matrix_m, matrix_v, matrix_p,
+                                   frustum, num_cameras, output_dhw, interpolation, boundary, blending_mode, output,
+                                   brick_ws ? workspace : nullptr, brick_ws, bricked, target, partial, stream);
   if (rc) return rc;
-  const int64_t npix = render_pixels(density_shape, num_cameras, output_dhw);
-  const unsigned blocks = blocks_for(npix, 256);
-  float* partial = reinterpret_cast<float*>(static_cast<char*>(workspace) + bricked);
-  hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(l2_partial_kernel, dim3(blocks), dim3(256), 0, s, output, target, npix, partial);
-  hipLaunchKernelGGL(l2_finish_kernel, dim3(1), dim3(256), 0, s, partial, static_cast<int>(blocks), loss_weight, loss);
+  hipLaunchKernelGGL(l2_finish_kernel, dim3(1), dim3(256), 0, as_stream(stream), partial,
+                     static_cast<int>(l2_partials(density_shape, num_cameras, output_dhw)), loss_weight, loss);
   return check_launch("ngt_render_density_l2_fwd");
 }
 
@@ -1454,8 +1517,8 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
                               float depth_step, const float* output, const float* target, float loss_weight,
                               const float* loss_grad, const float* matrix_m, const float* matrix_v,
                               const float* matrix_p, const float* frustum, int num_cameras, const int32_t* output_dhw,
-                              int interpolation, int boundary, int blending_mode, float* density_grad,
-                              void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
+                              int interpolation, int boundary, int blending_mode, const float* bricked,
+                              float* density_grad, void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
   std::vector<Pair> pairs;
   int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
                           boundary, blending_mode, 1, pairs, true);
@@ -1483,7 +1546,7 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
   if (gather && plan_gather(pairs, matrix_m, matrix_v, matrix_p, frustum, a, interpolation, boundary, 1, workspace,
                             workspace_bytes, gp)) {
     run_gather<true>(pairs, matrix_m, matrix_v, matrix_p, frustum, a, blending_mode, 4, output, nullptr, gp,
-                     density_grad, static_cast<int>(density_shape[0]), s);
+                     density_grad, static_cast<int>(density_shape[0]), s, bricked);
     return check_launch("ngt_render_density_l2_bwd");
   }
   NGT_REQUIRE(workspace_bytes >= img_need, NGT_ERR_WORKSPACE, "workspace too small for ngt_render_density_l2_bwd");

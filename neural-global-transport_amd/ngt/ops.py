@@ -527,12 +527,22 @@ class _RenderDensityL2(torch.autograd.Function):
         loss = torch.empty((), dtype=torch.float32, device=density.device)
         lp, _, keep = _lib.host_floats(light_rgb, 3)
         s_d = _lib.shape5(density)
+        # one haloed brick copy of the density serves the forward and the backward
+        bricked = None
+        if interp == INTERPOLATION["LINEAR"] and use_bricked_layout:
+            nb = lib.ngt_bricked_grid_bytes(s_d)
+            if nb > 0:
+                bricked, _ = _lib.workspace(nb, density.device)
+                _lib.check(lib.ngt_bricked_grid(_lib.ptr(density), s_d, boundary, _lib.ptr(bricked),
+                                                _lib.stream(density.device)))
         ws, nbytes = _lib.workspace(lib.ngt_render_density_l2_fwd_workspace_size(s_d, interp, cams.nv,
                                                                                  _lib.dhw(out_shape)), density.device)
         _lib.check(lib.ngt_render_density_l2_fwd(
             _lib.ptr(density), s_d, lp, float(depth_step), cams.m, cams.v, cams.p, cams.f, cams.nv,
-            _lib.dhw(out_shape), interp, boundary, mode, _lib.ptr(target), float(weight), _lib.ptr(out),
-            _lib.ptr(loss), _lib.ptr(ws), nbytes, _lib.stream(density.device)))
+            _lib.dhw(out_shape), interp, boundary, mode, _lib.ptr(target), float(weight),
+            None if bricked is None else _lib.ptr(bricked), _lib.ptr(out), _lib.ptr(loss), _lib.ptr(ws), nbytes,
+            _lib.stream(density.device)))
+        ctx.bricked = bricked
         ctx.save_for_backward(density, target, out)
         ctx.args = (light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight)
         ctx.mark_non_differentiable(out)
@@ -556,10 +566,12 @@ class _RenderDensityL2(torch.autograd.Function):
         else:
             nb = out.numel() * 4
         ws, nbytes = _lib.workspace(nb, density.device)
+        bricked = ctx.bricked
         _lib.check(lib.ngt_render_density_l2_bwd(
             _lib.ptr(density), s_d, lp, float(depth_step), _lib.ptr(out), _lib.ptr(target), float(weight),
             _lib.ptr(gl), cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape), interp, boundary, mode,
-            _lib.ptr(g), _lib.ptr(ws), nbytes, _lib.stream(density.device)))
+            None if bricked is None else _lib.ptr(bricked), _lib.ptr(g), _lib.ptr(ws), nbytes,
+            _lib.stream(density.device)))
         return g, None, None, None, None, None, None, None, None, None
 
 

@@ -109,3 +109,59 @@ def test_fused_l2_rejects_bad_target(ops):
     with pytest.raises(ValueError, match="target must be"):
         ops.render_density_fused_l2(rho, torch.zeros((1, 3) + out_dhw[1:] + (4,), device="cuda:0"), (1, 1, 1), ds,
                                     M, V, P, F, out_dhw)
+
+
+@pytest.mark.parametrize("views", [3, 9])
+def test_fused_l2_shared_brick_copy_equals_own_copy(ops, views):
+    """The op makes one haloed brick copy (ngt_bricked_grid) for the forward
+    and the backward; the C ABI called without it (each call making its own
+    copy) gives bit-identical images and loss, and the same gradient up to
+    summation order (the gather adjoint ranks samples within a bin by LDS
+    atomics, so the order of its sums varies between runs).  9 views take two
+    camera launches, so the per-work-group loss partials of both count."""
+    from ngt import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(70 + views)
+    dims = (12, 11, 13)
+    (M, V, P, F), ds, out_dhw = _scene(dims, views=views)
+    rho = dev(rng.uniform(0.0, 0.5, (1,) + dims + (1,)))
+    tgt = dev(rng.uniform(0.0, 0.3, (1, views) + out_dhw[1:] + (3,)))
+    rt = rho.clone().requires_grad_()
+    loss, img = ops.render_density_fused_l2(rt, tgt, (0.9, 0.8, 0.7), ds, M, V, P, F, out_dhw, loss_weight=0.5)
+    loss.backward()
+    cams = ops._Cams(M, V, P, F)
+    lp, _, keep = _lib.host_floats((0.9, 0.8, 0.7), 3)
+    s_d = _lib.shape5(rho)
+    od = _lib.dhw(out_dhw)
+    st = _lib.stream(rho.device)
+    img2 = torch.empty_like(img)
+    loss2 = torch.empty((), device="cuda:0")
+    fws, fnb = _lib.workspace(lib.ngt_render_density_l2_fwd_workspace_size(s_d, 1, cams.nv, od), rho.device)
+    _lib.check(lib.ngt_render_density_l2_fwd(_lib.ptr(rho), s_d, lp, float(ds), cams.m, cams.v, cams.p, cams.f,
+                                             cams.nv, od, 1, 0, 0, _lib.ptr(tgt), 0.5, None, _lib.ptr(img2),
+                                             _lib.ptr(loss2), _lib.ptr(fws), fnb, st))
+    g2 = torch.empty_like(rho)
+    one = torch.ones((), device="cuda:0")
+    bws, bnb = _lib.workspace(lib.ngt_render_density_l2_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f,
+                                                                           cams.nv, od, 1, 0), rho.device)
+    _lib.check(lib.ngt_render_density_l2_bwd(_lib.ptr(rho), s_d, lp, float(ds), _lib.ptr(img2), _lib.ptr(tgt), 0.5,
+                                             _lib.ptr(one), cams.m, cams.v, cams.p, cams.f, cams.nv, od, 1, 0, 0,
+                                             None, _lib.ptr(g2), _lib.ptr(bws), bnb, st))
+    torch.cuda.synchronize()
+    assert torch.equal(img, img2)
+    assert torch.equal(loss, loss2)
+    assert rel(host(rt.grad), host(g2)) < 1e-6
+    # and the loss against the oracle
+    cat = np.concatenate([host(rho) * np.array([0.9, 0.8, 0.7]), host(rho)], -1) * ds
+    eimg = RM.raymarch_grid_transform(cat, M, V, P, F, out_dhw, LINEAR, BORDER, B.BEER_LAMBERT)
+    eloss = 0.5 * float(np.sum((eimg[..., :3] - host(tgt)) ** 2))
+    assert abs(float(loss) - eloss) / abs(eloss) < FWD_TOL
+
+
+def test_bricked_grid_rejects_three_channels(ops):
+    from ngt import _lib
+    lib = _lib.load()
+    x = torch.zeros((1, 4, 4, 4, 3), device="cuda:0")
+    out = torch.empty(1 << 16, dtype=torch.uint8, device="cuda:0")
+    rc = lib.ngt_bricked_grid(_lib.ptr(x), _lib.shape5(x), 0, _lib.ptr(out), _lib.stream(x.device))
+    assert rc != 0 and b"Channel" in lib.ngt_last_error()
