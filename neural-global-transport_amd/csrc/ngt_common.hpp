@@ -21,7 +21,7 @@
 namespace ngt {
 
 enum Boundary : int { BORDER = 0, CLAMP = 1, WRAP = 2 };
-enum Filter : int { NEAREST = 0, LINEAR = 1, FMIN = 2, FMAX = 3 };
+enum Filter : int { NEAREST = 0, LINEAR = 1, FMIN = 2, FMAX = 3, QUADRATIC = 4 };
 enum BlendMode : int { BEER_LAMBERT = 0, ALPHA = 2, ALPHA_ADDITIVE = 3, ADDITIVE = 4 };
 
 constexpr float kFloatMax = 3.402823466e+38f;
@@ -454,9 +454,134 @@ NGT_DEVICE void read_minmax2(const float* __restrict__ g, const Dims& d, float q
   }
 }
 
+// ---------------------------------------------------- quadratic B-spline ---
+// 3x3x3 stencil (sampling.hpp:247-365): per axis the cells centre-1, centre,
+// centre+1 with centre = floor(q + 0.5), weighted by the quadratic B-spline
+// of their distance |q - i|: 3/4 - d^2 below 1/2, (3/2 - d)^2 / 2 below 3/2.
+NGT_DEVICE float bspline2(float d) {
+  return d < 0.5f ? 0.75f - d * d : (d < 1.5f ? 0.5f * (1.5f - d) * (1.5f - d) : 0.f);
+}
+
+struct QuadAxis {
+  int i[3];    // unfixed cell indices
+  float w[3];  // weights
+};
+
+NGT_DEVICE QuadAxis quad_axis(float q) {
+  QuadAxis a;
+  const int c = static_cast<int>(floorf(q + 0.5f));
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    a.i[t] = c - 1 + t;
+    a.w[t] = bspline2(fabsf(q - static_cast<float>(a.i[t])));
+  }
+  return a;
+}
+
+template <int BM>
+NGT_DEVICE int quad_fix(int i, int m) {
+  if constexpr (BM == CLAMP) return min(max(i, 0), m - 1);
+  else return wrap_index(i, m);
+}
+
+// read3DQuadratic: BORDER returns 0 everywhere (the reference returns before
+// reading, sampling.hpp:284-287); CLAMP / WRAP fix the indices and sum along
+// x, then y, then z.
+template <int C, int BM>
+NGT_DEVICE Vec<C> read_quadratic(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz) {
+  if constexpr (BM == BORDER) {
+    return vzero<C>();
+  } else {
+    const QuadAxis ax = quad_axis(qx), ay = quad_axis(qy), az = quad_axis(qz);
+    int ix[3], iy[3], iz[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      ix[t] = quad_fix<BM>(ax.i[t], d.x); iy[t] = quad_fix<BM>(ay.i[t], d.y); iz[t] = quad_fix<BM>(az.i[t], d.z);
+    }
+    Vec<C> r = vzero<C>();
+#pragma unroll
+    for (int tz = 0; tz < 3; ++tz) {
+      Vec<C> vz = vzero<C>();
+#pragma unroll
+      for (int ty = 0; ty < 3; ++ty) {
+        Vec<C> vy = vzero<C>();
+#pragma unroll
+        for (int tx = 0; tx < 3; ++tx) {
+          const Vec<C> v = vload<C>(g + cell_offset(d, ix[tx], iy[ty], iz[tz]) * C);
+#pragma unroll
+          for (int i = 0; i < C; ++i) vy.v[i] += ax.w[tx] * v.v[i];
+        }
+#pragma unroll
+        for (int i = 0; i < C; ++i) vz.v[i] += ay.w[ty] * vy.v[i];
+      }
+#pragma unroll
+      for (int i = 0; i < C; ++i) r.v[i] += az.w[tz] * vz.v[i];
+    }
+    return r;
+  }
+}
+
+// Exact adjoint of read_quadratic (nothing for BORDER, whose read is 0).
+template <int C, int BM>
+NGT_DEVICE void scatter_quadratic(float* __restrict__ g, const Dims& d, float qx, float qy, float qz,
+                                  const Vec<C>& val) {
+  if constexpr (BM != BORDER) {
+    const QuadAxis ax = quad_axis(qx), ay = quad_axis(qy), az = quad_axis(qz);
+#pragma unroll
+    for (int tz = 0; tz < 3; ++tz)
+#pragma unroll
+      for (int ty = 0; ty < 3; ++ty)
+#pragma unroll
+        for (int tx = 0; tx < 3; ++tx) {
+          const float w = ax.w[tx] * ay.w[ty] * az.w[tz];
+          const int64_t o = cell_offset(d, quad_fix<BM>(ax.i[tx], d.x), quad_fix<BM>(ay.i[ty], d.y),
+                                        quad_fix<BM>(az.i[tz], d.z));
+#pragma unroll
+          for (int i = 0; i < C; ++i) atomic_add(g + o * C + i, val.v[i] * w);
+        }
+  }
+}
+
+// scatterGradQuadratic (resample_grid.cu.cc:904-1080) with hit counts, for
+// the reference gradient semantics.  BORDER: positions with -1.5 < q < dims
+// scatter to the stencil cells whose centre-1 indices are >= 0 and centre+1
+// indices < dims, never to the centre-centre-centre cell; the reference does
+// not check the centre indices, which lie outside the grid for q + 0.5 < 0
+// or q + 0.5 >= dims, and those cells are skipped here.
+template <int C, int BM>
+NGT_DEVICE void scatter_quadratic_reference(float* __restrict__ g, const Dims& d, float qx, float qy, float qz,
+                                            const Vec<C>& val, uint32_t* __restrict__ hits) {
+  const QuadAxis ax = quad_axis(qx), ay = quad_axis(qy), az = quad_axis(qz);
+  if constexpr (BM == BORDER) {
+    if (!(qx > -1.5f && qy > -1.5f && qz > -1.5f && qx < static_cast<float>(d.x) && qy < static_cast<float>(d.y) &&
+          qz < static_cast<float>(d.z)))
+      return;
+  }
+#pragma unroll
+  for (int tz = 0; tz < 3; ++tz)
+#pragma unroll
+    for (int ty = 0; ty < 3; ++ty)
+#pragma unroll
+      for (int tx = 0; tx < 3; ++tx) {
+        int ix = ax.i[tx], iy = ay.i[ty], iz = az.i[tz];
+        if constexpr (BM == BORDER) {
+          if (tx == 1 && ty == 1 && tz == 1) continue;
+          if (ix < 0 || ix >= d.x || iy < 0 || iy >= d.y || iz < 0 || iz >= d.z) continue;
+        } else {
+          ix = quad_fix<BM>(ix, d.x); iy = quad_fix<BM>(iy, d.y); iz = quad_fix<BM>(iz, d.z);
+        }
+        const float w = ax.w[tx] * ay.w[ty] * az.w[tz];
+        const int64_t o = cell_offset(d, ix, iy, iz);
+#pragma unroll
+        for (int i = 0; i < C; ++i) atomic_add(g + o * C + i, val.v[i] * w);
+        atomicAdd(hits + o, 1u);
+      }
+}
+
 template <int C, int FM, int BM>
 NGT_DEVICE Vec<C> read_filtered(const float* __restrict__ g, const Dims& d, float qx, float qy, float qz) {
   if constexpr (FM == LINEAR) return read_linear<C, BM>(g, d, qx, qy, qz);
+  else if constexpr (FM == QUADRATIC) return read_quadratic<C, BM>(g, d, qx, qy, qz);
   else if constexpr (FM == NEAREST) return read_nearest<C, BM>(g, d, qx, qy, qz);
   else if constexpr (FM == FMIN) return read_minmax<C, BM, false>(g, d, qx, qy, qz);
   else return read_minmax<C, BM, true>(g, d, qx, qy, qz);
@@ -577,6 +702,7 @@ template <int C, int FM, int BM>
 NGT_DEVICE void scatter_filtered(const float* __restrict__ src, float* __restrict__ g, const Dims& d, float qx,
                                  float qy, float qz, const Vec<C>& val) {
   if constexpr (FM == LINEAR) scatter_linear<C, BM, false>(g, d, qx, qy, qz, val);
+  else if constexpr (FM == QUADRATIC) scatter_quadratic<C, BM>(g, d, qx, qy, qz, val);
   else if constexpr (FM == NEAREST) scatter_nearest<C, BM>(g, d, qx, qy, qz, val);
   else if constexpr (FM == FMIN) scatter_minmax<C, BM, false>(src, g, d, qx, qy, qz, val);
   else scatter_minmax<C, BM, true>(src, g, d, qx, qy, qz, val);

@@ -448,6 +448,8 @@ __global__ __launch_bounds__(256) void xform_fwd_kernel(XformArgs a, CamBatch cb
     r = read_mip<C, FM, BM, MM>(mips, cp.data_index, px, py, pz, lod, a.mip_bias, 0.5f);
   } else if constexpr (FM == LINEAR) {
     r = read_linear<C, BM>(a.input + cp.data_index * in_slice, a.din, px - 0.5f, py - 0.5f, pz - 0.5f);
+  } else if constexpr (FM == QUADRATIC) {
+    r = read_quadratic<C, BM>(a.input + cp.data_index * in_slice, a.din, px - 0.5f, py - 0.5f, pz - 0.5f);
   } else {
     r = read_nearest<C, BM>(a.input + cp.data_index * in_slice, a.din, px - 0.5f, py - 0.5f, pz - 0.5f);
   }
@@ -473,15 +475,22 @@ __global__ __launch_bounds__(256) void xform_bwd_kernel(XformArgs a, CamBatch cb
   const Vec<C> dy = vload<C>(out_grad + (static_cast<int64_t>(cp.out_index) * out_cells + cell_offset(a.dout, x, y, z)) * C);
   float* g = in_grad + cp.data_index * in_cells * C;
   if constexpr (REF) {
-    scatter_linear<C, BM, true>(g, a.din, px - 0.5f, py - 0.5f, pz - 0.5f, dy, hits + cp.data_index * in_cells);
+    // kScatter3DGradients: the quadratic scatter for QUADRATIC, the trilinear
+    // one for every other filter (resample_grid.cu.cc:1160-1185)
+    if constexpr (FM == QUADRATIC)
+      scatter_quadratic_reference<C, BM>(g, a.din, px - 0.5f, py - 0.5f, pz - 0.5f, dy,
+                                         hits + cp.data_index * in_cells);
+    else
+      scatter_linear<C, BM, true>(g, a.din, px - 0.5f, py - 0.5f, pz - 0.5f, dy, hits + cp.data_index * in_cells);
   } else if constexpr (BM != WRAP && MM != NGT_MIP_NONE) {
     const float lod = xform_lod(a, cp, ix, iy, iz, px, py, pz);
     const MipPick k = mip_pick<MM>(lod, a.mip_bias, grad.num_levels);
-    // the transform filters need no level data for their scatter (LINEAR, NEAREST)
+    // the transform filters need no level data for their scatter (LINEAR, NEAREST, QUADRATIC)
     scatter_level<C, FM, BM>(grad, grad, cp.data_index, k.lf, px, py, pz, 0.5f, vscale(dy, 1.f - k.wc));
     if (k.two) scatter_level<C, FM, BM>(grad, grad, cp.data_index, k.lc, px, py, pz, 0.5f, vscale(dy, k.wc));
   } else if constexpr (BM != WRAP) {
     if constexpr (FM == LINEAR) scatter_linear<C, BM, false>(g, a.din, px - 0.5f, py - 0.5f, pz - 0.5f, dy);
+    else if constexpr (FM == QUADRATIC) scatter_quadratic<C, BM>(g, a.din, px - 0.5f, py - 0.5f, pz - 0.5f, dy);
     else scatter_nearest<C, BM>(g, a.din, px - 0.5f, py - 0.5f, pz - 0.5f, dy);
   }
 }
@@ -543,6 +552,7 @@ __global__ __launch_bounds__(256) void lod_transform_kernel(XformArgs a, CamBatc
 #define NGT_SWITCH_FM2(F_, ...)                                      \
   switch (F_) {                                                      \
     case LINEAR: { constexpr int FF = LINEAR; __VA_ARGS__; } break;  \
+    case QUADRATIC: { constexpr int FF = QUADRATIC; __VA_ARGS__; } break; \
     default: { constexpr int FF = NEAREST; __VA_ARGS__; } break;     \
   }
 #define NGT_SWITCH_MM(M_, ...)                                                   \
@@ -645,8 +655,8 @@ int check_cams(const int64_t* in_shape, const float* M, const float* V, const fl
 
 int check_xform_attrs(int interpolation, int boundary, int coordinate_mode, float cell_center_offset,
                       const char* op) {
-  NGT_REQUIRE(interpolation == LINEAR || interpolation == NEAREST, NGT_ERR_UNIMPLEMENTED,
-              std::string(op) + " supports NEAREST and LINEAR interpolation");
+  NGT_REQUIRE(interpolation == LINEAR || interpolation == NEAREST || interpolation == QUADRATIC,
+              NGT_ERR_INVALID_ARGUMENT, std::string(op) + " supports NEAREST, LINEAR and QUADRATIC interpolation");
   NGT_REQUIRE(boundary == BORDER || boundary == CLAMP, NGT_ERR_INVALID_ARGUMENT, "invalid boundary");
   NGT_REQUIRE(cell_center_offset == 0.5f, NGT_ERR_INVALID_ARGUMENT,
               "Invalid cell center offset for given coordinate mode.");
@@ -868,7 +878,11 @@ int ngt_sample_grid_transform_bwd(const float* input, const int64_t* input_shape
     const CamBatch cb = cam_batch(pairs, b, matrix_m, matrix_v, matrix_p, frustum);
     const dim3 grid(blocks_for(a.dout.x, kBX), blocks_for(a.dout.y, kBY), static_cast<unsigned>(a.dout.z * cb.count));
     const dim3 block(kBX, kBY);
-    if (ref) {
+    if (ref && interpolation == QUADRATIC) {
+      NGT_SWITCH_C(si.c, NGT_SWITCH_BM(boundary,
+          hipLaunchKernelGGL((xform_bwd_kernel<CC, QUADRATIC, BB, true, NGT_MIP_NONE>), grid, block, 0, s, a, cb,
+                             grad, output_grad, input_grad, hits)));
+    } else if (ref) {
       NGT_SWITCH_C(si.c, NGT_SWITCH_BM(boundary,
           hipLaunchKernelGGL((xform_bwd_kernel<CC, LINEAR, BB, true, NGT_MIP_NONE>), grid, block, 0, s, a, cb, grad,
                              output_grad, input_grad, hits)));

@@ -185,8 +185,6 @@ class _Cams:
 def _xform_attrs(interpolation, boundary, mipmapping, num_mipmaps, mip_bias, coordinate_mode, cell_center_offset,
                  separate_camera_batch):
     interp = _enum(INTERPOLATION, interpolation, "interpolation")
-    if interp == 4:
-        raise NotImplementedError("QUADRATIC interpolation is not implemented")
     return dict(interpolation=interp, boundary=_enum(BOUNDARY, boundary, "boundary"),
                 mipmapping=_enum(MIPMAPPING, mipmapping, "mipmapping"), num_mipmaps=int(num_mipmaps),
                 mip_bias=float(mip_bias), coordinate_mode=_enum(COORDINATE_MODE, coordinate_mode, "coordinate_mode"),

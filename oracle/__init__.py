@@ -34,7 +34,7 @@ finite differences, adjoint identities) is listed in tests/test_oracle_*.py.
 """
 
 BORDER, CLAMP, WRAP = 0, 1, 2
-NEAREST, LINEAR, MIN, MAX = 0, 1, 2, 3
+NEAREST, LINEAR, MIN, MAX, QUADRATIC = 0, 1, 2, 3, 4
 
 BOUNDARY = {"BORDER": BORDER, "CLAMP": CLAMP, "WRAP": WRAP}
 FILTER = {"NEAREST": NEAREST, "LINEAR": LINEAR, "MIN": MIN, "MAX": MAX}

@@ -23,7 +23,7 @@ Test infrastructure only (see ``oracle/__init__.py``).
 """
 import numpy as np
 
-from . import LINEAR, NEAREST, MIN, MAX
+from . import LINEAR, NEAREST, MIN, MAX, QUADRATIC
 from . import sampling as S
 
 MIP_NONE, MIP_NEAREST, MIP_LINEAR = 0, 1, 2
@@ -128,6 +128,8 @@ def scatter_levels(level_shapes, pos, g, lod_raw, mip_mode, bias, cco, filter_mo
         q = pos / float(1 << m) - cco
         if filter_mode == LINEAR:
             S.scatter_linear(level_shapes[m], q, gm, boundary, mask="forward", out=grads[m])
+        elif filter_mode == QUADRATIC:
+            S.scatter_quadratic(level_shapes[m], q, gm, boundary, mask="forward", out=grads[m])
         elif filter_mode == NEAREST:
             S.scatter_nearest(level_shapes[m], q, gm, boundary, out=grads[m])
         else:

@@ -10,7 +10,7 @@ else ``[N, 1, D', H', W', C]`` with instance n using camera n
 """
 import numpy as np
 
-from . import BORDER, LINEAR, NEAREST, MIN, MAX
+from . import BORDER, LINEAR, NEAREST, MIN, MAX, QUADRATIC
 from . import sampling as S
 from . import geometry as G
 from . import mips as MP
@@ -228,11 +228,13 @@ def _transform_pairs(N, V, separate_camera_batch):
 def _sample2(grid, q, filter_mode, boundary):
     """Sampler2 (sampling.hpp:802-826): BORDER/CLAMP only, WRAP reads 0;
     modes other than LINEAR / QUADRATIC fall to NEAREST."""
-    from . import WRAP
+    from . import WRAP, QUADRATIC
     if boundary == WRAP:
         return np.zeros(q.shape[:-1] + (grid.shape[-1],))
     if filter_mode == LINEAR:
         return S.sample_linear(grid, q, boundary)
+    if filter_mode == QUADRATIC:
+        return S.sample_quadratic(grid, q, boundary)
     return S.sample_nearest(grid, q, boundary)
 
 
@@ -287,7 +289,7 @@ def sample_grid_transform_grad(inp, out_grad, M, V, P, F, filter_mode=LINEAR, bo
             pos = transform_positions(cx, out_dhw, inp.shape[1:4], coordinate_mode)
             lod = transform_lod(cx, out_dhw, inp.shape[1:4], coordinate_mode)
             levels = MP.generate_mips(inp[n], num_mipmaps)
-            fm = LINEAR if filter_mode == LINEAR else NEAREST
+            fm = filter_mode if filter_mode in (LINEAR, QUADRATIC) else NEAREST
             gl = MP.scatter_levels([l.shape for l in levels], pos, out_grad[n, vo], lod, mipmapping, mip_bias, 0.5,
                                    fm, boundary)
             in_grad[n] += MP.collapse_adjoint(gl)
@@ -308,12 +310,19 @@ def sample_grid_transform_grad(inp, out_grad, M, V, P, F, filter_mode=LINEAR, bo
             q = transform_positions(cx, out_dhw, inp.shape[1:4], coordinate_mode) - 0.5
             g = out_grad[n, vo]
             if semantics == "reference":
-                S.scatter_linear(inp.shape[1:], q, g, boundary, mask="reference", count=count, out=in_grad[n])
+                # kScatter3DGradients: the quadratic scatter for QUADRATIC, the
+                # trilinear one for every other filter (resample_grid.cu.cc:1160-1185)
+                if filter_mode == QUADRATIC:
+                    S.scatter_quadratic(inp.shape[1:], q, g, boundary, mask="reference", count=count, out=in_grad[n])
+                else:
+                    S.scatter_linear(inp.shape[1:], q, g, boundary, mask="reference", count=count, out=in_grad[n])
             else:
                 if boundary == WRAP:
                     continue
                 if filter_mode == LINEAR:
                     S.scatter_linear(inp.shape[1:], q, g, boundary, mask="forward", out=in_grad[n])
+                elif filter_mode == QUADRATIC:
+                    S.scatter_quadratic(inp.shape[1:], q, g, boundary, mask="forward", out=in_grad[n])
                 else:
                     S.scatter_nearest(inp.shape[1:], q, g, boundary, out=in_grad[n])
         if semantics == "reference":

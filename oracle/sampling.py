@@ -150,10 +150,98 @@ def sample_minmax(grid, q, boundary, is_max):
     return acc
 
 
+def bspline2_weight(dist):
+    """Quadratic B-spline weight of a cell at distance ``dist`` from the
+    sample (sampling.hpp:247-256): 3/4 - d^2 below 1/2, (3/2 - d)^2 / 2 below
+    3/2, else 0."""
+    d = np.asarray(dist, dtype=np.float64)
+    return np.where(d < 0.5, 0.75 - d * d, np.where(d < 1.5, 0.5 * (1.5 - d) ** 2, 0.0))
+
+
+def quadratic_stencil(q, dims):
+    """The 3x3x3 stencil of ``read3DQuadratic`` (sampling.hpp:268-283):
+    per axis the (unfixed) indices centre-1, centre, centre+1 with
+    centre = floor(q + 0.5), and their weights from the distances |q - i|."""
+    centre = np.floor(q + 0.5).astype(np.int64)
+    idx = [centre - 1, centre, centre + 1]
+    w = [bspline2_weight(np.abs(q - i)) for i in idx]
+    return idx, w
+
+
+def sample_quadratic(grid, q, boundary):
+    """read3DQuadratic (sampling.hpp:268-365): BORDER returns 0 everywhere
+    (the reference's early return, :284-287); CLAMP / WRAP fix the indices
+    and interpolate along x, then y, then z."""
+    dims = dims_xyz(grid.shape)
+    C = grid.shape[-1]
+    if boundary == BORDER:
+        return np.zeros(q.shape[:-1] + (C,))
+    idx, w = quadratic_stencil(q, dims)
+    idx = [_fix_index(i, dims, boundary) for i in idx]
+    ones = np.ones(q.shape[:-1], dtype=bool)
+    out = 0.0
+    for cz in range(3):
+        vz = 0.0
+        for cy in range(3):
+            vy = 0.0
+            for cx in range(3):
+                val = _gather(grid, idx[cx][..., 0], idx[cy][..., 1], idx[cz][..., 2], ones)
+                vy = vy + w[cx][..., 0:1] * val
+            vz = vz + w[cy][..., 1:2] * vy
+        out = out + w[cz][..., 2:3] * vz
+    return out
+
+
+def scatter_quadratic(grid_shape, q, g, boundary, mask="forward", count=None, out=None):
+    """Adjoint of ``sample_quadratic`` w.r.t. the grid values.
+
+    mask="forward": the exact adjoint (nothing for BORDER, whose forward is 0).
+    mask="reference": scatterGradQuadratic (resample_grid.cu.cc:904-1080):
+      BORDER positions with -1.5 < q < dims scatter to the stencil cells whose
+      'prev' indices are >= 0 and 'next' indices < dims, skipping the
+      centre-centre-centre cell; its 'centre' indices are not checked there,
+      which writes out of range for q+0.5 < 0 or q+0.5 >= dims, so here
+      those cells are skipped.  ``count`` receives one hit per scattered cell.
+    """
+    D, H, W = grid_shape[:3]
+    C = g.shape[-1]
+    if out is None:
+        out = np.zeros((D, H, W, C))
+    dims = dims_xyz(grid_shape)
+    if boundary == BORDER and mask == "forward":
+        return out
+    idx, w = quadratic_stencil(q, dims)
+    if boundary == BORDER:
+        pos_ok = np.all((q > -1.5) & (q < dims), axis=-1)
+    else:
+        pos_ok = np.ones(q.shape[:-1], dtype=bool)
+        idx = [_fix_index(i, dims, boundary) for i in idx]
+    for cz in range(3):
+        for cy in range(3):
+            for cx in range(3):
+                ix, iy, iz = idx[cx][..., 0], idx[cy][..., 1], idx[cz][..., 2]
+                v = pos_ok.copy()
+                if boundary == BORDER:
+                    if (cx, cy, cz) == (1, 1, 1):
+                        continue
+                    for c_, i_, m in ((cx, ix, dims[0]), (cy, iy, dims[1]), (cz, iz, dims[2])):
+                        if c_ == 0:
+                            v &= i_ >= 0
+                        elif c_ == 2:
+                            v &= i_ < m
+                        else:
+                            v &= (i_ >= 0) & (i_ < m)
+                wt = w[cx][..., 0] * w[cy][..., 1] * w[cz][..., 2]
+                _accumulate(out, count, ix, iy, iz, v, g * wt[..., None])
+    return out
+
+
 def sample(grid, q, filter_mode, boundary):
-    from . import NEAREST, LINEAR, MIN, MAX
+    from . import NEAREST, LINEAR, MIN, MAX, QUADRATIC
     if filter_mode == LINEAR:
         return sample_linear(grid, q, boundary)
+    if filter_mode == QUADRATIC:
+        return sample_quadratic(grid, q, boundary)
     if filter_mode == NEAREST:
         return sample_nearest(grid, q, boundary)
     if filter_mode == MIN:
