@@ -506,41 +506,54 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   const int b0z = (brem / (bricks_x * bricks_y)) * BZ;
   const int b1x = min(b0x + BX, a.din.x) - 1, b1y = min(b0y + BY, a.din.y) - 1, b1z = min(b0z + BZ, a.din.z) - 1;
 
-  // pixel rectangles of the views (one work-item per view)
-  if (tid < cb.count) {
-    const CamParams& cp = cb.cam[tid];
-    int x0 = 0, y0 = 0, nx = 0, ny = 0;
-    if (cp.data_index == n) {
-      float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
-      bool behind = false;
-      for (int c = 0; c < 8; ++c) {
-        const float px = (c & 1) ? static_cast<float>(b1x) + 1.5f : static_cast<float>(b0x) - 0.5f;
-        const float py = (c & 2) ? static_cast<float>(b1y) + 1.5f : static_cast<float>(b0y) - 0.5f;
-        const float pz = (c & 4) ? static_cast<float>(b1z) + 1.5f : static_cast<float>(b0z) - 0.5f;
-        float vx, vy, vz, vw;
-        mat4_apply(cp.mv, px, py, pz, 1.f, vx, vy, vz, vw);
-        float cx, cy, cz, cw;
-        mat4_apply(cp.p, vx, vy, vz, vw, cx, cy, cz, cw);
-        if (!(cw > 0.f)) { behind = true; continue; }
+  // pixel rectangles of the views: wavefront 0 projects the 8 corners of the
+  // support box for up to 8 views at once (lane = view * 8 + corner), then
+  // reduces each view's 8 lanes with shuffles
+  if (tid < 64) {
+    const int ci = tid >> 3, c = tid & 7;
+    const bool view = ci < cb.count && cb.cam[ci < cb.count ? ci : 0].data_index == n;
+    float umin = 3.0e38f, umax = -3.0e38f, vmin = 3.0e38f, vmax = -3.0e38f;
+    int behind = 0;
+    if (view) {
+      const CamParams& cp = cb.cam[ci];
+      const float px = (c & 1) ? static_cast<float>(b1x) + 1.5f : static_cast<float>(b0x) - 0.5f;
+      const float py = (c & 2) ? static_cast<float>(b1y) + 1.5f : static_cast<float>(b0y) - 0.5f;
+      const float pz = (c & 4) ? static_cast<float>(b1z) + 1.5f : static_cast<float>(b0z) - 0.5f;
+      float vx, vy, vz, vw;
+      mat4_apply(cp.mv, px, py, pz, 1.f, vx, vy, vz, vw);
+      float cx, cy, cz, cw;
+      mat4_apply(cp.p, vx, vy, vz, vw, cx, cy, cz, cw);
+      if (!(cw > 0.f)) {
+        behind = 1;
+      } else {
         const float iw = 1.f / cw;
-        const float u = (cx * iw * 0.5f + 0.5f) * static_cast<float>(a.out_w) - 0.5f;
-        const float w = (cy * iw * 0.5f + 0.5f) * static_cast<float>(a.out_h) - 0.5f;
-        umin = fminf(umin, u); umax = fmaxf(umax, u);
-        vmin = fminf(vmin, w); vmax = fmaxf(vmax, w);
+        umin = umax = (cx * iw * 0.5f + 0.5f) * static_cast<float>(a.out_w) - 0.5f;
+        vmin = vmax = (cy * iw * 0.5f + 0.5f) * static_cast<float>(a.out_h) - 0.5f;
       }
-      int x1 = a.out_w - 1, y1 = a.out_h - 1;
-      if (!behind) {
-        // clamp before the float -> int conversions (far-off corners project anywhere)
-        const float fw = static_cast<float>(a.out_w), fh = static_cast<float>(a.out_h);
-        umin = fminf(fmaxf(umin, -2.f), fw + 1.f); umax = fminf(fmaxf(umax, -2.f), fw + 1.f);
-        vmin = fminf(fmaxf(vmin, -2.f), fh + 1.f); vmax = fminf(fmaxf(vmax, -2.f), fh + 1.f);
-        x0 = max(0, static_cast<int>(ceilf(umin - 1e-3f))); x1 = min(x1, static_cast<int>(floorf(umax + 1e-3f)));
-        y0 = max(0, static_cast<int>(ceilf(vmin - 1e-3f))); y1 = min(y1, static_cast<int>(floorf(vmax + 1e-3f)));
-      }
-      nx = max(0, x1 - x0 + 1);
-      ny = max(0, y1 - y0 + 1);
     }
-    rect[tid][0] = x0; rect[tid][1] = y0; rect[tid][2] = nx; rect[tid][3] = ny;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+      umin = fminf(umin, __shfl_xor(umin, o, 64)); umax = fmaxf(umax, __shfl_xor(umax, o, 64));
+      vmin = fminf(vmin, __shfl_xor(vmin, o, 64)); vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+      behind |= __shfl_xor(behind, o, 64);
+    }
+    if (c == 0 && ci < cb.count) {
+      int x0 = 0, y0 = 0, nx = 0, ny = 0;
+      if (view) {
+        int x1 = a.out_w - 1, y1 = a.out_h - 1;
+        if (!behind) {
+          // clamp before the float -> int conversions (far-off corners project anywhere)
+          const float fw = static_cast<float>(a.out_w), fh = static_cast<float>(a.out_h);
+          umin = fminf(fmaxf(umin, -2.f), fw + 1.f); umax = fminf(fmaxf(umax, -2.f), fw + 1.f);
+          vmin = fminf(fmaxf(vmin, -2.f), fh + 1.f); vmax = fminf(fmaxf(vmax, -2.f), fh + 1.f);
+          x0 = max(0, static_cast<int>(ceilf(umin - 1e-3f))); x1 = min(x1, static_cast<int>(floorf(umax + 1e-3f)));
+          y0 = max(0, static_cast<int>(ceilf(vmin - 1e-3f))); y1 = min(y1, static_cast<int>(floorf(vmax + 1e-3f)));
+        }
+        nx = max(0, x1 - x0 + 1);
+        ny = max(0, y1 - y0 + 1);
+      }
+      rect[ci][0] = x0; rect[ci][1] = y0; rect[ci][2] = nx; rect[ci][3] = ny;
+    }
   }
   __syncthreads();
   if (tid == 0) {
