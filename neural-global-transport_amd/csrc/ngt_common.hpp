@@ -261,14 +261,6 @@ NGT_DEVICE uint32_t brick_corner(const Dims& d, const BrickDims& b, int x0, int 
 // Corner k (bit0 = x-ceil, bit1 = y-ceil, bit2 = z-ceil) relative to the floor corner.
 NGT_DEVICE constexpr uint32_t brick_corner_step(int k) { return (k & 1) + ((k >> 1) & 1) * 5 + (k >> 2) * 25; }
 
-// Cell (x, y, z) of the grid, x etc. in [-1, d], of brick-local slot l.
-NGT_DEVICE void brick_slot_cell(const BrickDims& b, uint32_t brick, uint32_t l, int& x, int& y, int& z) {
-  const uint32_t bz = brick / b.nbxy, r = brick - bz * b.nbxy;
-  const uint32_t by = r / b.nbx, bx = r - by * b.nbx;
-  const uint32_t k = l / 25u, j = (l / 5u) % 5u, i = l % 5u;
-  x = static_cast<int>(4u * bx + i) - 1; y = static_cast<int>(4u * by + j) - 1; z = static_cast<int>(4u * bz + k) - 1;
-}
-
 NGT_DEVICE bool corner_ok(const Corners& c, int k) {
   return ((k & 1) ? c.vx1 : c.vx0) & ((k & 2) ? c.vy1 : c.vy0) & ((k & 4) ? c.vz1 : c.vz0);
 }

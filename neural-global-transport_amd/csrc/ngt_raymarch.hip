@@ -929,33 +929,41 @@ void for_cam_batches(const std::vector<Pair>& pairs, const float* M, const float
   }
 }
 
-// Copy of an NDHWC grid in the haloed brick layout: one work-item per stored
-// slot (two bricks per work-group), so the writes are coalesced; halo slots
-// read the boundary mode's value (0 for BORDER, the clamped or wrapped cell
-// otherwise); the 3 padding slots of a brick are written as 0.
+// Copy of an NDHWC grid in the haloed brick layout.  A work-group writes one
+// row of bricks (all bx for one (by, bz)), 256 stored slots per pass, so its
+// writes are one contiguous run and the source rows it reads (5 y x 5 z rows
+// of the grid) are fetched once by one work-group instead of by the work-groups
+// of neighbouring bricks on other XCDs.  Halo slots read the boundary mode's
+// value (0 for BORDER, the clamped or wrapped cell otherwise); the 3 padding
+// slots of a brick are written as 0.
 template <int C, int BM>
 NGT_DEVICE void to_bricked_kernel_body(const VBlock& vb, const float* __restrict__ in, Dims d, BrickDims b,
                                        float* __restrict__ out) {
-  const uint32_t t = static_cast<uint32_t>(vb.x) * 256u + threadIdx.x;
-  if (t >= b.slice_cells) return;
   const int n = vb.y;
-  Vec<C> v = vzero<C>();
-  const uint32_t l = t % kBrickFloats;
-  if (l < 125u) {
-    int x, y, z;
-    brick_slot_cell(b, t / kBrickFloats, l, x, y, z);
-    bool ok = true;
-    if constexpr (BM == BORDER) {
-      ok = x >= 0 && x < d.x && y >= 0 && y < d.y && z >= 0 && z < d.z;
-    } else if constexpr (BM == CLAMP) {
-      x = min(max(x, 0), d.x - 1); y = min(max(y, 0), d.y - 1); z = min(max(z, 0), d.z - 1);
-    } else {
-      x = wrap_index(x, d.x); y = wrap_index(y, d.y); z = wrap_index(z, d.z);
+  const uint32_t row = static_cast<uint32_t>(vb.x);  // by + nby * bz
+  const uint32_t nby = b.nbxy / b.nbx;
+  const uint32_t by = row % nby, bz = row / nby;
+  const uint32_t first = row * b.nbx * kBrickFloats, count = b.nbx * kBrickFloats;
+  const int64_t cells = static_cast<int64_t>(d.x) * d.y * d.z;
+  for (uint32_t t = threadIdx.x; t < count; t += 256u) {
+    const uint32_t l = t % kBrickFloats, bx = t / kBrickFloats;
+    Vec<C> v = vzero<C>();
+    if (l < 125u) {
+      const uint32_t k = l / 25u, j = (l / 5u) % 5u, i = l % 5u;
+      int x = static_cast<int>(4u * bx + i) - 1, y = static_cast<int>(4u * by + j) - 1,
+          z = static_cast<int>(4u * bz + k) - 1;
+      bool ok = true;
+      if constexpr (BM == BORDER) {
+        ok = x >= 0 && x < d.x && y >= 0 && y < d.y && z >= 0 && z < d.z;
+      } else if constexpr (BM == CLAMP) {
+        x = min(max(x, 0), d.x - 1); y = min(max(y, 0), d.y - 1); z = min(max(z, 0), d.z - 1);
+      } else {
+        x = wrap_index(x, d.x); y = wrap_index(y, d.y); z = wrap_index(z, d.z);
+      }
+      if (ok) v = vload<C>(in + (n * cells + cell_offset(d, x, y, z)) * C);
     }
-    const int64_t cells = static_cast<int64_t>(d.x) * d.y * d.z;
-    if (ok) v = vload<C>(in + (n * cells + cell_offset(d, x, y, z)) * C);
+    vstore<C>(out + (static_cast<int64_t>(n) * b.slice_cells + first + t) * C, v);
   }
-  vstore<C>(out + (static_cast<int64_t>(n) * b.slice_cells + t) * C, v);
 }
 
 template <int C, int BM>
@@ -964,9 +972,12 @@ __global__ __launch_bounds__(256) void to_bricked_kernel(const float* __restrict
   to_bricked_kernel_body<C, BM>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), 0}, in, d, b, out);
 }
 
+// work-groups of a brick copy: one per row of bricks
+unsigned brick_rows(const BrickDims& b) { return b.slice_cells / kBrickFloats / b.nbx; }
+
 void launch_to_bricked(const float* in, const Dims& d, const BrickDims& b, int n_data, int C, int boundary, float* out,
                        hipStream_t s) {
-  const dim3 grid(blocks_for(b.slice_cells, 256), static_cast<unsigned>(n_data));
+  const dim3 grid(brick_rows(b), static_cast<unsigned>(n_data));
   NGT_SWITCH_C(C, NGT_SWITCH_BM(boundary,
       hipLaunchKernelGGL((to_bricked_kernel<CC, BB>), grid, dim3(256), 0, s, in, d, b, out)));
 }
@@ -1159,7 +1170,7 @@ struct RenBatchItem {
   CamBatch cb;
   int ntx, nty;        // 16 x 16 pixel tiles per view
   int kcap;
-  int brick_tiles;     // 256-slot tiles of the bricked copy
+  int brick_tiles;     // brick rows of the bricked copy (one work-group each)
   float* images;
   const float* images_grad;
   float* density_grad;
@@ -1630,7 +1641,7 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     it.ntx = static_cast<int>(blocks_for(it.a.out_w, 16));
     it.nty = static_cast<int>(blocks_for(it.a.out_h, 16));
     NGT_REQUIRE(bricked_fits(it.a.din, 1), NGT_ERR_INVALID_ARGUMENT, "batch density grid too large");
-    it.brick_tiles = static_cast<int>(blocks_for(it.a.bd.slice_cells, 256));
+    it.brick_tiles = static_cast<int>(brick_rows(it.a.bd));
     it.images = q.images; it.images_grad = q.images_grad; it.density_grad = q.density_grad;
     const size_t npix = static_cast<size_t>(it.a.out_w) * it.a.out_h;
     it.ws_bricked = static_cast<int64_t>(ws);
