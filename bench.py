@@ -67,8 +67,8 @@ OP_KERNELS = {
     "advect_fwd": ["advect_sl_kernel", "advect_mc_kernel"],
     "advect_bwd": ["advect_mc_bwd_brick_kernel", "advect_sl_bwd_brick_kernel", "advect_rho_gather_kernel",
                    "faces_from_centered_grad_kernel", "fill_kernel"],
-    "render_fwd": ["to_bricked_kernel", "raymarch_fwd_kernel", "l2_finish_kernel"],
-    "render_bwd": ["raymarch_bwd_prepass_kernel", "raymarch_bwd_brick_kernel"],
+    "render_fwd": ["to_bricked_kernel", "raymarch_fwd_record_kernel", "l2_finish_kernel"],
+    "render_bwd": ["raymarch_bwd_brick_kernel"],
 }
 COPY_BYTES = 256 << 20
 
@@ -277,11 +277,11 @@ def op_calls(torch, prob):
     brk = torch.empty(brk_b, dtype=torch.uint8, device=rho.device)  # the forward's brick copy, reused backward
     P = _lib.ptr
 
-    def render_fwd():
+    def render_fwd():  # brick copy, render + loss, and the recorded per-sample gradients
         _lib.check(lib.ngt_bricked_grid(P(rho), s_rho, 0, P(brk), st()))
         _lib.check(lib.ngt_render_density_l2_fwd(
             P(rho), s_rho, lp, prob["ds"], cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1, 0, 0, P(tgt), 1.0,
-            P(brk), P(img), P(loss), P(fws), fws_b, st()))
+            P(brk), P(rws), rws_b, P(img), P(loss), P(fws), fws_b, st()))
 
     calls = {
         "advect_fwd": lambda: _lib.check(lib.ngt_advect_fwd(
@@ -292,7 +292,7 @@ def op_calls(torch, prob):
         "render_fwd": render_fwd,
         "render_bwd": lambda: _lib.check(lib.ngt_render_density_l2_bwd(
             P(rho), s_rho, lp, prob["ds"], P(img), P(tgt), 1.0, P(loss_grad), cams.m, cams.v, cams.p, cams.f,
-            cams.nv, out_dhw, 1, 0, 0, P(brk), P(g_adv), P(rws), rws_b, st())),
+            cams.nv, out_dhw, 1, 0, 0, P(brk), 1, P(g_adv), P(rws), rws_b, st())),
     }
     # every raw buffer handed to the C ABI has the element count its entry
     # point reads (the ABI takes plain pointers and cannot check them)

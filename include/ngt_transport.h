@@ -242,15 +242,20 @@ int ngt_bricked_grid(const float* grid, const int64_t* shape, int boundary, floa
  * instead of being read from memory.  `bricked` (nullable): the haloed brick
  * copy of density made by ngt_bricked_grid with the same boundary; when
  * given, neither call makes its own copy (the workspace sizes still assume
- * NULL). */
+ * NULL).  `record` (nullable, LINEAR and BORDER only): a buffer of
+ * ngt_render_density_l2_bwd_workspace_size bytes in which the forward also
+ * records each ray and its per-sample gradients for a unit loss gradient,
+ * walking each ray a second time right after the first; _l2_bwd with
+ * `recorded` = 1 and that buffer as its workspace then only gathers them
+ * (scaled by *loss_grad). */
 size_t ngt_render_density_l2_fwd_workspace_size(const int64_t* density_shape, int interpolation, int num_cameras,
                                                 const int32_t* output_dhw);
 int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
                               float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
                               const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
                               int boundary, int blending_mode, const float* target, float loss_weight,
-                              const float* bricked, float* output, float* loss, void* workspace,
-                              size_t workspace_bytes, ngt_stream_t stream);
+                              const float* bricked, void* record, size_t record_bytes, float* output, float* loss,
+                              void* workspace, size_t workspace_bytes, ngt_stream_t stream);
 size_t ngt_render_density_l2_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
                                                 const float* matrix_v, const float* matrix_p, const float* frustum,
                                                 int num_cameras, const int32_t* output_dhw, int interpolation,
@@ -260,7 +265,8 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
                               const float* loss_grad, const float* matrix_m, const float* matrix_v,
                               const float* matrix_p, const float* frustum, int num_cameras, const int32_t* output_dhw,
                               int interpolation, int boundary, int blending_mode, const float* bricked,
-                              float* density_grad, void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+                              int recorded, float* density_grad, void* workspace, size_t workspace_bytes,
+                              ngt_stream_t stream);
 
 /* ---- AdvectGridSemiLangrange (advect.cc:21-71) with VelocityGrid.warp
  * semantics (data_structures.py:1690-1737) --------------------------------

@@ -41,7 +41,7 @@ struct RayArgs {
   // gradient is 2 * l2_scale * l2_loss_grad[0] * (image - target) on RGB and
   // 0 on the optical depth, formed in the gradient pass instead of read
   const float* l2_target;     // [N*V, H, W, 3]
-  const float* l2_loss_grad;  // one float on the device
+  const float* l2_loss_grad;  // one float on the device (nullptr: 1)
   float l2_scale;
   // fused L2 image loss (render forward): when set, each work-group of the
   // forward writes the sum over its pixels and RGB of (image - target)^2 to
@@ -403,7 +403,7 @@ struct RayRec {
 // channels, tpix: its first target value).
 template <int C>
 NGT_DEVICE Vec<C> l2_image_grad(const RayArgs& a, const float* __restrict__ img, int64_t tpix) {
-  const float s = 2.f * a.l2_scale * a.l2_loss_grad[0];
+  const float s = 2.f * a.l2_scale * (a.l2_loss_grad ? a.l2_loss_grad[0] : 1.f);
   const Vec<C> v = vload<C>(img);
   Vec<C> g = vzero<C>();
 #pragma unroll
@@ -458,6 +458,20 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
   raymarch_bwd_prepass_kernel_body<C, BM, MODE, LIGHT, BRICKED>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, out, out_grad, G, rays, kcap);
 }
 
+// The fused render + L2 forward that also records what the gather adjoint
+// needs: each ray's record and its per-sample gradients of the loss for a
+// unit upstream gradient (the backward scales the volume gradient by the
+// actual one).  A ray's second walk runs right after its first, while its
+// samples are still in the caches.
+template <int MODE>
+__global__ __launch_bounds__(256) void raymarch_fwd_record_kernel(RayArgs a, CamBatch cb, float* __restrict__ out,
+                                                                  int pair_base, float* __restrict__ G,
+                                                                  RayRec* __restrict__ rays, int kcap) {
+  const VBlock vb{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)};
+  raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(vb, a, cb, out);
+  raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(vb, a, cb, pair_base, out, nullptr, G, rays, kcap);
+}
+
 // Inclusive scan of one int per work-item over a 256-wide work-group: a
 // shuffle scan inside each wavefront, then the wavefront totals through LDS
 // (one barrier).  The caller synchronises before s_wtot is written again.
@@ -489,7 +503,8 @@ template <int CG, int BX, int BY, int BZ>
 NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
                                                                  const RayRec* __restrict__ rays,
                                                                  const float* __restrict__ G, int kcap,
-                                                                 float* __restrict__ in_grad, int accumulate) {
+                                                                 float* __restrict__ in_grad, int accumulate,
+                                                                 const float* __restrict__ grad_scale) {
   constexpr int NV = BX * BY * BZ;
   static_assert(NV == 512, "each of the 256 work-items owns two voxels of the brick");
   __shared__ float acc[NV * CG];
@@ -781,6 +796,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   }
   __syncthreads();
   const int64_t base = static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z;
+  const float scale = grad_scale ? grad_scale[0] : 1.f;
   for (int i = tid; i < NV; i += blockDim.x) {
     const int lx = i % BX, ly = (i / BX) % BY, lz = i / (BX * BY);
     const int x = b0x + lx, y = b0y + ly, z = b0z + lz;
@@ -788,7 +804,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
     float* d = in_grad + (base + cell_offset(a.din, x, y, z)) * CG;
 #pragma unroll
     for (int c = 0; c < CG; ++c) {
-      const float v = acc[i * CG + c];
+      const float v = acc[i * CG + c] * scale;
       d[c] = accumulate ? d[c] + v : v;
     }
   }
@@ -798,8 +814,9 @@ template <int CG, int BX, int BY, int BZ>
 __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamBatch cb, int pair_base,
                                                                  const RayRec* __restrict__ rays,
                                                                  const float* __restrict__ G, int kcap,
-                                                                 float* __restrict__ in_grad, int accumulate) {
-  raymarch_bwd_brick_kernel_body<CG, BX, BY, BZ>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, rays, G, kcap, in_grad, accumulate);
+                                                                 float* __restrict__ in_grad, int accumulate,
+                                                                 const float* __restrict__ grad_scale) {
+  raymarch_bwd_brick_kernel_body<CG, BX, BY, BZ>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, rays, G, kcap, in_grad, accumulate, grad_scale);
 }
 
 // Reference gradient of one camera: reverse walk from `end` with the
@@ -1089,12 +1106,18 @@ bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V,
 
 // bricked_in: a haloed brick copy of the input made by the caller (BORDER
 // halo), or nullptr to make one in the plan's workspace.
+// recorded: the plan's rays and per-sample gradients were written by the
+// forward (raymarch_fwd_record_kernel) for a unit loss gradient; only the
+// gather runs, scaled by *grad_scale.
 template <bool LIGHT>
 void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* F,
                 const RayArgs& a_in, int blending_mode, int C, const float* output, const float* output_grad,
-                const GatherPlan& gp, float* in_grad, int n_data, hipStream_t s, const float* bricked_in = nullptr) {
+                const GatherPlan& gp, float* in_grad, int n_data, hipStream_t s, const float* bricked_in = nullptr,
+                bool recorded = false, const float* grad_scale = nullptr) {
   RayArgs a = a_in;
-  if (bricked_in) {
+  if (recorded) {
+    a.bricked = nullptr;  // the gather reads no grid values
+  } else if (bricked_in) {
     a.bricked = bricked_in;
   } else {
     launch_to_bricked(a.input, a.din, a.bd, n_data, LIGHT ? 1 : C, BORDER, gp.bricked, s);
@@ -1109,7 +1132,9 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
     }
     const dim3 rgrid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
     const int base = static_cast<int>(b);
-    if constexpr (LIGHT) {
+    if (recorded) {
+      // per-sample gradients recorded by the forward
+    } else if constexpr (LIGHT) {
       NGT_SWITCH_MODE(blending_mode,
           hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<4, BORDER, MM, true, true>), rgrid, dim3(256), 0, s, a, cb,
                              base, output, output_grad, gp.G, gp.rays, gp.kcap));
@@ -1124,7 +1149,7 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
     const int CG = LIGHT ? 1 : C;
     NGT_SWITCH_C(CG,
         hipLaunchKernelGGL((raymarch_bwd_brick_kernel<CC, BX, BY, BZ>), dim3(bricks * n_data), dim3(256), 0, s, a, cb,
-                           base, gp.rays, gp.G, gp.kcap, in_grad, accumulate));
+                           base, gp.rays, gp.G, gp.kcap, in_grad, accumulate, recorded ? grad_scale : nullptr));
   }
 }
 
@@ -1248,7 +1273,7 @@ __global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* __r
     raymarch_bwd_brick_kernel_body<1, 8, 8, 8>(VBlock{t - v.brick[i], 0, 0}, a, it.cb, 0,
                                                reinterpret_cast<const RayRec*>(ws + it.ws_rays),
                                                reinterpret_cast<const float*>(ws + it.ws_G), it.kcap,
-                                               it.density_grad, 0);
+                                               it.density_grad, 0, nullptr);
     __syncthreads();  // the brick's LDS arrays are rewritten by the next brick
   }
 }
@@ -1266,7 +1291,8 @@ int render_density_fwd_impl(const float* density, const int64_t* density_shape, 
                             float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
                             const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
                             int boundary, int blending_mode, float* output, void* workspace, size_t workspace_bytes,
-                            const float* bricked_in, const float* l2_target, float* l2_partial, ngt_stream_t stream) {
+                            const float* bricked_in, const float* l2_target, float* l2_partial, ngt_stream_t stream,
+                            void* record = nullptr, size_t record_bytes = 0, float l2_scale = 0.f) {
   std::vector<Pair> pairs;
   int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
                           boundary, blending_mode, 1, pairs, true);
@@ -1285,12 +1311,29 @@ int render_density_fwd_impl(const float* density, const int64_t* density_shape, 
     a.bricked = static_cast<const float*>(workspace);
   }
   a.l2_target = l2_target;
+  a.l2_scale = l2_scale;  // the recorded image gradient 2 * l2_scale * (image - target)
+  // record: the backward's gather plan filled by the forward (fused L2 only)
+  GatherPlan gp;
+  if (record) {
+    NGT_REQUIRE(l2_target != nullptr && bricked && interpolation == LINEAR && boundary == BORDER,
+                NGT_ERR_INVALID_ARGUMENT, "recording the per-sample gradients needs LINEAR, BORDER and the L2 loss");
+    NGT_REQUIRE(plan_gather(pairs, matrix_m, matrix_v, matrix_p, frustum, a, interpolation, boundary, 1, record,
+                            record_bytes, gp),
+                NGT_ERR_WORKSPACE, "record buffer too small (ngt_render_density_l2_bwd_workspace_size)");
+  }
   int64_t partial_base = 0;
+  int pair_base = 0;
   for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
     const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
     a.l2_partial = l2_partial ? l2_partial + partial_base : nullptr;
     partial_base += static_cast<int64_t>(grid.x) * grid.y * grid.z;
-    if (bricked) {
+    const int base = pair_base;
+    pair_base += cb.count;
+    if (record) {
+      NGT_SWITCH_MODE(blending_mode,
+          hipLaunchKernelGGL((raymarch_fwd_record_kernel<MM>), grid, dim3(256), 0, s, a, cb, output, base, gp.G,
+                             gp.rays, gp.kcap));
+    } else if (bricked) {
       NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
           hipLaunchKernelGGL((raymarch_fwd_kernel<4, LINEAR, BB, MM, true, true>), grid, dim3(256), 0, s, a, cb,
                              output)));
@@ -1506,8 +1549,8 @@ int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape
                               float depth_step, const float* matrix_m, const float* matrix_v, const float* matrix_p,
                               const float* frustum, int num_cameras, const int32_t* output_dhw, int interpolation,
                               int boundary, int blending_mode, const float* target, float loss_weight,
-                              const float* bricked, float* output, float* loss, void* workspace,
-                              size_t workspace_bytes, ngt_stream_t stream) {
+                              const float* bricked, void* record, size_t record_bytes, float* output, float* loss,
+                              void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
   NGT_REQUIRE(target != nullptr && loss != nullptr, NGT_ERR_INVALID_ARGUMENT, "target and loss must be given");
   NGT_REQUIRE(output_dhw != nullptr && num_cameras > 0 && density_shape != nullptr, NGT_ERR_INVALID_ARGUMENT,
               "Invalid output_shape");
@@ -1520,7 +1563,8 @@ int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape
   float* partial = reinterpret_cast<float*>(static_cast<char*>(workspace) + brick_ws);
   int rc = render_density_fwd_impl(density, density_shape, light_rgb, depth_step, matrix_m, matrix_v, matrix_p,
                                    frustum, num_cameras, output_dhw, interpolation, boundary, blending_mode, output,
-                                   brick_ws ? workspace : nullptr, brick_ws, bricked, target, partial, stream);
+                                   brick_ws ? workspace : nullptr, brick_ws, bricked, target, partial, stream,
+                                   record, record_bytes, loss_weight);
   if (rc) return rc;
   hipLaunchKernelGGL(l2_finish_kernel, dim3(1), dim3(256), 0, as_stream(stream), partial,
                      static_cast<int>(l2_partials(density_shape, num_cameras, output_dhw)), loss_weight, loss);
@@ -1542,7 +1586,8 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
                               const float* loss_grad, const float* matrix_m, const float* matrix_v,
                               const float* matrix_p, const float* frustum, int num_cameras, const int32_t* output_dhw,
                               int interpolation, int boundary, int blending_mode, const float* bricked,
-                              float* density_grad, void* workspace, size_t workspace_bytes, ngt_stream_t stream) {
+                              int recorded, float* density_grad, void* workspace, size_t workspace_bytes,
+                              ngt_stream_t stream) {
   std::vector<Pair> pairs;
   int rc = check_ray_args(density_shape, matrix_m, matrix_v, matrix_p, frustum, num_cameras, output_dhw, interpolation,
                           boundary, blending_mode, 1, pairs, true);
@@ -1570,9 +1615,10 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
   if (gather && plan_gather(pairs, matrix_m, matrix_v, matrix_p, frustum, a, interpolation, boundary, 1, workspace,
                             workspace_bytes, gp)) {
     run_gather<true>(pairs, matrix_m, matrix_v, matrix_p, frustum, a, blending_mode, 4, output, nullptr, gp,
-                     density_grad, static_cast<int>(density_shape[0]), s, bricked);
+                     density_grad, static_cast<int>(density_shape[0]), s, bricked, recorded != 0, loss_grad);
     return check_launch("ngt_render_density_l2_bwd");
   }
+  NGT_REQUIRE(!recorded, NGT_ERR_WORKSPACE, "the recorded per-sample gradients need the gather workspace");
   NGT_REQUIRE(workspace_bytes >= img_need, NGT_ERR_WORKSPACE, "workspace too small for ngt_render_density_l2_bwd");
   float* dimg = static_cast<float*>(workspace);
   hipLaunchKernelGGL(l2_image_grad_kernel, dim3(blocks_for(npix, 256)), dim3(256), 0, s, a, output, npix, dimg);

@@ -535,12 +535,22 @@ class _RenderDensityL2(torch.autograd.Function):
                                                 _lib.stream(density.device)))
         ws, nbytes = _lib.workspace(lib.ngt_render_density_l2_fwd_workspace_size(s_d, interp, cams.nv,
                                                                                  _lib.dhw(out_shape)), density.device)
+        # with a gradient to come, the forward also records the gather adjoint's
+        # rays and per-sample gradients (LINEAR, BORDER)
+        record, rbytes = None, 0
+        if (ctx.needs_input_grad[0] and use_gather_adjoint and bricked is not None
+                and boundary == BOUNDARY["BORDER"]):
+            rb = lib.ngt_render_density_l2_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv,
+                                                              _lib.dhw(out_shape), interp, boundary)
+            if rb > 0:
+                record, rbytes = _lib.workspace(rb, density.device)
         _lib.check(lib.ngt_render_density_l2_fwd(
             _lib.ptr(density), s_d, lp, float(depth_step), cams.m, cams.v, cams.p, cams.f, cams.nv,
             _lib.dhw(out_shape), interp, boundary, mode, _lib.ptr(target), float(weight),
-            None if bricked is None else _lib.ptr(bricked), _lib.ptr(out), _lib.ptr(loss), _lib.ptr(ws), nbytes,
-            _lib.stream(density.device)))
+            None if bricked is None else _lib.ptr(bricked), None if record is None else _lib.ptr(record), rbytes,
+            _lib.ptr(out), _lib.ptr(loss), _lib.ptr(ws), nbytes, _lib.stream(density.device)))
         ctx.bricked = bricked
+        ctx.record = (record, rbytes)
         ctx.save_for_backward(density, target, out)
         ctx.args = (light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight)
         ctx.mark_non_differentiable(out)
@@ -558,18 +568,22 @@ class _RenderDensityL2(torch.autograd.Function):
         g = _lib.empty_grad(density)
         lp, _, keep = _lib.host_floats(light_rgb, 3)
         s_d = _lib.shape5(density)
-        if use_gather_adjoint:
-            nb = lib.ngt_render_density_l2_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv,
-                                                              _lib.dhw(out_shape), interp, boundary)
+        record, rbytes = ctx.record
+        if record is not None:
+            ws, nbytes = record, rbytes
         else:
-            nb = out.numel() * 4
-        ws, nbytes = _lib.workspace(nb, density.device)
+            if use_gather_adjoint:
+                nb = lib.ngt_render_density_l2_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv,
+                                                                  _lib.dhw(out_shape), interp, boundary)
+            else:
+                nb = out.numel() * 4
+            ws, nbytes = _lib.workspace(nb, density.device)
         bricked = ctx.bricked
         _lib.check(lib.ngt_render_density_l2_bwd(
             _lib.ptr(density), s_d, lp, float(depth_step), _lib.ptr(out), _lib.ptr(target), float(weight),
             _lib.ptr(gl), cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape), interp, boundary, mode,
-            None if bricked is None else _lib.ptr(bricked), _lib.ptr(g), _lib.ptr(ws), nbytes,
-            _lib.stream(density.device)))
+            None if bricked is None else _lib.ptr(bricked), 0 if record is None else 1, _lib.ptr(g), _lib.ptr(ws),
+            nbytes, _lib.stream(density.device)))
         return g, None, None, None, None, None, None, None, None, None
 
 

@@ -114,11 +114,14 @@ def test_fused_l2_rejects_bad_target(ops):
 @pytest.mark.parametrize("views", [3, 9])
 def test_fused_l2_shared_brick_copy_equals_own_copy(ops, views):
     """The op makes one haloed brick copy (ngt_bricked_grid) for the forward
-    and the backward; the C ABI called without it (each call making its own
-    copy) gives bit-identical images and loss, and the same gradient up to
-    summation order (the gather adjoint ranks samples within a bin by LDS
-    atomics, so the order of its sums varies between runs).  9 views take two
-    camera launches, so the per-work-group loss partials of both count."""
+    and the backward and has the forward record the per-sample gradients;
+    the C ABI called without either (each call making its own copy, the
+    backward walking the rays itself) gives bit-identical images and loss,
+    and the same gradient up to summation order (the gather adjoint ranks
+    samples within a bin by LDS atomics, so the order of its sums varies
+    between runs) and the unit-gradient recording (the loss gradient applied
+    at the end).  9 views take two camera launches, so the per-work-group
+    loss partials and recorded rays of both count."""
     from ngt import _lib
     lib = _lib.load()
     rng = np.random.default_rng(70 + views)
@@ -128,7 +131,7 @@ def test_fused_l2_shared_brick_copy_equals_own_copy(ops, views):
     tgt = dev(rng.uniform(0.0, 0.3, (1, views) + out_dhw[1:] + (3,)))
     rt = rho.clone().requires_grad_()
     loss, img = ops.render_density_fused_l2(rt, tgt, (0.9, 0.8, 0.7), ds, M, V, P, F, out_dhw, loss_weight=0.5)
-    loss.backward()
+    (loss * 1.75).backward()
     cams = ops._Cams(M, V, P, F)
     lp, _, keep = _lib.host_floats((0.9, 0.8, 0.7), 3)
     s_d = _lib.shape5(rho)
@@ -138,15 +141,15 @@ def test_fused_l2_shared_brick_copy_equals_own_copy(ops, views):
     loss2 = torch.empty((), device="cuda:0")
     fws, fnb = _lib.workspace(lib.ngt_render_density_l2_fwd_workspace_size(s_d, 1, cams.nv, od), rho.device)
     _lib.check(lib.ngt_render_density_l2_fwd(_lib.ptr(rho), s_d, lp, float(ds), cams.m, cams.v, cams.p, cams.f,
-                                             cams.nv, od, 1, 0, 0, _lib.ptr(tgt), 0.5, None, _lib.ptr(img2),
+                                             cams.nv, od, 1, 0, 0, _lib.ptr(tgt), 0.5, None, None, 0, _lib.ptr(img2),
                                              _lib.ptr(loss2), _lib.ptr(fws), fnb, st))
     g2 = torch.empty_like(rho)
-    one = torch.ones((), device="cuda:0")
+    one = torch.full((), 1.75, device="cuda:0")
     bws, bnb = _lib.workspace(lib.ngt_render_density_l2_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f,
                                                                            cams.nv, od, 1, 0), rho.device)
     _lib.check(lib.ngt_render_density_l2_bwd(_lib.ptr(rho), s_d, lp, float(ds), _lib.ptr(img2), _lib.ptr(tgt), 0.5,
                                              _lib.ptr(one), cams.m, cams.v, cams.p, cams.f, cams.nv, od, 1, 0, 0,
-                                             None, _lib.ptr(g2), _lib.ptr(bws), bnb, st))
+                                             None, 0, _lib.ptr(g2), _lib.ptr(bws), bnb, st))
     torch.cuda.synchronize()
     assert torch.equal(img, img2)
     assert torch.equal(loss, loss2)
