@@ -171,9 +171,10 @@ int ngt_reduce_grid_blend_bwd(const float* output_grad, const float* output, con
  * input [N,D,H,W,C] -> output [N,V|1,H',W',C] for output_dhw = (D',H',W')
  * (D' = samples per ray before BORDER clipping).  With LINEAR interpolation
  * and at least _fwd_workspace_size bytes of workspace the input is first
- * copied to a 4x4x4-bricked layout that the ray gathers read (fewer cache
- * lines per wavefront); with a null / smaller workspace the rays read the
- * NDHWC input directly.  Both give the same result. */
+ * copied to the haloed brick layout that the ray gathers read (one address
+ * per trilinear footprint, fewer cache lines per wavefront; see
+ * ngt_bricked_grid); with a null / smaller workspace the rays read the NDHWC
+ * input directly.  Both give the same result. */
 size_t ngt_raymarch_grid_transform_fwd_workspace_size(const int64_t* input_shape, int interpolation);
 int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_shape, const float* matrix_m,
                                     const float* matrix_v, const float* matrix_p, const float* frustum,

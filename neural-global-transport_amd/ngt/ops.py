@@ -34,8 +34,8 @@ gradient_semantics = "exact"
 use_gather_adjoint = True
 
 # Raymarch / render forwards with LINEAR interpolation hand the library a
-# workspace for a 4x4x4-bricked copy of the grid that the rays read; False
-# makes them read the NDHWC grid directly (same result).
+# workspace for a haloed 4x4x4-brick copy of the grid that the rays read;
+# False makes them read the NDHWC grid directly (same result).
 use_bricked_layout = True
 
 INTERPOLATION = {"NEAREST": 0, "LINEAR": 1, "MIN": 2, "MAX": 3, "QUADRATIC": 4}
