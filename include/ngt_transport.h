@@ -245,8 +245,9 @@ int ngt_bricked_grid(const float* grid, const int64_t* shape, int boundary, floa
  * given, neither call makes its own copy (the workspace sizes still assume
  * NULL).  `record` (nullable, LINEAR and BORDER only): a buffer of
  * ngt_render_density_l2_bwd_workspace_size bytes in which the forward also
- * records each ray and its per-sample gradients for a unit loss gradient,
- * walking each ray a second time right after the first; _l2_bwd with
+ * records each ray and its per-sample gradients for a unit loss gradient
+ * (BEER_LAMBERT and ADDITIVE within the same walk, the ALPHA blends in a
+ * second walk of each ray right after the first); _l2_bwd with
  * `recorded` = 1 and that buffer as its workspace then only gathers them
  * (scaled by *loss_grad). */
 size_t ngt_render_density_l2_fwd_workspace_size(const int64_t* density_shape, int interpolation, int num_cameras,

@@ -393,10 +393,13 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
 // samples with a corner in the brick - the same terms the scatter would add -
 // accumulates them in LDS and writes each voxel once, with no global atomics.
 
+// A ray of the gather adjoint, and how its stored per-sample values become
+// gradients: G = g_scale * (stored - g_off) (1 and 0 when the stored values
+// are the gradients themselves).
 struct RayRec {
   float sx, sy, sz, dx, dy, dz;
   int steps;
-  int pad;
+  float g_scale, g_off;
 };
 
 // Image gradient of the fused L2 loss at one pixel (img: the pixel's C
@@ -432,7 +435,8 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
   RayRec rec;
   rec.sx = r.sx; rec.sy = r.sy; rec.sz = r.sz; rec.dx = r.dx; rec.dy = r.dy; rec.dz = r.dz;
   rec.steps = r.steps;
-  rec.pad = 0;
+  rec.g_scale = 1.f;
+  rec.g_off = 0.f;
   rays[pair * npix + pixel] = rec;
   if (r.steps <= 0) return;
   const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
@@ -463,13 +467,97 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
 // unit upstream gradient (the backward scales the volume gradient by the
 // actual one).  A ray's second walk runs right after its first, while its
 // samples are still in the caches.
+// One-walk recording (fused density render, BEER_LAMBERT or ADDITIVE, L2
+// loss on RGB only, so the optical-depth channel's image gradient is 0).  With
+// s_j = rho_j ds the sample's cell is (light s_j, s_j), and the exact blend
+// adjoint gives, for the image gradient g,
+//   d loss / d rho_k = ds <light, g> (e_k - sum_{j>=k} e_j s_j)     (BEER_LAMBERT)
+//                    = ds <light, g> (h_k - Q_N),  h_k = e_k + Q_{k-1},
+// with e_j = exp(-d_j) and Q_k = sum_{j<=k} e_j s_j; for ADDITIVE it is
+// ds <light, g> (h_k = 1, Q_N = 0).  The walk stores h_k per sample and the
+// ray record carries g_scale = ds <light, g> (g for a unit loss gradient) and
+// g_off = Q_N, known once the walk ends.  The image and its loss partial are
+// exactly those of raymarch_fwd_kernel.
+template <int MODE>
+NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, const VBlock& vb, int x, int y,
+                                          float* __restrict__ out, float& l2, int pair_base, float* __restrict__ G,
+                                          RayRec* __restrict__ rays, int kcap) {
+  static_assert(MODE == BEER_LAMBERT || MODE == ADDITIVE, "one-walk recording: BEER_LAMBERT or ADDITIVE");
+  const CamParams& cp = cb.cam[vb.z];
+  const float* src = ray_source<1, true>(a, cp.data_index);
+  const Ray r = make_ray<BORDER>(a, cp, x, y);
+  const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
+  const int64_t pixel = static_cast<int64_t>(y) * a.out_w + x;
+  const int pair = pair_base + vb.z;
+  float* gp = G + static_cast<int64_t>(pair) * kcap * npix + pixel;
+  const int rec_steps = min(r.steps, kcap);
+  Vec<4> acc = vzero<4>();
+  float q = 0.f;  // Q_{k-1}
+  for (int k = 0; k < r.steps; k += kUnroll) {
+    float qx[kUnroll], qy[kUnroll], qz[kUnroll];
+    Vec<4> cells[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const float kf = static_cast<float>(k + u);
+      qx[u] = fmaf(kf, r.dx, r.sx) - 0.5f; qy[u] = fmaf(kf, r.dy, r.sy) - 0.5f; qz[u] = fmaf(kf, r.dz, r.sz) - 0.5f;
+    }
+    sample_cells<4, LINEAR, BORDER, true, true>(a, src, qx, qy, qz, cells);
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const bool live = k + u < r.steps;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cells[u].v[i] = live ? cells[u].v[i] : 0.f;
+      float h = 1.f;
+      if constexpr (MODE == BEER_LAMBERT) {
+        const float dd = fmaxf(acc.v[3] + cells[u].v[3], 0.f);
+        const float e = exp_neg(dd);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc.v[i] += e * cells[u].v[i];
+        acc.v[3] = dd;
+        h = e + q;
+        q += e * cells[u].v[3];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc.v[i] += cells[u].v[i];
+      }
+      if (k + u < rec_steps) gp[static_cast<int64_t>(k + u) * npix] = h;
+    }
+  }
+  const int64_t pix = (static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x;
+  vstore<4>(out + pix * 4, acc);
+  const float* t = a.l2_target + pix * 3;
+  const float d0 = acc.v[0] - t[0], d1 = acc.v[1] - t[1], d2 = acc.v[2] - t[2];
+  l2 = d0 * d0 + d1 * d1 + d2 * d2;
+  RayRec rec;
+  rec.sx = r.sx; rec.sy = r.sy; rec.sz = r.sz; rec.dx = r.dx; rec.dy = r.dy; rec.dz = r.dz;
+  rec.steps = rec_steps;
+  rec.g_scale = a.depth_step * 2.f * a.l2_scale * (a.light_r * d0 + a.light_g * d1 + a.light_b * d2);
+  rec.g_off = MODE == BEER_LAMBERT ? q : 0.f;
+  rays[static_cast<int64_t>(pair) * npix + pixel] = rec;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void raymarch_fwd_record_kernel(RayArgs a, CamBatch cb, float* __restrict__ out,
                                                                   int pair_base, float* __restrict__ G,
                                                                   RayRec* __restrict__ rays, int kcap) {
   const VBlock vb{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)};
-  raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(vb, a, cb, out);
-  raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(vb, a, cb, pair_base, out, nullptr, G, rays, kcap);
+  if constexpr (MODE == BEER_LAMBERT || MODE == ADDITIVE) {
+    int x, y;
+    pixel_of_thread(vb, x, y);
+    float l2 = 0.f;
+    if (x < a.out_w && y < a.out_h) raymarch_fwd_record_pixel<MODE>(a, cb, vb, x, y, out, l2, pair_base, G, rays, kcap);
+    __shared__ float s_w[4];  // the work-group's loss partial, as raymarch_fwd_kernel_body
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) l2 += __shfl_down(l2, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = l2;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      a.l2_partial[(static_cast<int64_t>(vb.z) * gridDim.y + vb.y) * gridDim.x + vb.x] =
+          (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+  } else {  // ALPHA blends: the image first, then the adjoint walk of each ray
+    raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(vb, a, cb, out);
+    raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(vb, a, cb, pair_base, out, nullptr, G, rays, kcap);
+  }
 }
 
 // Inclusive scan of one int per work-item over a 256-wide work-group: a
@@ -668,11 +756,13 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
       }
       __syncthreads();
       // sample i -> (bin, weights, gradient); bin < 0 when it does not touch the brick
-      auto locate = [&](int i, float& wx, float& wy, float& wz, int& rec, int& k) -> int {
+      auto locate = [&](int i, float& wx, float& wy, float& wz, int& rec, int& k, float& gs, float& go) -> int {
         const int lo_j = s_ray[i - i0];
         k = s_k0[lo_j] + (i - s_off[lo_j]);
         rec = s_rec[lo_j];
         const RayRec rr = rays[rec];
+        gs = rr.g_scale;
+        go = rr.g_off;
         const float kf = static_cast<float>(k);
         const float qx = fmaf(kf, rr.dx, rr.sx) - 0.5f;
         const float qy = fmaf(kf, rr.dy, rr.sy) - 0.5f;
@@ -702,10 +792,13 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
         irank[j] = 0;
         if (i < i_end) {
           int rec, k;
-          const int bin = locate(i, iwx[j], iwy[j], iwz[j], rec, k);
+          float gs, go;
+          const int bin = locate(i, iwx[j], iwy[j], iwz[j], rec, k, gs, go);
           if (bin >= 0) {
             ig[j] = vload<CG>(G + (static_cast<int64_t>(k) * npix + (rec % npix) +
                                    static_cast<int64_t>(rec / npix) * kcap * npix) * CG);
+#pragma unroll
+            for (int ch = 0; ch < CG; ++ch) ig[j].v[ch] = gs * (ig[j].v[ch] - go);
             irank[j] = atomicAdd(&bin_cnt[bin], 1);
             ibin[j] = bin;
           }

@@ -117,10 +117,11 @@ def test_fused_l2_shared_brick_copy_equals_own_copy(ops, views):
     and the backward and has the forward record the per-sample gradients;
     the C ABI called without either (each call making its own copy, the
     backward walking the rays itself) gives bit-identical images and loss,
-    and the same gradient up to summation order (the gather adjoint ranks
-    samples within a bin by LDS atomics, so the order of its sums varies
-    between runs) and the unit-gradient recording (the loss gradient applied
-    at the end).  9 views take two camera launches, so the per-work-group
+    and the same gradient up to rounding: the gather adjoint ranks samples
+    within a bin by LDS atomics, so the order of its sums varies between
+    runs, and the recording forms each sample's gradient as
+    ds <light, g> (h_k - Q_N) from running sums instead of the adjoint walk's
+    prefix sums (1e-5 here; both are checked against the oracle at 1e-4).  9 views take two camera launches, so the per-work-group
     loss partials and recorded rays of both count."""
     from ngt import _lib
     lib = _lib.load()
@@ -153,7 +154,7 @@ def test_fused_l2_shared_brick_copy_equals_own_copy(ops, views):
     torch.cuda.synchronize()
     assert torch.equal(img, img2)
     assert torch.equal(loss, loss2)
-    assert rel(host(rt.grad), host(g2)) < 1e-6
+    assert rel(host(rt.grad), host(g2)) < 1e-5
     # and the loss against the oracle
     cat = np.concatenate([host(rho) * np.array([0.9, 0.8, 0.7]), host(rho)], -1) * ds
     eimg = RM.raymarch_grid_transform(cat, M, V, P, F, out_dhw, LINEAR, BORDER, B.BEER_LAMBERT)
