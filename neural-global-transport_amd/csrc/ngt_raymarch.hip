@@ -396,10 +396,13 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
 // A ray of the gather adjoint, and how its stored per-sample values become
 // gradients: G = g_scale * (stored - g_off) (1 and 0 when the stored values
 // are the gradients themselves).
-struct RayRec {
+// The 8 floats a gather reads per sample come first and the record is
+// padded to 48 bytes, so they are two aligned 16-byte loads.
+struct alignas(16) RayRec {
   float sx, sy, sz, dx, dy, dz;
-  int steps;
   float g_scale, g_off;
+  int steps;
+  int pad[3];
 };
 
 // Image gradient of the fused L2 loss at one pixel (img: the pixel's C
@@ -432,7 +435,7 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
   const int pair = pair_base + static_cast<int>(vb.z);
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
   const int64_t pixel = static_cast<int64_t>(y) * a.out_w + x;
-  RayRec rec;
+  RayRec rec{};
   rec.sx = r.sx; rec.sy = r.sy; rec.sz = r.sz; rec.dx = r.dx; rec.dy = r.dy; rec.dz = r.dz;
   rec.steps = r.steps;
   rec.g_scale = 1.f;
@@ -528,7 +531,7 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   const float* t = a.l2_target + pix * 3;
   const float d0 = acc.v[0] - t[0], d1 = acc.v[1] - t[1], d2 = acc.v[2] - t[2];
   l2 = d0 * d0 + d1 * d1 + d2 * d2;
-  RayRec rec;
+  RayRec rec{};
   rec.sx = r.sx; rec.sy = r.sy; rec.sz = r.sz; rec.dx = r.dx; rec.dy = r.dy; rec.dz = r.dz;
   rec.steps = rec_steps;
   rec.g_scale = a.depth_step * 2.f * a.l2_scale * (a.light_r * d0 + a.light_g * d1 + a.light_b * d2);
