@@ -597,7 +597,10 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
                                                                  float* __restrict__ in_grad, int accumulate,
                                                                  const float* __restrict__ grad_scale) {
   constexpr int NV = BX * BY * BZ;
-  static_assert(NV == 512, "each of the 256 work-items owns two voxels of the brick");
+  // each of the 256 work-items owns a run of OX consecutive x-voxels of one
+  // (y, z) row of the brick
+  constexpr int OX = NV / 256;
+  static_assert(NV % 256 == 0 && BX % OX == 0 && (BX / OX) * BY * BZ == 256, "brick / owner layout");
   __shared__ float acc[NV * CG];
   __shared__ int rect[kMaxCamsPerLaunch][4];  // x0, y0, nx, ny (nx = 0: view skipped)
   __shared__ int ray_begin[kMaxCamsPerLaunch + 1];
@@ -701,9 +704,9 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   __shared__ int s_wtot[4];
   __shared__ unsigned char s_ray[kItems];
   // owned voxels: the x-pair of work-item tid (NV == 512)
-  float accv[2][CG];
+  float accv[OX][CG];
 #pragma unroll
-  for (int o = 0; o < 2; ++o)
+  for (int o = 0; o < OX; ++o)
 #pragma unroll
     for (int ch = 0; ch < CG; ++ch) accv[o][ch] = 0.f;
   const int total = ray_begin[cb.count];
@@ -841,42 +844,37 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
         }
       }
       __syncthreads();
-      // owners: work-item tid holds the x-pair (ox, ox + 1) of row (oy, oz);
-      // voxel v is corner d of the samples whose floor cell is v - d, so per
-      // (dy, dz) the pair reads the three consecutive bins ox .. ox + 2 (bin x
-      // index = floor x - b0x + 1): bin ox feeds voxel ox as its ceil corner,
-      // bin ox + 1 feeds ox as floor and ox + 1 as ceil corner, bin ox + 2
-      // feeds ox + 1 as floor corner
+      // owners: work-item tid holds the x-run ox .. ox + OX - 1 of row (oy,
+      // oz); voxel v is corner d of the samples whose floor cell is v - d, so
+      // per (dy, dz) the run reads the OX + 1 consecutive bins ox .. ox + OX
+      // (bin x index = floor x - b0x + 1): bin ox + b feeds voxel ox + b - 1
+      // as its floor corner (b >= 1) and voxel ox + b as its ceil corner
+      // (b < OX)
       {
-        const int ox = 2 * (tid % (BX / 2)), oy = (tid / (BX / 2)) % BY, oz = tid / (BX / 2 * BY);
+        const int ox = OX * (tid % (BX / OX)), oy = (tid / (BX / OX)) % BY, oz = tid / (BX / OX * BY);
 #pragma unroll
         for (int dyz = 0; dyz < 4; ++dyz) {
           const int dy = dyz & 1, dz = dyz >> 1;
           const int row = ((oz + 1 - dz) * (BY + 1) + (oy + 1 - dy)) * (BX + 1) + ox;
-          const int s0 = bin_start[row], s1 = bin_start[row + 1], s2 = bin_start[row + 2], s3 = bin_start[row + 3];
+          int sb[OX + 2];
+#pragma unroll
+          for (int b = 0; b < OX + 2; ++b) sb[b] = bin_start[row + b];
           // sample gradient times its y / z weight for this (dy, dz)
           auto tyz = [&](int q, const float4& rq, int ch) {
             const float g = CG == 1 ? rq.w : sg[q * CG + ch];
             return g * ((dy ? rq.y : 1.f - rq.y) * (dz ? rq.z : 1.f - rq.z));
           };
-          for (int q = s0; q < s1; ++q) {  // voxel ox as ceil corner
-            const float4 rq = srec[q];
 #pragma unroll
-            for (int ch = 0; ch < CG; ++ch) accv[0][ch] += tyz(q, rq, ch) * rq.x;
-          }
-          for (int q = s1; q < s2; ++q) {  // ox as floor, ox + 1 as ceil corner
-            const float4 rq = srec[q];
+          for (int b = 0; b <= OX; ++b) {
+            for (int q = sb[b]; q < sb[b + 1]; ++q) {
+              const float4 rq = srec[q];
 #pragma unroll
-            for (int ch = 0; ch < CG; ++ch) {
-              const float t = tyz(q, rq, ch);
-              accv[0][ch] += t * (1.f - rq.x);
-              accv[1][ch] += t * rq.x;
+              for (int ch = 0; ch < CG; ++ch) {
+                const float t = tyz(q, rq, ch);
+                if (b >= 1) accv[b - 1][ch] += t * (1.f - rq.x);
+                if (b < OX) accv[b][ch] += t * rq.x;
+              }
             }
-          }
-          for (int q = s2; q < s3; ++q) {  // ox + 1 as floor corner
-            const float4 rq = srec[q];
-#pragma unroll
-            for (int ch = 0; ch < CG; ++ch) accv[1][ch] += tyz(q, rq, ch) * (1.f - rq.x);
           }
         }
       }
@@ -884,9 +882,9 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
     }
   }
   {
-    const int ox = 2 * (tid % (BX / 2)), oy = (tid / (BX / 2)) % BY, oz = tid / (BX / 2 * BY);
+    const int ox = OX * (tid % (BX / OX)), oy = (tid / (BX / OX)) % BY, oz = tid / (BX / OX * BY);
 #pragma unroll
-    for (int o = 0; o < 2; ++o)
+    for (int o = 0; o < OX; ++o)
 #pragma unroll
       for (int ch = 0; ch < CG; ++ch) acc[((oz * BY + oy) * BX + ox + o) * CG + ch] = accv[o][ch];
   }
@@ -1239,7 +1237,7 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
           hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false, true>), rgrid, dim3(256), 0, s, a, cb,
                              base, output, output_grad, gp.G, gp.rays, gp.kcap)));
     }
-    constexpr int BX = 8, BY = 8, BZ = 8;
+    constexpr int BX = 16, BY = 8, BZ = 8;
     const unsigned bricks = blocks_for(a.din.x, BX) * blocks_for(a.din.y, BY) * blocks_for(a.din.z, BZ);
     const int accumulate = b > 0 ? 1 : 0;
     const int CG = LIGHT ? 1 : C;
