@@ -360,16 +360,98 @@ inline BrickGrid brick_grid(const Dims& d) {
                    static_cast<int>(blocks_for(d.z, kBrickZ))};
 }
 
+// The upper-z corner plane of the last in-tile scatter of a z-column walk,
+// held in registers: the next cell of the column usually has these four
+// corners as its lower plane, and the two contributions then reach the LDS
+// tile with one atomic add per corner instead of two.
+template <int C>
+struct HeldPlane {
+  int base = -1;  // tile slot of corner (x0, y0), -1 = nothing held
+  int dx = 0;     // slot step from x0 to x1 (0 or 1)
+  int dy = 0;     // slot step from y0 to y1 (0 or kTileX)
+  Vec<C> v[4];    // corners (x0,y0), (x1,y0), (x0,y1), (x1,y1)
+};
+
+template <int C>
+NGT_DEVICE void plane_add(float* tile, int base, int dx, int dy, const Vec<C> (&v)[4]) {
+  const int off[4] = {0, dx, dy, dx + dy};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float* t = tile + (base + off[k]) * C;
+#pragma unroll
+    for (int i = 0; i < C; ++i) lds_add(t + i, v[k].v[i]);
+  }
+}
+
+// Adds what `held` holds to the tile and empties it (end of a column walk).
+template <int C>
+NGT_DEVICE void flush_plane(float* tile, HeldPlane<C>& held) {
+  if (held.base >= 0) plane_add<C>(tile, held.base, held.dx, held.dy, held.v);
+  held.base = -1;
+}
+
 // Scatter val * w to the 8 corners of q: into the LDS tile (origin o*) when
-// the corner lies in it, else atomically into `ovf` (a full field).
+// the corner lies in it, else atomically into `ovf` (a full field).  Called
+// for consecutive cells of a z-column with one `held` plane per walk, which
+// the caller flushes at the end of the walk.
 template <int C, int BM>
-NGT_DEVICE void scatter_linear_tile(float* tile, int ox, int oy, int oz, float* __restrict__ ovf, const Dims& d,
-                                    float qx, float qy, float qz, const Vec<C>& val) {
+NGT_DEVICE void scatter_linear_tile(float* tile, HeldPlane<C>& held, int ox, int oy, int oz,
+                                    float* __restrict__ ovf, const Dims& d, float qx, float qy, float qz,
+                                    const Vec<C>& val) {
   const Corners c = make_corners<BM>(d, qx, qy, qz);
   if constexpr (BM == BORDER) {
     if (!c.inside) return;
   }
   const float fw[2][3] = {{1.f - c.wx, 1.f - c.wy, 1.f - c.wz}, {c.wx, c.wy, c.wz}};
+  const int lx0 = c.x0 - ox, lx1 = c.x1 - ox, ly0 = c.y0 - oy, ly1 = c.y1 - oy, lz0 = c.z0 - oz, lz1 = c.z1 - oz;
+  if (static_cast<unsigned>(lx0) < static_cast<unsigned>(kTileX) &&
+      static_cast<unsigned>(lx1) < static_cast<unsigned>(kTileX) &&
+      static_cast<unsigned>(ly0) < static_cast<unsigned>(kTileY) &&
+      static_cast<unsigned>(ly1) < static_cast<unsigned>(kTileY) &&
+      static_cast<unsigned>(lz0) < static_cast<unsigned>(kTileZ) &&
+      static_cast<unsigned>(lz1) < static_cast<unsigned>(kTileZ)) {
+    // the common case, all 8 corners in the tile (a BORDER corner outside the
+    // grid adds 0 to its halo slot instead of being skipped, which leaves
+    // every sum unchanged)
+    float wx[2] = {fw[0][0], fw[1][0]}, wy[2] = {fw[0][1], fw[1][1]}, wz[2] = {fw[0][2], fw[1][2]};
+    if constexpr (BM == BORDER) {
+      wx[0] = c.vx0 ? wx[0] : 0.f;
+      wx[1] = c.vx1 ? wx[1] : 0.f;
+      wy[0] = c.vy0 ? wy[0] : 0.f;
+      wy[1] = c.vy1 ? wy[1] : 0.f;
+      wz[0] = c.vz0 ? wz[0] : 0.f;
+      wz[1] = c.vz1 ? wz[1] : 0.f;
+    }
+    const int dx = lx1 - lx0, dy = (ly1 - ly0) * kTileX;
+    const int base0 = (lz0 * kTileY + ly0) * kTileX + lx0;
+    const int base1 = base0 + (lz1 - lz0) * (kTileX * kTileY);
+    Vec<C> lo[4], hi[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int bx = k & 1, by = k >> 1;
+      const float w0 = wx[bx] * wy[by] * wz[0], w1 = wx[bx] * wy[by] * wz[1];
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        lo[k].v[i] = val.v[i] * w0;
+        hi[k].v[i] = val.v[i] * w1;
+      }
+    }
+    if (held.base == base0 && held.dx == dx && held.dy == dy) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int i = 0; i < C; ++i) lo[k].v[i] += held.v[k].v[i];
+    } else {
+      flush_plane<C>(tile, held);
+    }
+    plane_add<C>(tile, base0, dx, dy, lo);
+    held.base = base1;
+    held.dx = dx;
+    held.dy = dy;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) held.v[k] = hi[k];
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
@@ -489,6 +571,7 @@ NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
   if (b.x < a.d.x && b.y < a.d.y) {
     const int zb = b.z0 + static_cast<int>(threadIdx.z) * kBrickZPer;
     const int z1 = min(zb + kBrickZPer, a.d.z);
+    HeldPlane<C> held;
 #pragma unroll 2
     for (int z = zb; z < z1; ++z) {
       float ux, uy, uz;
@@ -527,10 +610,11 @@ NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
       }
       if (g_dir) vstore<C>(g_dir + (b.pair * cells + cell) * C, g);
       vstore<C>(r_dir + (b.pair * cells + cell) * C, rd);
-      scatter_linear_tile<C, BM>(tile, b.ox, b.oy, b.oz, ovp, a.d, bx, by, bz, gback);
+      scatter_linear_tile<C, BM>(tile, held, b.ox, b.oy, b.oz, ovp, a.d, bx, by, bz, gback);
       add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, -a.ts * vdot(gback, dx),
                    -a.ts * vdot(gback, dy), -a.ts * vdot(gback, dz), vel_mode == 1 ? 0 : vel_mode);
     }
+    flush_plane<C>(tile, held);
   }
   tile_store<C>(tile, tiles, bg, b);
 }
@@ -566,6 +650,7 @@ NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
   if (b.x < a.d.x && b.y < a.d.y) {
     const int zb = b.z0 + static_cast<int>(threadIdx.z) * kBrickZPer;
     const int z1 = min(zb + kBrickZPer, a.d.z);
+    HeldPlane<C> held;
 #pragma unroll 2
     for (int z = zb; z < z1; ++z) {
       float ux, uy, uz;
@@ -587,12 +672,13 @@ NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
         for (int i = 0; i < C; ++i)
           if (!(r.v[i] >= 0.f)) g.v[i] = 0.f;
       }
-      scatter_linear_tile<C, BM>(tile, b.ox, b.oy, b.oz, ovp, a.d, px, py, pz, g);
+      scatter_linear_tile<C, BM>(tile, held, b.ox, b.oy, b.oz, ovp, a.d, px, py, pz, g);
       Vec<C> dx, dy, dz;
       read_gradient<C, BM>(src, a.d, px, py, pz, dx, dy, dz);
       add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, a.ts * vdot(g, dx), a.ts * vdot(g, dy),
                    a.ts * vdot(g, dz), vel_mode);
     }
+    flush_plane<C>(tile, held);
   }
   tile_store<C>(tile, tiles2, bg, b);
 }
