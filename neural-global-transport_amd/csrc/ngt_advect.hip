@@ -383,68 +383,121 @@ NGT_DEVICE void plane_add(float* tile, int base, int dx, int dy, const Vec<C> (&
   }
 }
 
-// Adds what `held` holds to the tile and empties it (end of a column walk).
+// Value of the previous / next lane of the 16-lane DPP row (row_shr:1 /
+// row_shl:1); 0 where that lane is outside the row or disabled.
+NGT_DEVICE int from_prev_lane(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false); }
+NGT_DEVICE int from_next_lane(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x101, 0xF, 0xF, false); }
+NGT_DEVICE float from_prev_lane(float v) {
+  return __builtin_bit_cast(float, from_prev_lane(__builtin_bit_cast(int, v)));
+}
+
+// plane_add for the lanes with `on` set, where a lane hands its x1 corner
+// column to the next lane of its row when that lane's x0 column is the same
+// pair of slots (work-items of a row hold consecutive x cells, so this is the
+// common case): each shared slot then takes one atomic add instead of two.
+// Every lane of the calling loop executes it (the exchanges read the
+// neighbours' registers); a lane whose neighbour is disabled or outside the
+// row sees 0 and keeps its own adds.
+template <int C>
+NGT_DEVICE void plane_add_row_merged(float* tile, bool on, int base, int dx, int dy, Vec<C> (&v)[4]) {
+  const int flag = dy != 0 ? 1 : 0;
+  const int give_key = on ? 2 * (base + dx) + 2 + flag : 0;  // slots of my x1 column
+  const bool take = on && from_prev_lane(give_key) == 2 * base + 2 + flag;
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const float r0 = from_prev_lane(v[1].v[i]), r1 = from_prev_lane(v[3].v[i]);
+    if (take) {
+      v[0].v[i] += r0;
+      v[2].v[i] += r1;
+    }
+  }
+  const bool given = from_next_lane(take ? 1 : 0) != 0;
+  if (on) {
+    float* t0 = tile + base * C;
+    float* t2 = tile + (base + dy) * C;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      lds_add(t0 + i, v[0].v[i]);
+      lds_add(t2 + i, v[2].v[i]);
+    }
+    if (!given) {
+      float* t1 = tile + (base + dx) * C;
+      float* t3 = tile + (base + dx + dy) * C;
+#pragma unroll
+      for (int i = 0; i < C; ++i) {
+        lds_add(t1 + i, v[1].v[i]);
+        lds_add(t3 + i, v[3].v[i]);
+      }
+    }
+  }
+}
+
+// Adds what `held` holds to the tile and empties it.  Executed by every lane
+// of the walk at its end.
 template <int C>
 NGT_DEVICE void flush_plane(float* tile, HeldPlane<C>& held) {
-  if (held.base >= 0) plane_add<C>(tile, held.base, held.dx, held.dy, held.v);
+  plane_add_row_merged<C>(tile, held.base >= 0, held.base, held.dx, held.dy, held.v);
   held.base = -1;
 }
 
 // Scatter val * w to the 8 corners of q: into the LDS tile (origin o*) when
 // the corner lies in it, else atomically into `ovf` (a full field).  Called
-// for consecutive cells of a z-column with one `held` plane per walk, which
-// the caller flushes at the end of the walk.
+// by every lane of a z-column walk for its consecutive cells (`on` false for
+// a lane that scatters nothing), with one `held` plane per walk that
+// flush_plane empties at the end of the walk.
 template <int C, int BM>
-NGT_DEVICE void scatter_linear_tile(float* tile, HeldPlane<C>& held, int ox, int oy, int oz,
+NGT_DEVICE void scatter_linear_tile(float* tile, HeldPlane<C>& held, bool on, int ox, int oy, int oz,
                                     float* __restrict__ ovf, const Dims& d, float qx, float qy, float qz,
                                     const Vec<C>& val) {
   const Corners c = make_corners<BM>(d, qx, qy, qz);
-  if constexpr (BM == BORDER) {
-    if (!c.inside) return;
-  }
+  bool inside = on;
+  if constexpr (BM == BORDER) inside = on && c.inside;
   const float fw[2][3] = {{1.f - c.wx, 1.f - c.wy, 1.f - c.wz}, {c.wx, c.wy, c.wz}};
   const int lx0 = c.x0 - ox, lx1 = c.x1 - ox, ly0 = c.y0 - oy, ly1 = c.y1 - oy, lz0 = c.z0 - oz, lz1 = c.z1 - oz;
-  if (static_cast<unsigned>(lx0) < static_cast<unsigned>(kTileX) &&
-      static_cast<unsigned>(lx1) < static_cast<unsigned>(kTileX) &&
-      static_cast<unsigned>(ly0) < static_cast<unsigned>(kTileY) &&
-      static_cast<unsigned>(ly1) < static_cast<unsigned>(kTileY) &&
-      static_cast<unsigned>(lz0) < static_cast<unsigned>(kTileZ) &&
-      static_cast<unsigned>(lz1) < static_cast<unsigned>(kTileZ)) {
-    // the common case, all 8 corners in the tile (a BORDER corner outside the
-    // grid adds 0 to its halo slot instead of being skipped, which leaves
-    // every sum unchanged)
-    float wx[2] = {fw[0][0], fw[1][0]}, wy[2] = {fw[0][1], fw[1][1]}, wz[2] = {fw[0][2], fw[1][2]};
-    if constexpr (BM == BORDER) {
-      wx[0] = c.vx0 ? wx[0] : 0.f;
-      wx[1] = c.vx1 ? wx[1] : 0.f;
-      wy[0] = c.vy0 ? wy[0] : 0.f;
-      wy[1] = c.vy1 ? wy[1] : 0.f;
-      wz[0] = c.vz0 ? wz[0] : 0.f;
-      wz[1] = c.vz1 ? wz[1] : 0.f;
-    }
-    const int dx = lx1 - lx0, dy = (ly1 - ly0) * kTileX;
-    const int base0 = (lz0 * kTileY + ly0) * kTileX + lx0;
-    const int base1 = base0 + (lz1 - lz0) * (kTileX * kTileY);
-    Vec<C> lo[4], hi[4];
+  const bool in_tile = inside && static_cast<unsigned>(lx0) < static_cast<unsigned>(kTileX) &&
+                       static_cast<unsigned>(lx1) < static_cast<unsigned>(kTileX) &&
+                       static_cast<unsigned>(ly0) < static_cast<unsigned>(kTileY) &&
+                       static_cast<unsigned>(ly1) < static_cast<unsigned>(kTileY) &&
+                       static_cast<unsigned>(lz0) < static_cast<unsigned>(kTileZ) &&
+                       static_cast<unsigned>(lz1) < static_cast<unsigned>(kTileZ);
+  // the common case, all 8 corners in the tile (a BORDER corner outside the
+  // grid adds 0 to its halo slot instead of being skipped, which leaves every
+  // sum unchanged)
+  float wx[2] = {fw[0][0], fw[1][0]}, wy[2] = {fw[0][1], fw[1][1]}, wz[2] = {fw[0][2], fw[1][2]};
+  if constexpr (BM == BORDER) {
+    wx[0] = c.vx0 ? wx[0] : 0.f;
+    wx[1] = c.vx1 ? wx[1] : 0.f;
+    wy[0] = c.vy0 ? wy[0] : 0.f;
+    wy[1] = c.vy1 ? wy[1] : 0.f;
+    wz[0] = c.vz0 ? wz[0] : 0.f;
+    wz[1] = c.vz1 ? wz[1] : 0.f;
+  }
+  const int dx = lx1 - lx0, dy = (ly1 - ly0) * kTileX;
+  const int base0 = (lz0 * kTileY + ly0) * kTileX + lx0;
+  const int base1 = base0 + (lz1 - lz0) * (kTileX * kTileY);
+  Vec<C> lo[4], hi[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int bx = k & 1, by = k >> 1;
-      const float w0 = wx[bx] * wy[by] * wz[0], w1 = wx[bx] * wy[by] * wz[1];
+  for (int k = 0; k < 4; ++k) {
+    const int bx = k & 1, by = k >> 1;
+    const float w0 = wx[bx] * wy[by] * wz[0], w1 = wx[bx] * wy[by] * wz[1];
 #pragma unroll
-      for (int i = 0; i < C; ++i) {
-        lo[k].v[i] = val.v[i] * w0;
-        hi[k].v[i] = val.v[i] * w1;
-      }
+    for (int i = 0; i < C; ++i) {
+      lo[k].v[i] = val.v[i] * w0;
+      hi[k].v[i] = val.v[i] * w1;
     }
+  }
+  if (in_tile) {
     if (held.base == base0 && held.dx == dx && held.dy == dy) {
 #pragma unroll
       for (int k = 0; k < 4; ++k)
 #pragma unroll
         for (int i = 0; i < C; ++i) lo[k].v[i] += held.v[k].v[i];
-    } else {
-      flush_plane<C>(tile, held);
+    } else if (held.base >= 0) {
+      plane_add<C>(tile, held.base, held.dx, held.dy, held.v);
     }
-    plane_add<C>(tile, base0, dx, dy, lo);
+  }
+  plane_add_row_merged<C>(tile, in_tile, base0, dx, dy, lo);
+  if (in_tile) {
     held.base = base1;
     held.dx = dx;
     held.dy = dy;
@@ -452,6 +505,7 @@ NGT_DEVICE void scatter_linear_tile(float* tile, HeldPlane<C>& held, int ox, int
     for (int k = 0; k < 4; ++k) held.v[k] = hi[k];
     return;
   }
+  if (!inside) return;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
@@ -568,16 +622,21 @@ NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
   const float* src = rho + n * cells * C;
   const float* slp = sl + b.pair * cells * C;
   float* ovp = ovf + b.pair * cells * C;
-  if (b.x < a.d.x && b.y < a.d.y) {
+  // every lane of the wave runs the walk (the tile scatter exchanges values
+  // between neighbouring lanes): a lane outside the grid reads its clamped
+  // edge cell, stores nothing and scatters nothing
+  const bool lane_ok = b.x < a.d.x && b.y < a.d.y;
+  const int cx = min(b.x, a.d.x - 1), cy = min(b.y, a.d.y - 1);
+  {
     const int zb = b.z0 + static_cast<int>(threadIdx.z) * kBrickZPer;
     const int z1 = min(zb + kBrickZPer, a.d.z);
     HeldPlane<C> held;
 #pragma unroll 2
     for (int z = zb; z < z1; ++z) {
       float ux, uy, uz;
-      load_velocity(a, v, b.x, b.y, z, ux, uy, uz);
-      const int64_t cell = cell_offset(a.d, b.x, b.y, z);
-      const float fx = static_cast<float>(b.x), fy = static_cast<float>(b.y), fz = static_cast<float>(z);
+      load_velocity(a, v, cx, cy, z, ux, uy, uz);
+      const int64_t cell = cell_offset(a.d, cx, cy, z);
+      const float fx = static_cast<float>(cx), fy = static_cast<float>(cy), fz = static_cast<float>(z);
       const float tx = a.ts * ux, ty = a.ts * uy, tz = a.ts * uz;
       const float bx = fx + (-tx), by = fy + (-ty), bz = fz + (-tz);
       Vec<C> g = vload<C>(g_out + (b.pair * cells + cell) * C);
@@ -608,11 +667,13 @@ NGT_DEVICE void advect_mc_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
         rd.v[i] = 0.5f * keep.v[i] * g.v[i];
         gback.v[i] = -rd.v[i];
       }
-      if (g_dir) vstore<C>(g_dir + (b.pair * cells + cell) * C, g);
-      vstore<C>(r_dir + (b.pair * cells + cell) * C, rd);
-      scatter_linear_tile<C, BM>(tile, held, b.ox, b.oy, b.oz, ovp, a.d, bx, by, bz, gback);
-      add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, -a.ts * vdot(gback, dx),
-                   -a.ts * vdot(gback, dy), -a.ts * vdot(gback, dz), vel_mode == 1 ? 0 : vel_mode);
+      if (lane_ok) {
+        if (g_dir) vstore<C>(g_dir + (b.pair * cells + cell) * C, g);
+        vstore<C>(r_dir + (b.pair * cells + cell) * C, rd);
+        add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, -a.ts * vdot(gback, dx),
+                     -a.ts * vdot(gback, dy), -a.ts * vdot(gback, dz), vel_mode == 1 ? 0 : vel_mode);
+      }
+      scatter_linear_tile<C, BM>(tile, held, lane_ok, b.ox, b.oy, b.oz, ovp, a.d, bx, by, bz, gback);
     }
     flush_plane<C>(tile, held);
   }
@@ -647,22 +708,27 @@ NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
   const int64_t cells = static_cast<int64_t>(a.d.x) * a.d.y * a.d.z;
   const float* src = rho + n * cells * C;
   float* ovp = g_rho + n * cells * C;
-  if (b.x < a.d.x && b.y < a.d.y) {
+  // every lane of the wave runs the walk (the tile scatter exchanges values
+  // between neighbouring lanes): a lane outside the grid reads its clamped
+  // edge cell, stores nothing and scatters nothing
+  const bool lane_ok = b.x < a.d.x && b.y < a.d.y;
+  const int cx = min(b.x, a.d.x - 1), cy = min(b.y, a.d.y - 1);
+  {
     const int zb = b.z0 + static_cast<int>(threadIdx.z) * kBrickZPer;
     const int z1 = min(zb + kBrickZPer, a.d.z);
     HeldPlane<C> held;
 #pragma unroll 2
     for (int z = zb; z < z1; ++z) {
       float ux, uy, uz;
-      load_velocity(a, v, b.x, b.y, z, ux, uy, uz);
-      const int64_t cell = cell_offset(a.d, b.x, b.y, z);
-      const float px = static_cast<float>(b.x) + a.ts * ux;
-      const float py = static_cast<float>(b.y) + a.ts * uy;
+      load_velocity(a, v, cx, cy, z, ux, uy, uz);
+      const int64_t cell = cell_offset(a.d, cx, cy, z);
+      const float px = static_cast<float>(cx) + a.ts * ux;
+      const float py = static_cast<float>(cy) + a.ts * uy;
       const float pz = static_cast<float>(z) + a.ts * uz;
       Vec<C> g = vload<C>(g_in + (b.pair * cells + cell) * C);
       if constexpr (ORDER2) {
         const Vec<C> o = vload<C>(ovf1 + (b.pair * cells + cell) * C);
-        const Vec<C> t = gather_tiles<C>(tiles1, b.pair, bg, b.x, b.y, z);
+        const Vec<C> t = gather_tiles<C>(tiles1, b.pair, bg, cx, cy, z);
 #pragma unroll
         for (int i = 0; i < C; ++i) g.v[i] += o.v[i] + t.v[i];
       }
@@ -672,11 +738,12 @@ NGT_DEVICE void advect_sl_bwd_brick_kernel_body(const VBlock& vb, const AdvArgs&
         for (int i = 0; i < C; ++i)
           if (!(r.v[i] >= 0.f)) g.v[i] = 0.f;
       }
-      scatter_linear_tile<C, BM>(tile, held, b.ox, b.oy, b.oz, ovp, a.d, px, py, pz, g);
+      scatter_linear_tile<C, BM>(tile, held, lane_ok, b.ox, b.oy, b.oz, ovp, a.d, px, py, pz, g);
       Vec<C> dx, dy, dz;
       read_gradient<C, BM>(src, a.d, px, py, pz, dx, dy, dz);
-      add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, a.ts * vdot(g, dx), a.ts * vdot(g, dy),
-                   a.ts * vdot(g, dz), vel_mode);
+      if (lane_ok)
+        add_vel_grad(g_vc + (static_cast<int64_t>(v) * cells + cell) * 3, a.ts * vdot(g, dx), a.ts * vdot(g, dy),
+                     a.ts * vdot(g, dz), vel_mode);
     }
     flush_plane<C>(tile, held);
   }
