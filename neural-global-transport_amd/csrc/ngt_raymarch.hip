@@ -393,6 +393,18 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
 // samples with a corner in the brick - the same terms the scatter would add -
 // accumulates them in LDS and writes each voxel once, with no global atomics.
 
+// Recorded per-sample gradients G of one (data, camera) pair: the image's
+// pixels (row-major) in groups of kGPix, each group holding its pixels'
+// samples k side by side, (k, pixel) at
+// (pixel / kGPix) * kcap * kGPix + k * kGPix + pixel % kGPix.  The gather
+// reads consecutive samples of one ray kGPix * 4 bytes apart; the recording
+// writes the kGPix pixels of a group at one k together.
+constexpr int kGPix = 8;
+inline __host__ __device__ int64_t g_pixels(int64_t npix) { return (npix + kGPix - 1) / kGPix * kGPix; }
+NGT_DEVICE int64_t g_offset(int k, int64_t pixel, int kcap) {
+  return (pixel / kGPix) * kcap * kGPix + static_cast<int64_t>(k) * kGPix + pixel % kGPix;
+}
+
 // A ray of the gather adjoint, and how its stored per-sample values become
 // gradients: G = g_scale * (stored - g_off) (1 and 0 when the stored values
 // are the gradients themselves).
@@ -443,15 +455,15 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
   rays[pair * npix + pixel] = rec;
   if (r.steps <= 0) return;
   const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
-  // G[pair][k][pixel]: the samples k of neighbouring rays are stored side by side
   struct {
     const RayArgs& a;
     float* gp;
-    int64_t npix, pixel;
+    int64_t pixel;
+    int kcap;
     NGT_DEVICE void one(int k, float, float, float, const Vec<C>& gc) {
-      vstore<CG>(gp + (static_cast<int64_t>(k) * npix + pixel) * CG, fold_cell_grad<C, LIGHT>(a, gc));
+      vstore<CG>(gp + g_offset(k, pixel, kcap) * CG, fold_cell_grad<C, LIGHT>(a, gc));
     }
-  } sink{a, G + static_cast<int64_t>(pair) * kcap * npix * CG, npix, pixel};
+  } sink{a, G + static_cast<int64_t>(pair) * kcap * g_pixels(npix) * CG, pixel, kcap};
   const Vec<C> g_img = a.l2_target ? l2_image_grad<C>(a, out + pix, pix / C * 3) : vload<C>(out_grad + pix);
   walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT, BRICKED>(a, src, r, g_img, out + pix, sink);
 }
@@ -492,7 +504,7 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
   const int64_t pixel = static_cast<int64_t>(y) * a.out_w + x;
   const int pair = pair_base + vb.z;
-  float* gp = G + static_cast<int64_t>(pair) * kcap * npix + pixel;
+  float* gp = G + static_cast<int64_t>(pair) * kcap * g_pixels(npix) + g_offset(0, pixel, kcap);
   const int rec_steps = min(r.steps, kcap);
   Vec<4> acc = vzero<4>();
   float q = 0.f;  // Q_{k-1}
@@ -523,7 +535,7 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc.v[i] += cells[u].v[i];
       }
-      if (k + u < rec_steps) gp[static_cast<int64_t>(k + u) * npix] = h;
+      if (k + u < rec_steps) gp[static_cast<int64_t>(k + u) * kGPix] = h;
     }
   }
   const int64_t pix = (static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x;
@@ -697,6 +709,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   __shared__ int s_off[257];
   __shared__ int s_k0[256];
   __shared__ int s_rec[256];
+  __shared__ int64_t s_gofs[256];  // G offset of each ray's sample 0
   __shared__ int bin_cnt[NB];
   __shared__ int bin_start[NB + 1];
   __shared__ float4 srec[kItems];
@@ -714,6 +727,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   for (int base_r = 0; base_r < total; base_r += 256) {
     const int r = base_r + tid;
     int cnt = 0, k0 = 0, rec_index = 0;
+    int64_t gofs = 0;
     if (r < total) {
       int ci = 0;
       while (ci + 1 < cb.count && r >= ray_begin[ci + 1]) ++ci;
@@ -721,6 +735,8 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
       const int px = rect[ci][0] + local % rect[ci][2];
       const int py = rect[ci][1] + local / rect[ci][2];
       rec_index = (pair_base + ci) * static_cast<int>(npix) + py * a.out_w + px;
+      gofs = static_cast<int64_t>(pair_base + ci) * kcap * g_pixels(npix) +
+             g_offset(0, static_cast<int64_t>(py) * a.out_w + px, kcap);
       const RayRec rr = rays[rec_index];
       if (rr.steps > 0) {
         const float sv[3] = {rr.sx, rr.sy, rr.sz};
@@ -751,6 +767,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
     if (tid == 0) s_off[0] = 0;
     s_k0[tid] = k0;
     s_rec[tid] = rec_index;
+    s_gofs[tid] = gofs;
     __syncthreads();
     const int items = s_off[256];
     for (int i0 = 0; i0 < items; i0 += kItems) {
@@ -762,10 +779,11 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
       }
       __syncthreads();
       // sample i -> (bin, weights, gradient); bin < 0 when it does not touch the brick
-      auto locate = [&](int i, float& wx, float& wy, float& wz, int& rec, int& k, float& gs, float& go) -> int {
+      auto locate = [&](int i, float& wx, float& wy, float& wz, int64_t& gofs, int& k, float& gs, float& go) -> int {
         const int lo_j = s_ray[i - i0];
         k = s_k0[lo_j] + (i - s_off[lo_j]);
-        rec = s_rec[lo_j];
+        const int rec = s_rec[lo_j];
+        gofs = s_gofs[lo_j];
         const RayRec rr = rays[rec];
         gs = rr.g_scale;
         go = rr.g_off;
@@ -797,12 +815,12 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
         ibin[j] = -1;
         irank[j] = 0;
         if (i < i_end) {
-          int rec, k;
+          int64_t gofs;
+          int k;
           float gs, go;
-          const int bin = locate(i, iwx[j], iwy[j], iwz[j], rec, k, gs, go);
+          const int bin = locate(i, iwx[j], iwy[j], iwz[j], gofs, k, gs, go);
           if (bin >= 0) {
-            ig[j] = vload<CG>(G + (static_cast<int64_t>(k) * npix + (rec % npix) +
-                                   static_cast<int64_t>(rec / npix) * kcap * npix) * CG);
+            ig[j] = vload<CG>(G + (gofs + static_cast<int64_t>(k) * kGPix) * CG);
 #pragma unroll
             for (int ch = 0; ch < CG; ++ch) ig[j].v[ch] = gs * (ig[j].v[ch] - go);
             irank[j] = atomicAdd(&bin_cnt[bin], 1);
@@ -1177,7 +1195,7 @@ size_t gather_workspace_bytes(const int64_t* shape, const float* M, const float*
   const int kcap = ray_step_cap(pairs, M, V, P, F, a);
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   return align_up(pairs.size() * npix * sizeof(RayRec)) +
-         align_up(pairs.size() * static_cast<size_t>(kcap) * npix * grad_channels * sizeof(float)) +
+         align_up(pairs.size() * static_cast<size_t>(kcap) * g_pixels(npix) * grad_channels * sizeof(float)) +
          align_up(bricked_bytes(shape, grad_channels));
 }
 
@@ -1189,7 +1207,8 @@ bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V,
   gp.kcap = ray_step_cap(pairs, M, V, P, F, a);
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   const size_t rays_bytes = align_up(pairs.size() * npix * sizeof(RayRec));
-  const size_t g_bytes = align_up(pairs.size() * static_cast<size_t>(gp.kcap) * npix * grad_channels * sizeof(float));
+  const size_t g_bytes =
+      align_up(pairs.size() * static_cast<size_t>(gp.kcap) * g_pixels(npix) * grad_channels * sizeof(float));
   const size_t b_bytes = align_up(static_cast<size_t>(a.n_data) * a.bd.slice_cells * grad_channels * sizeof(float));
   if (workspace_bytes < rays_bytes + g_bytes + b_bytes) return false;
   gp.rays = static_cast<RayRec*>(workspace);
@@ -1789,7 +1808,7 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     it.ws_rays = static_cast<int64_t>(ws);
     ws += align_up(q.num_views * npix * sizeof(RayRec));
     it.ws_G = static_cast<int64_t>(ws);
-    ws += align_up(q.num_views * static_cast<size_t>(it.kcap) * npix * sizeof(float));
+    ws += align_up(q.num_views * static_cast<size_t>(it.kcap) * g_pixels(npix) * sizeof(float));
     tr += static_cast<int64_t>(q.num_views) * it.ntx * it.nty;
     tz += it.brick_tiles;
     tb += static_cast<int64_t>(blocks_for(it.a.din.x, 8)) * blocks_for(it.a.din.y, 8) * blocks_for(it.a.din.z, 8);
