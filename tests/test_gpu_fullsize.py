@@ -58,7 +58,11 @@ def test_render_gather_adjoint_equals_atomic_scatter(setup):
     (gt,) = torch.autograd.grad(out, xr, y)
     lhs = float((out.double() * y.double()).sum())
     rhs = float((x.double() * gt.double()).sum())
-    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    # the fp32 rounding of the image and of the gradient sums (taken in no
+    # fixed order) scales with the magnitudes of the terms, not with the
+    # cancelling inner product itself
+    l1 = float((out.double() * y.double()).abs().sum())
+    assert abs(lhs - rhs) <= 1e-7 * l1, (lhs, rhs, l1)
 
 
 def test_raymarch_gather_equals_scatter_multichannel(setup):
@@ -104,7 +108,10 @@ def test_advect_adjoint_identity_full_size(setup):
     (g,) = torch.autograd.grad(out, rho, y)
     lhs = float((out.double() * y.double()).sum())
     rhs = float((setup["rho"].double() * g.double()).sum())
-    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    # fp32 rounding (atomics in no fixed order) scales with the magnitudes of
+    # the terms, not with the cancelling inner product
+    l1 = float((out.double() * y.double()).abs().sum())
+    assert abs(lhs - rhs) <= 1e-7 * l1, (lhs, rhs, l1)
 
 
 def test_full_step_gradient_fd(setup):
