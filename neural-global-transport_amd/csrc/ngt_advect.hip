@@ -390,44 +390,65 @@ NGT_DEVICE int from_next_lane(int v) { return __builtin_amdgcn_update_dpp(0, v, 
 NGT_DEVICE float from_prev_lane(float v) {
   return __builtin_bit_cast(float, from_prev_lane(__builtin_bit_cast(int, v)));
 }
+// Value of lane (l + 32) % 64 (v_permlane32_swap of the wave's two halves).
+NGT_DEVICE int from_other_half(int v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return __lane_id() < 32 ? static_cast<int>(r[1]) : static_cast<int>(r[0]);
+}
+NGT_DEVICE float from_other_half(float v) {
+  return __builtin_bit_cast(float, from_other_half(__builtin_bit_cast(int, v)));
+}
 
-// plane_add for the lanes with `on` set, where a lane hands its x1 corner
-// column to the next lane of its row when that lane's x0 column is the same
-// pair of slots (work-items of a row hold consecutive x cells, so this is the
-// common case): each shared slot then takes one atomic add instead of two.
-// Every lane of the calling loop executes it (the exchanges read the
-// neighbours' registers); a lane whose neighbour is disabled or outside the
-// row sees 0 and keeps its own adds.
+// plane_add for the lanes with `on` set, with the corners that neighbouring
+// lanes share added once.  A wave holds two brick rows, y (lanes 0-31) and
+// y + 1 (lanes 32-63), of consecutive x cells, so usually
+//  - a lane's x1 column is the x0 column of the next lane of its DPP row: it
+//    hands that column over;
+//  - then a lane of row y has as its y1 row the y0 row of lane + 32, which
+//    takes it where it still adds those slots itself.
+// Every lane of the calling loop executes it (the exchanges read the other
+// lanes' registers); a lane whose partner is outside the row or disabled
+// sees 0 and keeps its own adds.
 template <int C>
 NGT_DEVICE void plane_add_row_merged(float* tile, bool on, int base, int dx, int dy, Vec<C> (&v)[4]) {
-  const int flag = dy != 0 ? 1 : 0;
-  const int give_key = on ? 2 * (base + dx) + 2 + flag : 0;  // slots of my x1 column
-  const bool take = on && from_prev_lane(give_key) == 2 * base + 2 + flag;
+  static_assert(kBrickX == 32, "the two halves of a wave are consecutive brick rows");
+  const int fx = dx != 0 ? 1 : 0, fy = dy != 0 ? 1 : 0;
+  // x: my x1 column (slots base + dx, base + dx + dy) to the next lane
+  const int give_key = on ? 2 * (base + dx) + 2 + fy : 0;
+  const bool take_x = on && from_prev_lane(give_key) == 2 * base + 2 + fy;
 #pragma unroll
   for (int i = 0; i < C; ++i) {
     const float r0 = from_prev_lane(v[1].v[i]), r1 = from_prev_lane(v[3].v[i]);
-    if (take) {
+    if (take_x) {
       v[0].v[i] += r0;
       v[2].v[i] += r1;
     }
   }
-  const bool given = from_next_lane(take ? 1 : 0) != 0;
+  const bool own1 = !(from_next_lane(take_x ? 1 : 0) != 0);  // still adds its x1 column
+  // y: the y1 row (slots base + dy, base + dy + dx) of a lane of row y to lane + 32
+  const bool upper = __lane_id() >= 32;
+  const int ykey = (on && !upper) ? ((2 * (base + dy) + 2 + fx) << 1) | (own1 ? 1 : 0) : 0;
+  const int got = from_other_half(ykey);
+  const bool take_y = on && upper && (got >> 1) == 2 * base + 2 + fx;
+  const bool take_y3 = take_y && (got & 1) && own1;  // the x1 slot of the row, where both still own it
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const float r2 = from_other_half(v[2].v[i]), r3 = from_other_half(v[3].v[i]);
+    if (take_y) v[0].v[i] += r2;
+    if (take_y3) v[1].v[i] += r3;
+  }
+  const int back = from_other_half((take_y ? 1 : 0) | (take_y3 ? 2 : 0));
+  const bool own2 = upper || !(back & 1);
+  const bool own3 = own1 && (upper || !(back & 2));
   if (on) {
-    float* t0 = tile + base * C;
-    float* t2 = tile + (base + dy) * C;
+    const int off[4] = {0, dx, dy, dx + dy};
+    const bool own[4] = {true, own1, own2, own3};
 #pragma unroll
-    for (int i = 0; i < C; ++i) {
-      lds_add(t0 + i, v[0].v[i]);
-      lds_add(t2 + i, v[2].v[i]);
-    }
-    if (!given) {
-      float* t1 = tile + (base + dx) * C;
-      float* t3 = tile + (base + dx + dy) * C;
+    for (int k = 0; k < 4; ++k) {
+      if (!own[k]) continue;
+      float* t = tile + (base + off[k]) * C;
 #pragma unroll
-      for (int i = 0; i < C; ++i) {
-        lds_add(t1 + i, v[1].v[i]);
-        lds_add(t3 + i, v[3].v[i]);
-      }
+      for (int i = 0; i < C; ++i) lds_add(t + i, v[k].v[i]);
     }
   }
 }
