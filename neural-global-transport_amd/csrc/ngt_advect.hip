@@ -339,10 +339,12 @@ __global__ __launch_bounds__(256) void faces_from_centered_grad_kernel(Dims d, i
 // to an overflow field with global fp32 atomics.  Each tile is stored whole
 // to a per-brick partial buffer with plain coalesced stores, and the
 // consumer gathers, for every cell, the (at most 2 per axis) tiles that
-// cover it in a fixed order: no global atomics in the common case and a
-// deterministic sum.  The halo sets the partial-buffer traffic (tile cells
-// per brick cell: 2.53 with kHalo = 2, 3.64 with 3).
-constexpr int kBrickX = 32, kBrickY = 8, kBrickZ = 8, kHalo = 2;
+// cover it in a fixed order: no global atomics in the common case.  (Within
+// a tile the LDS float atomics add in no fixed order.)  The halo sets the
+// partial-buffer traffic: tile cells per brick cell 2.11 with kHalo = 2,
+// 2.86 with 3.  16 cells in z (rather than 8) shorten that ratio and give
+// the column walks of the scatter more shared corner planes; 32 was slower.
+constexpr int kBrickX = 32, kBrickY = 8, kBrickZ = 16, kHalo = 2;
 // a work-group is kBrickX x kBrickY x kBrickZT work-items; each walks
 // kBrickZ / kBrickZT cells of its (x, y) column
 constexpr int kBrickZT = 2, kBrickZPer = kBrickZ / kBrickZT;
