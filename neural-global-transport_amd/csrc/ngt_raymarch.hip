@@ -387,7 +387,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
 
 // ---------------------------------------------------------------------------
 // Brick-gather adjoint (BORDER, LINEAR).  Pass 1 stores every sample's
-// gradient G[pair][k][pixel] and the ray parameters; pass 2 runs one
+// gradient G (layout at g_offset) and the ray parameters; pass 2 runs one
 // work-group per brick of voxels that collects, from the rays whose pixel
 // centres fall in the projection of the brick's trilinear support, exactly the
 // samples with a corner in the brick - the same terms the scatter would add -
