@@ -257,7 +257,7 @@ def smooth_field(rng, shape, amp):
 
 
 @pytest.mark.parametrize("order,clamp", [(1, "NONE"), (2, "NONE"), (2, "MC_SMOOTH"), (1, "NEGATIVE"), (2, "NEGATIVE")])
-@pytest.mark.parametrize("bound", [BORDER, CLAMP])
+@pytest.mark.parametrize("bound", [BORDER, CLAMP, WRAP])
 @pytest.mark.parametrize("staggered", [False, True])
 def test_advect(ngt_ops, order, clamp, bound, staggered):
     rng = np.random.default_rng(700 + order + bound + 3 * staggered)

@@ -73,7 +73,8 @@ def test_mc_smooth_revert_invariant_and_mask(boundary):
 
 
 @pytest.mark.parametrize("order,clamp,boundary", [(1, "NONE", BORDER), (2, "NONE", BORDER), (2, "MC_SMOOTH", BORDER),
-                                                   (2, "MC_SMOOTH", CLAMP), (1, "NEGATIVE", CLAMP)])
+                                                   (2, "MC_SMOOTH", CLAMP), (1, "NEGATIVE", CLAMP),
+                                                   (1, "NONE", WRAP), (2, "MC_SMOOTH", WRAP), (2, "NEGATIVE", WRAP)])
 def test_warp_backward_fd(order, clamp, boundary):
     rng = np.random.default_rng(3)
     shape = (7, 6, 8)
