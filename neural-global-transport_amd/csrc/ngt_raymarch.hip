@@ -807,8 +807,8 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
       // registers until they are placed
       constexpr int kPer = kItems / 256;
       int ibin[kPer], irank[kPer];
-      float iwx[kPer], iwy[kPer], iwz[kPer];
-      Vec<CG> ig[kPer];
+      float iwx[kPer], iwy[kPer], iwz[kPer], igs[kPer], igo[kPer];
+      Vec<CG> ig[kPer];  // raw G: scaled at placement, so the loads complete during the bin scan
 #pragma unroll
       for (int j = 0; j < kPer; ++j) {
         const int i = i0 + tid + 256 * j;
@@ -821,8 +821,8 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
           const int bin = locate(i, iwx[j], iwy[j], iwz[j], gofs, k, gs, go);
           if (bin >= 0) {
             ig[j] = vload<CG>(G + (gofs + static_cast<int64_t>(k) * kGPix) * CG);
-#pragma unroll
-            for (int ch = 0; ch < CG; ++ch) ig[j].v[ch] = gs * (ig[j].v[ch] - go);
+            igs[j] = gs;
+            igo[j] = go;
             irank[j] = atomicAdd(&bin_cnt[bin], 1);
             ibin[j] = bin;
           }
@@ -854,6 +854,8 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
 #pragma unroll
       for (int j = 0; j < kPer; ++j) {
         if (ibin[j] < 0) continue;
+#pragma unroll
+        for (int ch = 0; ch < CG; ++ch) ig[j].v[ch] = igs[j] * (ig[j].v[ch] - igo[j]);
         const int pos = bin_start[ibin[j]] + irank[j];
         srec[pos] = make_float4(iwx[j], iwy[j], iwz[j], ig[j].v[0]);
         if constexpr (CG > 1) {
