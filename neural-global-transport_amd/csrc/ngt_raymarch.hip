@@ -710,8 +710,11 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   __shared__ int s_k0[256];
   __shared__ int s_rec[256];
   __shared__ int64_t s_gofs[256];  // G offset of each ray's sample 0
-  __shared__ int bin_cnt[NB];
+  __shared__ float4 s_ps[256];     // each ray's start and step x
+  __shared__ float2 s_pd[256];     // each ray's step y, z
+  // per bin: the sample count, then (scanned in place) the first slot
   __shared__ int bin_start[NB + 1];
+  int* const bin_cnt = bin_start;
   __shared__ float4 srec[kItems];
   __shared__ float sg[CG > 1 ? kItems * CG : 1];
   __shared__ int s_wtot[4];
@@ -738,6 +741,8 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
       gofs = static_cast<int64_t>(pair_base + ci) * kcap * g_pixels(npix) +
              g_offset(0, static_cast<int64_t>(py) * a.out_w + px, kcap);
       const RayRec rr = rays[rec_index];
+      s_ps[tid] = make_float4(rr.sx, rr.sy, rr.sz, rr.dx);
+      s_pd[tid] = make_float2(rr.dy, rr.dz);
       if (rr.steps > 0) {
         const float sv[3] = {rr.sx, rr.sy, rr.sz};
         const float dv[3] = {rr.dx, rr.dy, rr.dz};
@@ -784,13 +789,16 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
         k = s_k0[lo_j] + (i - s_off[lo_j]);
         const int rec = s_rec[lo_j];
         gofs = s_gofs[lo_j];
-        const RayRec rr = rays[rec];
-        gs = rr.g_scale;
-        go = rr.g_off;
+        // the loss-gradient factors are first used at placement: their load
+        // completes during the bin scan
+        gs = rays[rec].g_scale;
+        go = rays[rec].g_off;
+        const float4 ps = s_ps[lo_j];
+        const float2 pd = s_pd[lo_j];
         const float kf = static_cast<float>(k);
-        const float qx = fmaf(kf, rr.dx, rr.sx) - 0.5f;
-        const float qy = fmaf(kf, rr.dy, rr.sy) - 0.5f;
-        const float qz = fmaf(kf, rr.dz, rr.sz) - 0.5f;
+        const float qx = fmaf(kf, ps.w, ps.x) - 0.5f;
+        const float qy = fmaf(kf, pd.x, ps.y) - 0.5f;
+        const float qz = fmaf(kf, pd.y, ps.z) - 0.5f;
         const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
         wx = qx - fx; wy = qy - fy; wz = qz - fz;
         const int lx = static_cast<int>(fmaxf(fminf(fx, 1e8f), -1e8f)) - fb0m1x;
