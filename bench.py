@@ -45,7 +45,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--graph", action="store_true", help="replay the fused step as a captured HIP graph")
+    p.add_argument("--graph", action="store_true",
+                   help="replay the fused step as a captured HIP graph (the default; kept for old command lines)")
+    p.add_argument("--eager", action="store_true", help="time the eagerly launched step only, no HIP graph")
     p.add_argument("--chain", action="store_true",
                    help="instead of the single step: the 30-step checkpointed chain at 96x160x96 (render every 5th "
                         "step, 5 views 240x135x96); prints its own JSON line (time, peak device memory)")
@@ -560,7 +562,8 @@ def main():
     print("bench: fused step", file=sys.stderr, flush=True)
     ms = time_steps(torch, step, args.steps, args.warmup)
     ms_graph = None
-    if args.graph:
+    graph_check = None
+    if not args.eager:
         g = torch.cuda.CUDAGraph()
         static = make_leaves(torch, prob)
 
@@ -578,6 +581,17 @@ def main():
         with torch.cuda.graph(g):
             gstep()
         ms_graph = time_steps(torch, g.replay, args.steps, args.warmup)
+        # the replayed step computes what the eager step computes (gradients
+        # to rounding: their sums use atomics in no fixed order)
+        zero()
+        step()
+        torch.cuda.synchronize()
+        graph_check = max(float((a.grad - b.grad).abs().max() / b.grad.abs().max().clamp_min(1e-30))
+                          for a, b in zip(static, leaves))
+        if not graph_check < 1e-4:
+            print("bench: HIP-graph replay disagrees with the eager step (max rel %.3g)" % graph_check,
+                  file=sys.stderr, flush=True)
+            sys.exit(1)
 
     ms_plain = None
     if not args.no_plain:
@@ -629,11 +643,12 @@ def main():
         "config": {"workload": "MacCormack step (MC_SMOOTH, BORDER) of 128^3 density with MAC curl-noise velocity "
                                "+ 5-view 256x144x128 fused render, L2 loss, full backward",
                    "grid": list(GRID), "views": VIEWS, "image_hw": list(IMAGE_HW), "depth_slices": DEPTH,
-                   "graph": bool(args.graph)},
+                   "graph": not args.eager},
         "ray_samples_per_s": pixels * DEPTH / (best_ms * 1e-3),
         "hbm_frac_step": step_bytes / (best_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "ms_per_step_eager": ms,
         "ms_per_step_graph": ms_graph,
+        "graph_check_max_rel": graph_check,
         "ms_per_step_plain": ms_plain,
         "vs_plain": (ms_plain / best_ms) if ms_plain else None,
         "roofline": {"kernel": "ngt_" + dominant.replace("render", "render_density_l2") + " (" +
