@@ -413,7 +413,7 @@ NGT_DEVICE float from_other_half(float v) {
 // sees 0 and keeps its own adds.
 template <int C>
 NGT_DEVICE void plane_add_row_merged(float* tile, bool on, int base, int dx, int dy, Vec<C> (&v)[4]) {
-  static_assert(kBrickX == 32, "the two halves of a wave are consecutive brick rows");
+  static_assert(kBrickX == 32 && kBrickY % 2 == 0, "the two halves of a wave are consecutive brick rows");
   const int fx = dx != 0 ? 1 : 0, fy = dy != 0 ? 1 : 0;
   // x: my x1 column (slots base + dx, base + dx + dy) to the next lane
   const int give_key = on ? 2 * (base + dx) + 2 + fy : 0;
