@@ -369,8 +369,8 @@ inline BrickGrid brick_grid(const Dims& d) {
 template <int C>
 struct HeldPlane {
   int base = -1;  // tile slot of corner (x0, y0), -1 = nothing held
-  int dx = 0;     // slot step from x0 to x1 (0 or 1)
-  int dy = 0;     // slot step from y0 to y1 (0 or kTileX)
+  int dx = 0;     // slot step from x0 to x1: 0 or 1, or across the tile where WRAP wraps x1
+  int dy = 0;     // slot step from y0 to y1: 0 or kTileX, or across the tile under WRAP
   Vec<C> v[4];    // corners (x0,y0), (x1,y0), (x0,y1), (x1,y1)
 };
 
