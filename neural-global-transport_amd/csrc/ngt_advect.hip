@@ -414,10 +414,15 @@ NGT_DEVICE float from_other_half(float v) {
 template <int C>
 NGT_DEVICE void plane_add_row_merged(float* tile, bool on, int base, int dx, int dy, Vec<C> (&v)[4]) {
   static_assert(kBrickX == 32 && kBrickY % 2 == 0, "the two halves of a wave are consecutive brick rows");
-  const int fx = dx != 0 ? 1 : 0, fy = dy != 0 ? 1 : 0;
+  // keys carry the exact slot steps (WRAP can make them anything across the
+  // tile): x column = (first slot, y row step), y row = (first slot, x step);
+  // 0 = nothing offered
+  static_assert(kTileX <= 64 && kTileY <= 16 && kTileCells < (1 << 18), "merge key ranges");
+  const int ry = dy / kTileX + 16;  // 5 bits
+  const int rx = dx + 64;           // 7 bits
   // x: my x1 column (slots base + dx, base + dx + dy) to the next lane
-  const int give_key = on ? 2 * (base + dx) + 2 + fy : 0;
-  const bool take_x = on && from_prev_lane(give_key) == 2 * base + 2 + fy;
+  const int give_key = on ? ((base + dx + 1) << 5) | ry : 0;
+  const bool take_x = on && from_prev_lane(give_key) == (((base + 1) << 5) | ry);
 #pragma unroll
   for (int i = 0; i < C; ++i) {
     const float r0 = from_prev_lane(v[1].v[i]), r1 = from_prev_lane(v[3].v[i]);
@@ -429,9 +434,9 @@ NGT_DEVICE void plane_add_row_merged(float* tile, bool on, int base, int dx, int
   const bool own1 = !(from_next_lane(take_x ? 1 : 0) != 0);  // still adds its x1 column
   // y: the y1 row (slots base + dy, base + dy + dx) of a lane of row y to lane + 32
   const bool upper = __lane_id() >= 32;
-  const int ykey = (on && !upper) ? ((2 * (base + dy) + 2 + fx) << 1) | (own1 ? 1 : 0) : 0;
+  const int ykey = (on && !upper) ? ((((base + dy + 1) << 7) | rx) << 1) | (own1 ? 1 : 0) : 0;
   const int got = from_other_half(ykey);
-  const bool take_y = on && upper && (got >> 1) == 2 * base + 2 + fx;
+  const bool take_y = on && upper && (got >> 1) == (((base + 1) << 7) | rx);
   const bool take_y3 = take_y && (got & 1) && own1;  // the x1 slot of the row, where both still own it
 #pragma unroll
   for (int i = 0; i < C; ++i) {

@@ -399,3 +399,39 @@ def test_advect_three_channels_self_advection(ngt_ops, clamp):
     out.backward(dev(g[None, None]))
     eg_in, eg_v = A.warp_backward(vc, vc, g, 1.0, 2, clamp, BORDER)
     assert rel(host(vt.grad)[0], eg_in + eg_v) < GRAD_TOL
+
+
+@pytest.mark.parametrize("shape,C,order,clamp,staggered", [((5, 6, 7), 1, 2, "MC_SMOOTH", True),
+                                                            ((4, 9, 13), 2, 2, "MC_SMOOTH", False),
+                                                            ((6, 5, 20), 1, 1, "NONE", False),
+                                                            ((3, 7, 33), 2, 2, "NEGATIVE", True)])
+def test_advect_wrap_grids_narrower_than_a_tile(ngt_ops, shape, C, order, clamp, staggered):
+    """WRAP on grids narrower than the backward's LDS tile: wrapped corners
+    land inside the tile at slot steps other than 0 / 1, and neighbouring
+    work-items' corners coincide in wrapped and unwrapped arrangements."""
+    rng = np.random.default_rng(sum(shape) + 10 * C + order)
+    D, H, W = shape
+    rho = f32(np.stack([np.clip(smooth_field(rng, shape, 1.0), -0.2, 1.0) for _ in range(C)], -1))
+    if staggered:
+        vx = f32(rng.uniform(-1.9, 1.9, (D, H, W + 1)))[..., None]
+        vy = f32(rng.uniform(-1.9, 1.9, (D, H + 1, W)))[..., None]
+        vz = f32(rng.uniform(-1.9, 1.9, (D + 1, H, W)))[..., None]
+        vc = A.centered_from_staggered(vx, vy, vz)
+        vel = (dev(vx[None]).requires_grad_(), dev(vy[None]).requires_grad_(), dev(vz[None]).requires_grad_())
+    else:
+        vc = f32(rng.uniform(-1.9, 1.9, shape + (3,)))
+        vel = dev(vc[None]).requires_grad_()
+    exp = A.warp_forward(rho, vc, 1.0, order, clamp, WRAP)
+    rt = dev(rho[None]).requires_grad_()
+    out = ngt_ops.advect_grid_semi_langrange(rt, vel, timestep=-1.0, order=order, clamp_extrema=clamp,
+                                             boundary="WRAP", separate_velocity_batch=False)
+    assert rel(host(out)[0, 0], exp) < FWD_TOL
+    g = f32(rng.standard_normal(exp.shape))
+    out.backward(dev(g[None, None]))
+    eg_rho, eg_v = A.warp_backward(rho, vc, g, 1.0, order, clamp, WRAP)
+    assert rel(host(rt.grad)[0], eg_rho) < GRAD_TOL
+    if staggered:
+        for t, e in zip(vel, A.centered_from_staggered_adjoint(eg_v)):
+            assert rel(host(t.grad)[0], e) < GRAD_TOL
+    else:
+        assert rel(host(vel.grad)[0], eg_v) < GRAD_TOL
