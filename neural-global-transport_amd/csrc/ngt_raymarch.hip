@@ -413,8 +413,8 @@ NGT_DEVICE int64_t g_offset(int k, int64_t pixel, int kcap) {
 struct alignas(16) RayRec {
   float sx, sy, sz, dx, dy, dz;
   float g_scale, g_off;
-  int steps;
-  int pad[3];
+  int steps;    // samples recorded (at most the plan's kcap)
+  int pad[3];   // pad[0]: the ray's own sample count; steps < pad[0] would mean a cut ray
 };
 
 // Image gradient of the fused L2 loss at one pixel (img: the pixel's C
@@ -443,6 +443,7 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
   constexpr int CG = GradChannels<C, LIGHT>::value;
   const float* src = ray_source<CIN, BRICKED>(a, cp.data_index);
   Ray r = make_ray<BM>(a, cp, x, y);
+  const int all_steps = r.steps;
   r.steps = min(r.steps, kcap);
   const int pair = pair_base + static_cast<int>(vb.z);
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
@@ -450,6 +451,7 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
   RayRec rec{};
   rec.sx = r.sx; rec.sy = r.sy; rec.sz = r.sz; rec.dx = r.dx; rec.dy = r.dy; rec.dz = r.dz;
   rec.steps = r.steps;
+  rec.pad[0] = all_steps;
   rec.g_scale = 1.f;
   rec.g_off = 0.f;
   rays[pair * npix + pixel] = rec;
@@ -546,6 +548,7 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   RayRec rec{};
   rec.sx = r.sx; rec.sy = r.sy; rec.sz = r.sz; rec.dx = r.dx; rec.dy = r.dy; rec.dz = r.dz;
   rec.steps = rec_steps;
+  rec.pad[0] = r.steps;
   rec.g_scale = a.depth_step * 2.f * a.l2_scale * (a.light_r * d0 + a.light_g * d1 + a.light_b * d2);
   rec.g_off = MODE == BEER_LAMBERT ? q : 0.f;
   rays[static_cast<int64_t>(pair) * npix + pixel] = rec;
@@ -1132,54 +1135,98 @@ size_t bricked_bytes(const int64_t* shape, int channels) {
 // ------------------------------------------------------ gather planning ----
 
 // Upper bound of the samples per BORDER ray: steps = int(len / |step|) + 2
-// with len <= (cell-centre AABB diagonal + 2); |step| is minimised over a set
-// of pixels (it grows away from the principal point) and taken 3% smaller.
+// with len <= (cell-centre AABB diagonal + 2).  |step| is the exact minimum
+// over the image: when the inverse projection's last row has no x / y terms
+// (every OpenGL perspective or orthographic camera, centred or not), the
+// view-space depth of an NDC depth does not depend on the pixel, so the
+// object-space step s(ix, iy) = (IDXtoOS(ix, iy, zmax + .5) - IDXtoOS(ix, iy,
+// .5)) / zmax is affine in the pixel coordinates and |s|^2 is a convex
+// quadratic, minimised in closed form over the rectangle of pixel centres
+// (interior stationary point or the minimum of each edge); any other
+// projection evaluates every pixel.  1 % slack absorbs the kernels' fp32
+// rounding of the ray setup.
+double min_ray_step(const CamParams& cp, const RayArgs& a) {
+  const double zmax = static_cast<double>(std::max(a.out_d - 1, 1));
+  auto to_os = [&](double ix, double iy, double iz, double o[3]) {
+    const double n[4] = {ix / a.out_w * 2.0 - 1.0, iy / a.out_h * 2.0 - 1.0, iz / a.out_d * 2.0 - 1.0, 1.0};
+    double vs[4];
+    for (int r = 0; r < 4; ++r) {
+      vs[r] = 0.0;
+      for (int c = 0; c < 4; ++c) vs[r] += static_cast<double>(cp.p_inv[r * 4 + c]) * n[c];
+    }
+    for (int r = 0; r < 4; ++r) vs[r] /= vs[3];
+    for (int r = 0; r < 3; ++r) {
+      o[r] = 0.0;
+      for (int c = 0; c < 4; ++c) o[r] += static_cast<double>(cp.mv_inv[r * 4 + c]) * vs[c];
+    }
+  };
+  auto step_at = [&](double ix, double iy, double s[3]) {
+    double s0[3], s1[3];
+    to_os(ix, iy, 0.5, s0);
+    to_os(ix, iy, zmax + 0.5, s1);
+    for (int r = 0; r < 3; ++r) s[r] = (s1[r] - s0[r]) / zmax;
+  };
+  auto dot = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+  const double u0 = 0.5, u1 = a.out_w - 0.5, v0 = 0.5, v1 = a.out_h - 0.5;
+  double best = 1e300;
+  if (cp.p_inv[12] == 0.f && cp.p_inv[13] == 0.f) {
+    // s(u, v) = c + u A + v B
+    double c[3], A[3], B[3], sa[3], sb[3];
+    step_at(0.0, 0.0, c);
+    step_at(1.0, 0.0, sa);
+    step_at(0.0, 1.0, sb);
+    for (int r = 0; r < 3; ++r) { A[r] = sa[r] - c[r]; B[r] = sb[r] - c[r]; }
+    const double aa = dot(A, A), ab = dot(A, B), bb = dot(B, B), ac = dot(A, c), bc = dot(B, c);
+    auto f = [&](double u, double v) {
+      double w[3];
+      for (int r = 0; r < 3; ++r) w[r] = c[r] + u * A[r] + v * B[r];
+      return dot(w, w);
+    };
+    auto clampd = [](double x, double lo, double hi) { return std::min(std::max(x, lo), hi); };
+    const double det = aa * bb - ab * ab;
+    if (det > 1e-12 * aa * bb) {
+      const double u = (-ac * bb + bc * ab) / det, v = (-bc * aa + ac * ab) / det;
+      if (u >= u0 && u <= u1 && v >= v0 && v <= v1) best = f(u, v);
+    }
+    for (double u : {u0, u1}) {  // edges u = const: minimise over v
+      const double v = bb > 0.0 ? clampd(-(bc + u * ab) / bb, v0, v1) : v0;
+      best = std::min(best, f(u, v));
+    }
+    for (double v : {v0, v1}) {  // edges v = const: minimise over u
+      const double u = aa > 0.0 ? clampd(-(ac + v * ab) / aa, u0, u1) : u0;
+      best = std::min(best, f(u, v));
+    }
+  } else {
+    for (int y = 0; y < a.out_h; ++y)
+      for (int x = 0; x < a.out_w; ++x) {
+        double s[3];
+        step_at(x + 0.5, y + 0.5, s);
+        best = std::min(best, dot(s, s));
+      }
+  }
+  return std::sqrt(std::max(best, 0.0));
+}
+
 int ray_step_cap(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* Fr,
                  const RayArgs& a) {
   double min_step = 1e300;
-  const double zmax = static_cast<double>(std::max(a.out_d - 1, 1));
-  std::vector<int> xs, ys;
-  auto pick = [](int n, std::vector<int>& v) {
-    for (int i = 0; i < 9; ++i) v.push_back(static_cast<int>((static_cast<long>(n - 1) * i) / 8));
-    v.push_back((n - 1) / 2);
-    v.push_back(n / 2);
-  };
-  pick(a.out_w, xs);
-  pick(a.out_h, ys);
   for (const Pair& p : pairs) {
     const CamParams cp = make_cam(M + 16 * p.n, V + 16 * p.v, P + 16 * p.v, Fr + 6 * p.v, p.n, p.out);
-    auto to_os = [&](double ix, double iy, double iz, double o[3]) {
-      const double n[4] = {ix / a.out_w * 2.0 - 1.0, iy / a.out_h * 2.0 - 1.0, iz / a.out_d * 2.0 - 1.0, 1.0};
-      double vs[4];
-      for (int r = 0; r < 4; ++r) {
-        vs[r] = 0.0;
-        for (int c = 0; c < 4; ++c) vs[r] += static_cast<double>(cp.p_inv[r * 4 + c]) * n[c];
-      }
-      for (int r = 0; r < 4; ++r) vs[r] /= vs[3];
-      for (int r = 0; r < 3; ++r) {
-        o[r] = 0.0;
-        for (int c = 0; c < 4; ++c) o[r] += static_cast<double>(cp.mv_inv[r * 4 + c]) * vs[c];
-      }
-    };
-    for (int y : ys)
-      for (int x : xs) {
-        double s0[3], s1[3];
-        to_os(x + 0.5, y + 0.5, 0.5, s0);
-        to_os(x + 0.5, y + 0.5, zmax + 0.5, s1);
-        const double dx = (s1[0] - s0[0]) / zmax, dy = (s1[1] - s0[1]) / zmax, dz = (s1[2] - s0[2]) / zmax;
-        const double l = std::sqrt(dx * dx + dy * dy + dz * dz);
-        if (l > 0.0 && l < min_step) min_step = l;
-      }
+    const double l = min_ray_step(cp, a);
+    if (l > 0.0 && l < min_step) min_step = l;
   }
   const double diag = std::sqrt(static_cast<double>(a.din.x - 1) * (a.din.x - 1) +
                                 static_cast<double>(a.din.y - 1) * (a.din.y - 1) +
                                 static_cast<double>(a.din.z - 1) * (a.din.z - 1));
+  // beyond this the per-sample record would outgrow the grid by far: -1, and
+  // the callers take the scatter adjoint (no gather plan) instead
   const int hard_cap = 16 * (a.din.x + a.din.y + a.din.z + a.out_d);
-  if (!(min_step < 1e300)) return hard_cap;
-  const double k = std::ceil((diag + 2.0) / (0.97 * min_step)) + 3.0;
+  if (!(min_step < 1e300)) return -1;
+  const double k = std::ceil((diag + 2.0) / (0.99 * min_step)) + 3.0;
+  if (!(k <= hard_cap)) return -1;
   // a multiple of kUnroll: the per-sample gradient rows are written kUnroll
   // samples (16 bytes) at a time
-  const int cap = static_cast<int>(std::min<double>(k, hard_cap));
+  const int cap = static_cast<int>(k);
   return (cap + kUnroll - 1) / kUnroll * kUnroll;
 }
 
@@ -1203,6 +1250,7 @@ size_t gather_workspace_bytes(const int64_t* shape, const float* M, const float*
   if (make_pairs(static_cast<int>(shape[0]), num_cameras, separate != 0, pairs) != NGT_OK) return 0;
   const RayArgs a = make_ray_args(nullptr, shape, out_dhw);
   const int kcap = ray_step_cap(pairs, M, V, P, F, a);
+  if (kcap <= 0) return 0;
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   return align_up(pairs.size() * npix * sizeof(RayRec)) +
          align_up(pairs.size() * static_cast<size_t>(kcap) * g_pixels(npix) * grad_channels * sizeof(float)) +
@@ -1215,6 +1263,7 @@ bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V,
   if (interpolation != LINEAR || boundary != BORDER || workspace == nullptr || !bricked_fits(a.din, grad_channels))
     return false;
   gp.kcap = ray_step_cap(pairs, M, V, P, F, a);
+  if (gp.kcap <= 0) return false;
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   const size_t rays_bytes = align_up(pairs.size() * npix * sizeof(RayRec));
   const size_t g_bytes =
@@ -1528,6 +1577,16 @@ size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape
                                 interpolation, boundary, separate_camera_batch, static_cast<int>(input_shape[4]));
 }
 
+int ngt_raymarch_step_cap(const int64_t* input_shape, const float* matrix_m, const float* matrix_v,
+                          const float* matrix_p, const float* frustum, int num_cameras, const int32_t* output_dhw,
+                          int separate_camera_batch) {
+  if (!input_shape || !matrix_m || !matrix_v || !matrix_p || !frustum || !output_dhw || num_cameras <= 0) return -1;
+  std::vector<Pair> pairs;
+  if (make_pairs(static_cast<int>(input_shape[0]), num_cameras, separate_camera_batch != 0, pairs) != NGT_OK)
+    return -1;
+  return ray_step_cap(pairs, matrix_m, matrix_v, matrix_p, frustum, make_ray_args(nullptr, input_shape, output_dhw));
+}
+
 int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_shape, const float* output,
                                     const float* output_grad, const float* matrix_m, const float* matrix_v,
                                     const float* matrix_p, const float* frustum, int num_cameras,
@@ -1807,6 +1866,8 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     for (int v = 0; v < q.num_views; ++v)
       it.cb.cam[v] = make_cam(q.matrix_m, q.matrix_v + 16 * v, q.matrix_p + 16 * v, q.frustum + 6 * v, 0, v);
     it.kcap = ray_step_cap(pairs, q.matrix_m, q.matrix_v, q.matrix_p, q.frustum, it.a);
+    NGT_REQUIRE(it.kcap > 0, NGT_ERR_INVALID_ARGUMENT,
+                "batch render: rays too long for the gather adjoint's per-sample record");
     it.ntx = static_cast<int>(blocks_for(it.a.out_w, 16));
     it.nty = static_cast<int>(blocks_for(it.a.out_h, 16));
     NGT_REQUIRE(bricked_fits(it.a.din, 1), NGT_ERR_INVALID_ARGUMENT, "batch density grid too large");

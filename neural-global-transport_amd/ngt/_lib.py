@@ -45,6 +45,7 @@ _SIGNATURES = {
     "ngt_raymarch_grid_transform_fwd_workspace_size": (_sz, [_c_i64p, _i]),
     "ngt_raymarch_grid_transform_fwd": (_i, [_vp, _c_i64p, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i, _i, _i,
                                              _vp, _vp, _sz, _vp]),
+    "ngt_raymarch_step_cap": (_i, [_c_i64p, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i]),
     "ngt_raymarch_grid_transform_bwd_workspace_size": (_sz, [_c_i64p, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i,
                                                             _i, _i]),
     "ngt_raymarch_grid_transform_bwd": (_i, [_vp, _c_i64p, _vp, _vp, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i,

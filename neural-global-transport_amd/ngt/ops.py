@@ -549,9 +549,11 @@ class _RenderDensityL2(torch.autograd.Function):
             _lib.dhw(out_shape), interp, boundary, mode, _lib.ptr(target), float(weight),
             None if bricked is None else _lib.ptr(bricked), None if record is None else _lib.ptr(record), rbytes,
             _lib.ptr(out), _lib.ptr(loss), _lib.ptr(ws), nbytes, _lib.stream(density.device)))
-        ctx.bricked = bricked
-        ctx.record = (record, rbytes)
-        ctx.save_for_backward(density, target, out)
+        # the brick copy and the recorded gradients are saved like the inputs,
+        # so that a checkpointed caller drops them with everything else and
+        # the recomputed forward makes them again
+        ctx.save_for_backward(density, target, out, bricked, record)
+        ctx.record_bytes = rbytes
         ctx.args = (light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight)
         ctx.mark_non_differentiable(out)
         ctx.set_materialize_grads(False)  # no zero-filled gradient for the images
@@ -559,7 +561,7 @@ class _RenderDensityL2(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_loss, grad_out):
-        density, target, out = ctx.saved_tensors
+        density, target, out, bricked, record = ctx.saved_tensors
         light_rgb, depth_step, cams, out_shape, interp, boundary, mode, weight = ctx.args
         lib = _lib.load()
         if grad_loss is None:
@@ -568,7 +570,7 @@ class _RenderDensityL2(torch.autograd.Function):
         g = _lib.empty_grad(density)
         lp, _, keep = _lib.host_floats(light_rgb, 3)
         s_d = _lib.shape5(density)
-        record, rbytes = ctx.record
+        rbytes = ctx.record_bytes
         if record is not None:
             ws, nbytes = record, rbytes
         else:
@@ -578,7 +580,6 @@ class _RenderDensityL2(torch.autograd.Function):
             else:
                 nb = out.numel() * 4
             ws, nbytes = _lib.workspace(nb, density.device)
-        bricked = ctx.bricked
         _lib.check(lib.ngt_render_density_l2_bwd(
             _lib.ptr(density), s_d, lp, float(depth_step), _lib.ptr(out), _lib.ptr(target), float(weight),
             _lib.ptr(gl), cams.m, cams.v, cams.p, cams.f, cams.nv, _lib.dhw(out_shape), interp, boundary, mode,

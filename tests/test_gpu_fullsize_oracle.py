@@ -1,0 +1,152 @@
+"""Full-size parity against the FP64 oracle through committed fixtures
+(tests/golden/make_fullsize.py wrote them; the GPU run regenerates the same
+seeded inputs with ngt.synth and never runs the oracle itself).
+
+* The differentiable step at 128^3 (the benchmark's workload): MacCormack +
+  MC_SMOOTH on a MAC velocity, 5-view 256x144x128 BEER_LAMBERT render, L2
+  loss against a target rendered from the seed-3 density.  Loss, advected
+  density, images, and dL/drho at 64 voxels / dL/dv at 64 faces against
+  both the oracle's adjoint and FP64 central differences of the loss;
+  full-field checks by the L2 norm and a seeded random projection of every
+  field (a relative L2 error estimate of the whole field).
+* The forward render at 128^3 (16 blobs): the fused raymarch, the fused
+  density renderer (light formed in the kernel), and the plain path
+  sample_grid_transform (TRANSFORM_LINDEPTH, 128 slices) -> reduce_grid_blend,
+  all 5 views.
+
+Tolerances: forward values 1e-5 (of the field's max |.|), projections and
+norms 1e-5 of the field's L2 norm; gradients 1e-4 of max |g| per sample and
+1e-4 of ||g||_2 for projections and norms.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+GRID = (128, 128, 128)
+IMG = (144, 256)
+DEPTH = 128
+VIEWS = 5
+OUT_DHW = (DEPTH,) + IMG
+FWD_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def prob():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from ngt import ops, synth, _lib
+    from ngt.renderer import Renderer
+    _lib.load()
+    grid_t, cams = synth.ring_setup(GRID, IMG, DEPTH, VIEWS)
+    M, V, P, F = Renderer()._transform_params([grid_t], cams)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    rho = t(synth.blobs(GRID, 16, seed=2)[None])
+    faces = tuple(t(c[None]) for c in synth.curl_noise_staggered(GRID, seed=4, vmax=2.0))
+    ds = float(np.float32(cams[0].depth_step))
+    light = (1.0, 1.0, 1.0)
+    with torch.no_grad():
+        target = ops.render_density_fused(t(synth.blobs(GRID, 16, seed=3)[None]), light, ds, M, V, P, F,
+                                          OUT_DHW)[..., :3].contiguous()
+    return dict(ops=ops, M=M, V=V, P=P, F=F, rho=rho, faces=faces, ds=ds, light=light, target=target)
+
+
+def checks(field, seed):
+    f = np.asarray(field, np.float64)
+    r = np.random.default_rng(seed).standard_normal(f.shape)
+    return float(np.sqrt(np.sum(f * f))), float(np.sum(f * r))
+
+
+def check_field(got, gold, key_norm, key_proj, seed, tol, label):
+    norm, proj = checks(got, seed)
+    ref_norm, ref_proj = float(gold[key_norm]), float(gold[key_proj])
+    print("%s: norm %.9e (oracle %.9e), projection %.9e (oracle %.9e)" % (label, norm, ref_norm, proj, ref_proj),
+          flush=True)
+    assert abs(norm - ref_norm) <= tol * ref_norm, label
+    assert abs(proj - ref_proj) <= tol * ref_norm, label
+
+
+def test_step128_against_oracle(prob):
+    ops = prob["ops"]
+    gold = np.load(os.path.join(GOLD, "fullsize_step128.npz"))
+    rho = prob["rho"].clone().requires_grad_()
+    faces = tuple(f.clone().requires_grad_() for f in prob["faces"])
+    print("case: step 128^3, MC_SMOOTH + 5-view render + L2", flush=True)
+    adv = ops.advect_grid_semi_langrange(rho, faces, timestep=-1.0, order=2, clamp_extrema="MC_SMOOTH",
+                                         boundary="BORDER", separate_velocity_batch=False).squeeze(1)
+    loss, img = ops.render_density_fused_l2(adv, prob["target"], prob["light"], prob["ds"], prob["M"], prob["V"],
+                                            prob["P"], prob["F"], OUT_DHW)
+    loss.backward()
+    torch.cuda.synchronize()
+    # forward
+    lv = float(loss)
+    print("loss %.9e (oracle %.9e)" % (lv, float(gold["loss"])), flush=True)
+    assert abs(lv - float(gold["loss"])) <= FWD_TOL * float(gold["loss"])
+    a = adv.detach()[0].cpu().double().numpy()
+    check_field(a, gold, "adv_norm", "adv_proj", 15, FWD_TOL, "advected density")
+    assert abs(np.max(np.abs(a)) - float(gold["adv_max"])) <= FWD_TOL * float(gold["adv_max"])
+    im = img.detach()[0].cpu().double().numpy()
+    check_field(im, gold, "img_norm", "img_proj", 17, FWD_TOL, "images")
+    got = im[tuple(gold["img_pix"].T)]
+    assert np.max(np.abs(got - gold["img_val"])) <= FWD_TOL * float(gold["img_max"])
+    # dL/drho: 64 voxels against the oracle's adjoint and FP64 central differences
+    g = rho.grad[0].cpu().double().numpy()
+    gmax = float(gold["rho_gmax"])
+    got = g[tuple(gold["rho_idx"].T)][:, 0]
+    e_adj = np.max(np.abs(got - gold["rho_adj"])) / gmax
+    e_fd = np.max(np.abs(got - gold["rho_fd"])) / gmax
+    print("dL/drho at 64 voxels: |gpu - adjoint| %.2e, |gpu - central difference| %.2e of max |g|" % (e_adj, e_fd),
+          flush=True)
+    assert e_adj <= GRAD_TOL and e_fd <= GRAD_TOL
+    check_field(g, gold, "rho_gnorm", "rho_proj", 11, GRAD_TOL, "dL/drho")
+    # dL/dv: 64 faces over the three MAC components
+    gf = [f.grad[0].cpu().double().numpy() for f in faces]
+    fmax = gold["face_gmax"]
+    e_adj = e_fd = 0.0
+    for (k, z, y, x), adj, fd in zip(gold["face_idx"], gold["face_adj"], gold["face_fd"]):
+        v = gf[k][z, y, x, 0]
+        e_adj = max(e_adj, abs(v - adj) / fmax[k])
+        e_fd = max(e_fd, abs(v - fd) / fmax[k])
+    print("dL/dv at 64 faces: |gpu - adjoint| %.2e, |gpu - central difference| %.2e of max |g|" % (e_adj, e_fd),
+          flush=True)
+    assert e_adj <= GRAD_TOL and e_fd <= GRAD_TOL
+    for k in range(3):
+        norm, proj = checks(gf[k], 12 + k)
+        print("dL/dv%s: norm %.9e (oracle %.9e)" % ("xyz"[k], norm, gold["face_gnorm"][k]), flush=True)
+        assert abs(norm - gold["face_gnorm"][k]) <= GRAD_TOL * gold["face_gnorm"][k]
+        assert abs(proj - gold["face_proj"][k]) <= GRAD_TOL * gold["face_gnorm"][k]
+
+
+def test_render128_fused_and_plain(prob):
+    ops = prob["ops"]
+    gold = np.load(os.path.join(GOLD, "fullsize_render128.npz"))
+    rho, ds, light = prob["rho"], prob["ds"], prob["light"]
+    M, V, P, F = prob["M"], prob["V"], prob["P"], prob["F"]
+    cells = torch.cat([rho * light[0], rho * light[1], rho * light[2], rho], dim=-1) * ds
+    pix = tuple(gold["pix"].T)
+
+    def compare(img, key, label):
+        print("case: render 128^3, %s" % label, flush=True)
+        im = img.detach().cpu().double().numpy()
+        assert im.shape == (VIEWS,) + IMG + (4,)
+        err = np.max(np.abs(im[pix] - gold[key + "_val"])) / float(gold[key + "_max"])
+        print("%s: max error of 4096 pixels %.2e of max |img|" % (label, err), flush=True)
+        assert err <= FWD_TOL
+        check_field(im, gold, key + "_norm", key + "_proj", 21, FWD_TOL, label)
+
+    with torch.no_grad():
+        compare(ops.raymarch_grid_transform(cells, M, V, P, F, OUT_DHW)[0], "fused", "raymarch_grid_transform")
+        compare(ops.render_density_fused(rho, light, ds, M, V, P, F, OUT_DHW)[0], "fused", "render_density_fused")
+        plain = []
+        for v in range(VIEWS):
+            fr = ops.sample_grid_transform(cells, M, V[v:v + 1], P[v:v + 1], F[v:v + 1], OUT_DHW,
+                                           coordinate_mode="TRANSFORM_LINDEPTH")
+            plain.append(ops.reduce_grid_blend(fr[:, 0], "BEER_LAMBERT")[0])
+            del fr
+        compare(torch.stack(plain), "plain", "sample_grid_transform + reduce_grid_blend")

@@ -169,3 +169,61 @@ def test_bricked_grid_rejects_three_channels(ops):
     out = torch.empty(1 << 16, dtype=torch.uint8, device="cuda:0")
     rc = lib.ngt_bricked_grid(_lib.ptr(x), _lib.shape5(x), 0, _lib.ptr(out), _lib.stream(x.device))
     assert rc != 0 and b"Channel" in lib.ngt_last_error()
+
+
+def _record_rays(ops, rho, tgt, light, ds, M, V, P, F, out_dhw):
+    """The fused L2 forward with a record buffer, through the C ABI; returns
+    the per-ray records as int32 [pairs * pixels, 12] (RayRec: steps at 8,
+    the ray's own sample count at 9)."""
+    from ngt import _lib
+    lib = _lib.load()
+    cams = ops._Cams(M, V, P, F)
+    lp, _, keep = _lib.host_floats(light, 3)
+    s_d = _lib.shape5(rho)
+    od = _lib.dhw(out_dhw)
+    st = _lib.stream(rho.device)
+    brk = torch.empty(lib.ngt_bricked_grid_bytes(s_d), dtype=torch.uint8, device=rho.device)
+    _lib.check(lib.ngt_bricked_grid(_lib.ptr(rho), s_d, 0, _lib.ptr(brk), st))
+    rbytes = lib.ngt_render_density_l2_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv, od, 1, 0)
+    rec = torch.zeros(rbytes, dtype=torch.uint8, device=rho.device)
+    fws, fnb = _lib.workspace(lib.ngt_render_density_l2_fwd_workspace_size(s_d, 1, cams.nv, od), rho.device)
+    img = torch.empty((1, cams.nv) + tuple(out_dhw[1:]) + (4,), device=rho.device)
+    loss = torch.empty((), device=rho.device)
+    _lib.check(lib.ngt_render_density_l2_fwd(_lib.ptr(rho), s_d, lp, float(ds), cams.m, cams.v, cams.p, cams.f,
+                                             cams.nv, od, 1, 0, 0, _lib.ptr(tgt), 1.0, _lib.ptr(brk), _lib.ptr(rec),
+                                             rbytes, _lib.ptr(img), _lib.ptr(loss), _lib.ptr(fws), fnb, st))
+    torch.cuda.synchronize()
+    n = cams.nv * out_dhw[1] * out_dhw[2]
+    return rec[:n * 48].view(torch.int32).reshape(n, 12).cpu().numpy()
+
+
+def test_recorded_rays_are_never_cut(ops):
+    """Every recorded ray keeps all its samples (recorded count == the ray's
+    own count): the 128^3 five-view workload and off-axis, wide-angle and
+    inside-the-grid cameras."""
+    from ngt import synth
+    from ngt.renderer import Renderer
+    from oracle import geometry as G
+    grid_t, cams = synth.ring_setup((128, 128, 128), (144, 256), 128, 5)
+    M, V, P, F = Renderer()._transform_params([grid_t], cams)
+    scenes = [((128, 128, 128), (M, V, P, F), (128, 144, 256))]
+    dims = (24, 30, 20)
+    Mo = G.op_matrix(G.grid_transform(dims, rotation_deg=(9.0, -31.0, 4.0), center=True,
+                                      scale_=[1.2 / dims[2], 0.9 / dims[1], 1.0 / dims[0]])).astype(np.float32)
+    views = []
+    for dist, trbl, near, yaw, pitch in ((2.4, [0.9, 0.2, 0.1, -0.6], 1.4, 17.0, 0.0),
+                                         (1.9, [0.05, 1.6, -0.7, -0.1], 0.6, 40.0, 25.0),
+                                         (1.0, [0.8, 0.8, -0.8, -0.8], 0.3, -60.0, 10.0)):
+        C = G.rotation_xyz_deg([pitch, yaw, 0.0]) @ G.translation([0.0, 0.0, dist])
+        views.append((G.op_matrix(np.linalg.inv(C)).astype(np.float32),
+                      G.op_matrix(G.projection(near, dist + 1.5, trbl)).astype(np.float32),
+                      G.frustum_params(near, dist + 1.5, trbl).astype(np.float32)))
+    scenes.append((dims, ([Mo], [v[0] for v in views], [v[1] for v in views], np.stack([v[2] for v in views])),
+                   (40, 36, 48)))
+    for dims, (M, V, P, F), out_dhw in scenes:
+        print("case: record %s -> %d views %s" % (dims, len(V), out_dhw), flush=True)
+        rho = torch.from_numpy(synth.blobs(dims, 8, seed=5)[None]).cuda()
+        tgt = torch.zeros((1, len(V)) + tuple(out_dhw[1:]) + (3,), device="cuda:0")
+        rec = _record_rays(ops, rho, tgt, (1.0, 1.0, 1.0), 0.01, M, V, P, F, out_dhw)
+        assert rec[:, 9].max() > 0
+        np.testing.assert_array_equal(rec[:, 8], rec[:, 9])

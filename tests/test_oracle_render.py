@@ -304,3 +304,92 @@ def test_raymarch_reference_grad_normalisation_ortho():
     for z in range(d):
         np.testing.assert_allclose(ref[0, z], g[0, 0], atol=1e-9)
         np.testing.assert_allclose(exact[0, z], g[0, 0], atol=1e-9)
+
+
+# --- BORDER ray clip, pinned by an independent box intersection -----------------
+
+def _box_hits(origin, direction, lo, hi):
+    """Entry / exit ray parameters of the box [lo, hi] found by intersecting
+    the ray with each of the six face planes and keeping the hits that lie
+    on the face (not the slab formula); None when the ray misses."""
+    ts = []
+    for a in range(3):
+        if direction[a] == 0.0:
+            continue
+        for b in (lo[a], hi[a]):
+            t = (b - origin[a]) / direction[a]
+            p = origin + t * direction
+            others = [o for o in range(3) if o != a]
+            if all(lo[o] - 1e-9 <= p[o] <= hi[o] + 1e-9 for o in others):
+                ts.append(t)
+    if not ts:
+        return None
+    return min(ts), max(ts)
+
+
+def _border_band_weight(q, dims):
+    """BORDER trilinear read of a constant 1 field at q (per-axis in-grid
+    corner weights; 0 outside [-0.5, dims - 0.5))."""
+    w = 1.0
+    for a in range(3):
+        if not (-0.5 <= q[a] < dims[a] - 0.5):
+            return 0.0
+        w *= min(1.0, q[a] + 1.0, dims[a] - q[a])
+    return w
+
+
+@pytest.mark.parametrize("views,dist,near,dims,out_dhw", [(5, 3.0, 2.0, (12, 10, 14), (20, 9, 16)),
+                                                          (3, 1.2, 0.9, (9, 13, 11), (15, 12, 10))])
+def test_raymarch_border_clip_independent(views, dist, near, dims, out_dhw):
+    """Every pixel of perspective ring cameras (one set far outside the grid,
+    one whose near plane cuts the grid, so that rays start inside it): the first sample
+    is 1 unit along the normalised ray before the entry into the cell-centre
+    box [0.5, dims - 0.5] (or the unclipped start when that entry is less
+    than 1 unit ahead), the last sample lies past exit + 1 by at most one
+    step, misses are exactly the rays with no face hit ahead, and a constant
+    field's ADDITIVE render is the band-weighted count of those
+    independently generated positions (raymarch_grid.cu.cc:109-200)."""
+    M = G.grid_transform(dims, translation_=(0.013, -0.021, 0.008), center=True, scale_=[1.0 / max(dims)] * 3)
+    cams = G.look_at_ring_cameras(views, dist, 50.0, out_dhw[2] / out_dhw[1], near, dist + 1.0,
+                                  height=0.05, start_deg=7.0)
+    D, H, W = out_dhw
+    in_dims = np.array([dims[2], dims[1], dims[0]], dtype=np.float64)
+    lo, hi = np.full(3, 0.5), in_dims - 0.5
+    zmax = float(max(D - 1, 1))
+    const = 0.7
+    n_inside = n_miss = 0
+    for V, P, F in cams:
+        args = ([G.op_matrix(M)], [G.op_matrix(V)], [G.op_matrix(P)], [F], out_dhw)
+        cx = G.CameraXform(M, V, P, F)
+        start, end, step, steps, hit = RM.ray_setup(cx, out_dhw, dims, BORDER)
+        img = RM.raymarch_grid_transform(np.full((1,) + tuple(dims) + (1,), const), *args, boundary=BORDER,
+                                         blend_mode=B.ADDITIVE)[0, 0, ..., 0]
+        for y in range(H):
+            for x in range(W):
+                s0 = G.idx_to_os(cx, np.array([[x + 0.5, y + 0.5, 0.5]]), np.array([W, H, D], float))[0]
+                e0 = G.idx_to_os(cx, np.array([[x + 0.5, y + 0.5, zmax + 0.5]]), np.array([W, H, D], float))[0]
+                st = (e0 - s0) / zmax
+                d = (e0 - s0) / np.linalg.norm(e0 - s0)
+                box = _box_hits(s0, d, lo, hi)
+                if box is None or box[1] < 0.0:
+                    n_miss += 1
+                    assert not hit[y, x] and img[y, x] == 0.0
+                    continue
+                assert hit[y, x]
+                t_in, t_out = box
+                n_inside += t_in < 1.0
+                first = s0 + d * max(t_in - 1.0, 0.0)
+                np.testing.assert_allclose(start[y, x], first, rtol=0, atol=1e-9)
+                np.testing.assert_allclose(step[y, x], st, rtol=0, atol=1e-12)
+                last = start[y, x] + (steps[y, x] - 1) * step[y, x]
+                exit1 = s0 + d * (t_out + 1.0)
+                past = np.dot(last - exit1, d)
+                assert 0.0 < past <= np.linalg.norm(st) + 1e-9, (past, np.linalg.norm(st))
+                np.testing.assert_allclose(np.cross(last - s0, d), 0.0, atol=1e-9)
+                k = np.arange(steps[y, x])
+                pos = first[None] + k[:, None] * st[None]
+                expect = const * sum(_border_band_weight(p - 0.5, in_dims) for p in pos)
+                assert img[y, x] == pytest.approx(expect, rel=1e-12, abs=1e-12)
+    assert n_miss > 0
+    if dist < 2.0:
+        assert n_inside > 0
