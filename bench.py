@@ -73,6 +73,9 @@ OP_KERNELS = {
     "render_bwd": ["raymarch_bwd_brick_kernel"],
 }
 COPY_BYTES = 256 << 20
+# the op whose single kernel is the step's largest (see main)
+ROOFLINE_OP = "render_bwd"
+assert len(OP_KERNELS[ROOFLINE_OP]) == 1
 
 
 def traffic_probe():
@@ -336,12 +339,12 @@ def _oracle_fp64_estimate():
     return float(t_adv * np.prod(GRID) + t_ren * VIEWS * IMAGE_HW[0] * IMAGE_HW[1] * DEPTH)
 
 
-def cpu_baseline(reps=5):
+def cpu_baseline(reps=10, warmup=2):
     """The FP32 OpenMP CPU restatement (oracle/cpu_port.cpp) timed on this
     host's cores on the full workload: the same 128^3 MacCormack step and
-    5-view 256x144x128 render + L2 loss, forward and backward, ``reps`` timed
-    steps after one warm-up (median).  The FP64 NumPy oracle's single-thread
-    estimate is reported beside it."""
+    5-view 256x144x128 render + L2 loss, forward and backward, the median of
+    ``reps`` timed steps after ``warmup`` untimed ones.  The FP64 NumPy
+    oracle's single-thread estimate is reported beside it."""
     from oracle import cpu_port
     from ngt import synth
     rho = synth.blobs(GRID, 16, seed=2)[..., 0]
@@ -361,7 +364,8 @@ def cpu_baseline(reps=5):
         g_adv = cpu_port.render_l2_bwd(adv, M[0], V, P, (DEPTH,) + IMAGE_HW, light, ds, img, target)
         cpu_port.advect_mc_bwd(rho, vx, vy, vz, -1.0, sl, g_adv)
 
-    step()
+    for _ in range(warmup):
+        step()
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -372,8 +376,10 @@ def cpu_baseline(reps=5):
             "kind": "port",
             "sample": "FP32 OpenMP C++ restatement (oracle/cpu_port.cpp, g++ -O3 -march=x86-64-v3 -fopenmp) of the "
                       "full step: 128^3 MacCormack+MC_SMOOTH and 5-view 256x144x128 render + L2, fwd+bwd; median "
-                      "of %d steps after 1 warm-up" % reps,
-            "seconds_per_step": t_step, "cpu_model": _cpu_model(),
+                      "of %d steps after %d warm-ups" % (reps, warmup),
+            "seconds_per_step": t_step, "seconds_min_max": [float(min(times)), float(max(times))],
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+            "cpus_available": len(os.sched_getaffinity(0)), "openmp_threads": cpu_port.threads(),
             "oracle_fp64_1thread_seconds_per_step_est": _oracle_fp64_estimate()}
 
 
@@ -614,7 +620,11 @@ def main():
     #   render + L2 bwd 8 B/voxel + 28 B/pixel (image and RGB target in)
     algo = {"advect_fwd": cells * 20, "advect_bwd": cells * 36, "render_fwd": cells * 4 + pixels * 28,
             "render_bwd": cells * 8 + pixels * 28}
-    dominant = max(ops_ms, key=ops_ms.get)
+    # the roofline names the step's dominant kernel (rocprofv3 kernel-trace
+    # summary of this workload): the render adjoint's brick gather,
+    # raymarch_bwd_brick_kernel, which the render + L2 backward launches alone,
+    # so that call's HIP-event duration is the kernel's
+    dominant = ROOFLINE_OP
     ms_kernel = ops_ms[dominant]
     kbytes = algo[dominant]
     achieved = kbytes / (ms_kernel * 1e-3) / 1e9
@@ -651,8 +661,8 @@ def main():
         "graph_check_max_rel": graph_check,
         "ms_per_step_plain": ms_plain,
         "vs_plain": (ms_plain / best_ms) if ms_plain else None,
-        "roofline": {"kernel": "ngt_" + dominant.replace("render", "render_density_l2") + " (" +
-                               " + ".join(OP_KERNELS[dominant]) + ")",
+        "roofline": {"kernel": " + ".join(OP_KERNELS[dominant]) + " (the whole ngt_" +
+                               dominant.replace("render", "render_density_l2") + " call)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "ms_per_launch": ms_kernel,
                      "algorithmic_bytes_per_launch": kbytes},

@@ -112,7 +112,8 @@ def _scatter(shape, p, g, boundary, semantics):
     return S.scatter_linear(shape, p, g, boundary, mask="forward")
 
 
-def warp_backward(rho, v, g_out, dt=1.0, order=1, clamp="NONE", boundary=BORDER, semantics="exact"):
+def warp_backward(rho, v, g_out, dt=1.0, order=1, clamp="NONE", boundary=BORDER, semantics="exact",
+                  decisions=None):
     """Gradient of ``warp_forward`` w.r.t. (rho, v_centered).
 
     semantics="exact": the true adjoint (the where / max masks follow the
@@ -121,11 +122,19 @@ def warp_backward(rho, v, g_out, dt=1.0, order=1, clamp="NONE", boundary=BORDER,
     semantics="reference": the same chain with each SampleGridLutGrad input
     gradient divided by its corner-hit count, as TF's autodiff over the
     reference ops computes it.
+    decisions: optional {"oob": mask, "neg_pass": mask} replacing the
+    forward's branch choices (the MC_SMOOTH revert mask, the NEGATIVE pass
+    mask), e.g. another evaluation's choices at ties, where the forward is
+    not differentiable and either branch is a valid one-sided derivative.
     """
     rho = np.asarray(rho, dtype=np.float64)
     v = np.asarray(v, dtype=np.float64)
     g = np.asarray(g_out, dtype=np.float64).copy()
     out, parts = warp_forward(rho, v, dt, order, clamp, boundary, return_parts=True)
+    for key, mask in (decisions or {}).items():
+        if key not in parts:
+            raise ValueError("no branch choice '%s' in this warp" % key)
+        parts[key] = np.asarray(mask, dtype=bool)
     clamp = clamp.upper()
     i = _idx(rho.shape[:3])
     lut = -dt * v

@@ -24,7 +24,11 @@ fullsize_step128.npz - the differentiable step at 128^3:
     advected value leaving [0, inf) where the blend's max(d + sigma, 0)
     switches) are redrawn;
   * per gradient field: max |g|, ||g||_2 and <g, r> with r a seeded standard
-    normal field (seeds 11 rho, 12-14 faces).
+    normal field (seeds 11 rho, 12-14 faces), over the entries whose value
+    the arithmetic fixes: left out (and listed, rho_out / face_out) are the
+    faces beside a velocity component whose fp32 sample position may round
+    onto an integer and the voxels / faces that MC_SMOOTH ties reach
+    (tests/advect_masks.py).
 fullsize_render128.npz - the forward render at 128^3 (16 blobs, seed 2):
   fused raymarch (raymarch_grid_transform) and the plain path
   (sample_grid_transform TRANSFORM_LINDEPTH into 128 depth slices, then
@@ -41,6 +45,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "neural-global-transport_amd"))
+sys.path.insert(0, os.path.dirname(HERE))
 
 from oracle import BORDER, LINEAR  # noqa: E402
 from oracle import advect as A  # noqa: E402
@@ -48,6 +53,7 @@ from oracle import blend as B  # noqa: E402
 from oracle import geometry as G  # noqa: E402
 from oracle import raymarch as RM  # noqa: E402
 from oracle import resample as R  # noqa: E402
+import advect_masks as AM  # noqa: E402
 
 GRID = (128, 128, 128)
 IMG = (144, 256)
@@ -274,9 +280,10 @@ def make_step():
                                               B.BEER_LAMBERT)[0]
     log("render adjoint %.1fs" % (time.time() - t0))
     g_adv = ds * (g_cells[..., :3] @ LIGHT + g_cells[..., 3])[..., None]
-    g_rho, g_vc = A.warp_backward(rho, vc, g_adv, 1.0, 2, "MC_SMOOTH", BORDER)
+    g_rho, g_vc, rho_keep, vel_keep, info = AM.oracle_adjoint(rho, vc, g_adv, 2, "MC_SMOOTH", BORDER)
     g_faces = A.centered_from_staggered_adjoint(g_vc)
-    log("advect adjoint %.1fs" % (time.time() - t0))
+    face_keep = [~m for m in AM.faces_of_cells(~vel_keep)]
+    log("advect adjoint %.1fs, left out: %s" % (time.time() - t0, info))
 
     rays = Rays(cams)
     # the local march reproduces the full render
@@ -292,6 +299,8 @@ def make_step():
     for c in pick(rng, g_rho, 200, taken):
         if len(idx) == 64:
             break
+        if not rho_keep[c]:
+            continue
         d = fd_point(state, "rho", c)
         if d is None:
             redrawn += 1
@@ -306,7 +315,9 @@ def make_step():
                                                                                            time.time() - t0))
     assert err < 1e-6, err
     out.update(rho_idx=np.array(idx, np.int64), rho_fd=np.array(fd), rho_adj=np.array(adj), rho_redrawn=redrawn)
-    out["rho_gmax"], out["rho_gnorm"], out["rho_proj"] = checksums(g_rho, 11)
+    out["rho_gmax"], out["rho_gnorm"], out["rho_proj"] = checksums(np.where(rho_keep[..., None], g_rho, 0.0), 11)
+    out["rho_gmax"] = float(np.max(np.abs(g_rho)))
+    out["rho_out"] = np.argwhere(~rho_keep).astype(np.int64)
 
     fidx, ffd, fadj, redrawn = [], [], [], 0
     for k, count in ((0, 22), (1, 21), (2, 21)):
@@ -315,6 +326,8 @@ def make_step():
         for c in pick(rng, g_faces[k], 200, taken):
             if got == count:
                 break
+            if not face_keep[k][c][0]:
+                continue
             d = fd_point(state, k, c)
             if d is None:
                 redrawn += 1
@@ -331,8 +344,11 @@ def make_step():
     assert err < 1e-6, err
     out.update(face_idx=np.array(fidx, np.int64), face_fd=np.array(ffd), face_adj=np.array(fadj),
                face_redrawn=redrawn)
-    sums = [checksums(g, 12 + k) for k, g in enumerate(g_faces)]
-    out["face_gmax"] = np.array([s[0] for s in sums])
+    sums = [checksums(np.where(m, g, 0.0), 12 + k) for k, (g, m) in enumerate(zip(g_faces, face_keep))]
+    out["face_gmax"] = np.array([np.max(np.abs(g)) for g in g_faces])
+    out["face_out"] = np.concatenate([np.concatenate([np.full((len(ix), 1), k), ix], 1)
+                                      for k, ix in enumerate(np.argwhere(~m[..., 0]) for m in face_keep)]).astype(
+        np.int64)
     out["face_gnorm"] = np.array([s[1] for s in sums])
     out["face_proj"] = np.array([s[2] for s in sums])
 

@@ -17,7 +17,12 @@ cell by cell with the oracle's.
 
 Tolerances: forward 1e-5 and gradients 1e-4, normwise relative
 (max |gpu - oracle| / max |oracle|); revert masks exact outside an fp32
-rounding margin.
+rounding margin.  Gradients are compared everywhere except at the
+non-differentiable points that fp32 and fp64 may resolve differently
+(tests/advect_masks.py: velocity components whose fp32 sample position
+rounds onto an integer, and the influence of MC_SMOOTH ties whose device
+choice is not visible in its outputs); each test prints how many cells that
+leaves out.
 """
 import numpy as np
 import pytest
@@ -25,6 +30,8 @@ import torch
 
 from oracle import BORDER, CLAMP, WRAP
 from oracle import advect as A
+
+import advect_masks as AM
 
 pytestmark = pytest.mark.gpu
 
@@ -70,7 +77,8 @@ def smooth_field(rng, shape, amp):
 
 def _case_inputs(rng, shape, C, staggered, vmax):
     D, H, W = shape
-    rho = f32(np.stack([np.clip(smooth_field(rng, shape, 1.0), -0.2, 1.0) for _ in range(C)], -1))
+    # smooth, never flat (no clipping): branch ties stay isolated
+    rho = f32(np.stack([0.4 + 0.6 * smooth_field(rng, shape, 1.0) for _ in range(C)], -1))
     if staggered:
         faces = [f32(smooth_field(rng, (D, H, W + 1), vmax))[..., None],
                  f32(smooth_field(rng, (D, H + 1, W), vmax))[..., None],
@@ -88,12 +96,28 @@ def _vel_leaves(faces, vc):
     return dev(vc[None]).requires_grad_()
 
 
-def _check_vel_grad(vel, faces, eg_v):
+def _gpu_parts(ngt_ops, rho_np, vel_np, order, clamp, bound, out):
+    """The device's SL and unclamped MacCormack results for the same inputs
+    (its branch choices are read off them), with its tested result."""
+    if clamp == "NONE":
+        return None
+    args = dict(timestep=-1.0, boundary=BNAME[bound], separate_velocity_batch=False)
+    with torch.no_grad():
+        sl = ngt_ops.advect_grid_semi_langrange(rho_np, vel_np, order=1, clamp_extrema="NONE", **args)
+        corr = ngt_ops.advect_grid_semi_langrange(rho_np, vel_np, order=2, clamp_extrema="NONE", **args)
+    return {"sl": host(sl)[0, 0], "corr": host(corr)[0, 0], "out": host(out)[0, 0]}
+
+
+def _check_grads(rt, vel, faces, rho, vc, g, order, clamp, bound, gpu, label=""):
+    eg_rho, eg_v, rho_keep, vel_keep, info = AM.oracle_adjoint(rho, vc, g, order, clamp, bound, gpu)
+    print("%s left out: %s" % (label, info), flush=True)
+    assert info["rho_left_out"] <= 0.005 * rho_keep.size and info["vel_left_out"] <= 0.005 * vel_keep.size
+    assert AM.rel_on(host(rt.grad)[0], eg_rho, rho_keep[..., None]) < GRAD_TOL
     if faces is not None:
-        for t, e in zip(vel, A.centered_from_staggered_adjoint(eg_v)):
-            assert rel(host(t.grad)[0], e) < GRAD_TOL
+        for t, e, m in zip(vel, A.centered_from_staggered_adjoint(eg_v), AM.faces_of_cells(~vel_keep)):
+            assert AM.rel_on(host(t.grad)[0], e, ~m) < GRAD_TOL
     else:
-        assert rel(host(vel.grad)[0], eg_v) < GRAD_TOL
+        assert AM.rel_on(host(vel.grad)[0], eg_v, vel_keep) < GRAD_TOL
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -118,9 +142,9 @@ def test_advect_across_bricks(ngt_ops, shape, order, clamp, bound, staggered):
     assert rel(host(out)[0, 0], exp) < FWD_TOL
     g = f32(rng.standard_normal(exp.shape))
     out.backward(dev(g[None, None]))
-    eg_rho, eg_v = A.warp_backward(rho, vc, g, 1.0, order, clamp, bound)
-    assert rel(host(rt.grad)[0], eg_rho) < GRAD_TOL
-    _check_vel_grad(vel, faces, eg_v)
+    vel_np = tuple(dev(f[None]) for f in faces) if staggered else dev(vc[None])
+    gpu = _gpu_parts(ngt_ops, dev(rho[None]), vel_np, order, clamp, bound, out)
+    _check_grads(rt, vel, faces, rho, vc, g, order, clamp, bound, gpu)
 
 
 @pytest.mark.parametrize("C,separate,vmax", [(1, True, 1.9), (2, True, 3.5), (2, False, 1.0)])
@@ -145,19 +169,30 @@ def test_advect_across_bricks_batches(ngt_ops, C, separate, vmax):
     pairs = [(n, v, v) for n in range(N) for v in range(V)] if separate else [(n, n, 0) for n in range(N)]
     g = f32(rng.standard_normal(out.shape))
     out.backward(dev(g))
+    args = dict(timestep=-1.0, boundary="BORDER", separate_velocity_batch=separate)
+    with torch.no_grad():
+        vel_np = tuple(dev(c) for c in faces)
+        sl = host(ngt_ops.advect_grid_semi_langrange(dev(rho), vel_np, order=1, clamp_extrema="NONE", **args))
+        corr = host(ngt_ops.advect_grid_semi_langrange(dev(rho), vel_np, order=2, clamp_extrema="NONE", **args))
     eg_rho = np.zeros_like(rho)
     eg_v = [np.zeros_like(c) for c in faces]
+    rho_keep = np.ones(rho.shape[:4], dtype=bool)
+    face_keep = [np.ones(c.shape, dtype=bool) for c in faces]
     for n, v, slot in pairs:
         vc = A.centered_from_staggered(faces[0][v], faces[1][v], faces[2][v])
         exp = A.warp_forward(rho[n], vc, 1.0, 2, "MC_SMOOTH", BORDER)
         assert rel(host(out)[n, slot], exp) < FWD_TOL
-        gr, gv = A.warp_backward(rho[n], vc, g[n, slot], 1.0, 2, "MC_SMOOTH", BORDER)
+        gpu = {"sl": sl[n, slot], "corr": corr[n, slot], "out": host(out)[n, slot]}
+        gr, gv, rk, vk, info = AM.oracle_adjoint(rho[n], vc, g[n, slot], 2, "MC_SMOOTH", BORDER, gpu)
+        print("pair (%d, %d) left out: %s" % (n, v, info), flush=True)
         eg_rho[n] += gr
-        for k, e in enumerate(A.centered_from_staggered_adjoint(gv)):
+        rho_keep[n] &= rk
+        for k, (e, m) in enumerate(zip(A.centered_from_staggered_adjoint(gv), AM.faces_of_cells(~vk))):
             eg_v[k][v] += e
-    assert rel(host(rt.grad), eg_rho) < GRAD_TOL
-    for t, e in zip(vel, eg_v):
-        assert rel(host(t.grad), e) < GRAD_TOL
+            face_keep[k][v] &= ~m
+    assert AM.rel_on(host(rt.grad), eg_rho, rho_keep[..., None]) < GRAD_TOL
+    for t, e, m in zip(vel, eg_v, face_keep):
+        assert AM.rel_on(host(t.grad), e, m) < GRAD_TOL
 
 
 def _gpu_revert_mask(ngt_ops, rho, vel, bound):
@@ -218,9 +253,8 @@ def test_advect_64_workload(ngt_ops):
         # the grid, or reads only zero cells) are zeros on the GPU too
         assert np.all(host(out)[0, 0][exp == 0] == 0)
         out.backward(dev(g[None, None]))
-        eg_rho, eg_v = A.warp_backward(rho, vc, g, 1.0, order, clamp, BORDER)
-        assert rel(host(rt.grad)[0], eg_rho) < GRAD_TOL
-        assert rel(host(vt.grad)[0], eg_v) < GRAD_TOL
+        gpu = _gpu_parts(ngt_ops, dev(rho[None]), dev(vc[None]), order, clamp, BORDER, out)
+        _check_grads(rt, vt, None, rho, vc, g, order, clamp, BORDER, gpu, "64^3 order %d" % order)
     _check_revert_mask(ngt_ops, rho, dev(vc[None]), vc, BORDER)
 
 

@@ -85,7 +85,7 @@ def test_step128_against_oracle(prob):
     loss.backward()
     torch.cuda.synchronize()
     # forward
-    lv = float(loss)
+    lv = float(loss.detach())
     print("loss %.9e (oracle %.9e)" % (lv, float(gold["loss"])), flush=True)
     assert abs(lv - float(gold["loss"])) <= FWD_TOL * float(gold["loss"])
     a = adv.detach()[0].cpu().double().numpy()
@@ -104,6 +104,10 @@ def test_step128_against_oracle(prob):
     print("dL/drho at 64 voxels: |gpu - adjoint| %.2e, |gpu - central difference| %.2e of max |g|" % (e_adj, e_fd),
           flush=True)
     assert e_adj <= GRAD_TOL and e_fd <= GRAD_TOL
+    # the norms and projections leave out what fp32 and fp64 may resolve
+    # differently (non-differentiable points; listed in the fixture)
+    g[tuple(gold["rho_out"].T)] = 0.0
+    print("dL/drho: %d voxels left out of the checksums" % len(gold["rho_out"]), flush=True)
     check_field(g, gold, "rho_gnorm", "rho_proj", 11, GRAD_TOL, "dL/drho")
     # dL/dv: 64 faces over the three MAC components
     gf = [f.grad[0].cpu().double().numpy() for f in faces]
@@ -116,7 +120,11 @@ def test_step128_against_oracle(prob):
     print("dL/dv at 64 faces: |gpu - adjoint| %.2e, |gpu - central difference| %.2e of max |g|" % (e_adj, e_fd),
           flush=True)
     assert e_adj <= GRAD_TOL and e_fd <= GRAD_TOL
+    fo = gold["face_out"]
+    print("dL/dv: %d faces left out of the checksums" % len(fo), flush=True)
     for k in range(3):
+        sel = fo[fo[:, 0] == k][:, 1:]
+        gf[k][tuple(sel.T) + (0,)] = 0.0
         norm, proj = checks(gf[k], 12 + k)
         print("dL/dv%s: norm %.9e (oracle %.9e)" % ("xyz"[k], norm, gold["face_gnorm"][k]), flush=True)
         assert abs(norm - gold["face_gnorm"][k]) <= GRAD_TOL * gold["face_gnorm"][k]
