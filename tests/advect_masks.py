@@ -13,13 +13,15 @@ reference's kernels) and fp64 (the oracle) may land on different sides:
   value are continuous there).  At 128^3 about 400 of 6M components.
 * A branch choice within rounding of its switch: MC_SMOOTH (the corrected
   value within a relative 1e-5 of the corners' min or max) and NEGATIVE (the
-  value within rounding of 0).  Where the device's choice is visible in its
-  outputs (the clamped result equals exactly one of the SL and the
-  unclamped MacCormack results, or the sign of the unclamped value), the
-  oracle takes the device's choice; either choice is a valid one-sided
-  derivative, and the device's backward must follow its own forward.  Where
-  it is not visible (SL and corrected values equal in fp32), the cells whose
-  gradient depends on that choice are left out of the comparison.
+  value within rounding of 0).  There the oracle takes the device's choice,
+  read off its outputs: its unclamped MacCormack value against the min / max
+  of the corners (exact in either precision), or the sign of its unclamped
+  value.  Either choice is a valid one-sided derivative, and the device's
+  backward must follow its own forward.  Without the device's results (the
+  full-size fixtures), the cells whose gradient depends on a tie's choice are
+  left out instead.  Exact ties on plateaus (all values involved equal, as in
+  clipped densities) are evaluated exactly in both precisions and are not
+  ties of rounding.
 """
 import numpy as np
 
@@ -61,9 +63,12 @@ def branch_ties(rho, vc, parts, order, clamp, boundary, dt=1.0):
     """bool [D,H,W,C]: branch choices within rounding of their switch."""
     clamp = clamp.upper()
     if clamp == "MC_SMOOTH":
-        corr, mn, mx = parts["corr"], parts["mn"], parts["mx"]
-        scale = np.maximum.reduce([np.abs(parts["fwd"]), np.abs(rho), np.abs(parts["back"]), np.abs(mn), np.abs(mx)])
-        return np.minimum(np.abs(corr - mn), np.abs(corr - mx)) <= REL_TIE * scale
+        corr, mn, mx, fwd, back = parts["corr"], parts["mn"], parts["mx"], parts["fwd"], parts["back"]
+        scale = np.maximum.reduce([np.abs(fwd), np.abs(rho), np.abs(back), np.abs(mn), np.abs(mx)])
+        # on a plateau (every value involved equal: clipped densities) fp32
+        # evaluates the tie exactly as fp64 does, so it is no tie of rounding
+        flat = (fwd == back) & (back == rho) & (mn == mx) & (mx == fwd)
+        return (np.minimum(np.abs(corr - mn), np.abs(corr - mx)) <= REL_TIE * scale) & ~flat
     if clamp == "NEGATIVE":
         i = A._idx(rho.shape[:3])
         p = i - dt * vc
@@ -91,10 +96,18 @@ def oracle_adjoint(rho, vc, g, order, clamp, boundary, gpu=None, dt=1.0):
     decisions = {}
     undecided = np.zeros(rho.shape, dtype=bool)
     if clamp == "MC_SMOOTH":
-        decided = gpu["sl"] != gpu["corr"] if gpu is not None else np.zeros(ties.shape, dtype=bool)
-        reverted = (gpu["out"] == gpu["sl"]) & decided if gpu is not None else decided
-        decisions["oob"] = np.where(ties & decided, reverted, parts["oob"])
-        undecided = ties & ~decided
+        if gpu is not None:
+            # the device's choice: its unclamped value against the corners'
+            # min / max (exact in either precision); where the clamped result
+            # shows it (SL and corrected values differ), it must agree
+            dev_oob = (gpu["corr"] < parts["mn"]) | (gpu["corr"] > parts["mx"])
+            shown = gpu["sl"] != gpu["corr"]
+            assert np.array_equal((gpu["out"] == gpu["sl"])[shown], dev_oob[shown]), \
+                "the clamped result does not follow the device's own revert decision"
+            decisions["oob"] = np.where(ties, dev_oob, parts["oob"])
+        else:
+            decisions["oob"] = parts["oob"]
+            undecided = ties
     elif clamp == "NEGATIVE":
         if gpu is not None:
             value = gpu["corr"] if order == 2 else gpu["sl"]
