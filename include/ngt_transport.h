@@ -267,6 +267,22 @@ int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape
                               int boundary, int blending_mode, const float* target, float loss_weight,
                               const float* bricked, void* record, size_t record_bytes, float* output, float* loss,
                               void* workspace, size_t workspace_bytes, ngt_stream_t stream);
+/* Allocation-free variant of ngt_render_density_l2_fwd.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Same arguments and results without the workspace: the scratch (the haloed
+ * brick copy of the density when `bricked` is null, the loss partials) is a
+ * static device array of ngt_static_scratch_bytes() bytes per GPU, held
+ * until the process ends; calls needing more are rejected with
+ * NGT_ERR_WORKSPACE (a 1-channel grid up to about 176^3).  All calls on one
+ * GPU share that array: calls that overlap on one GPU race. */
+size_t ngt_static_scratch_bytes(void);
+int ngt_render_density_l2_fwd_noalloc(const float* density, const int64_t* density_shape, const float* light_rgb,
+                                      float depth_step, const float* matrix_m, const float* matrix_v,
+                                      const float* matrix_p, const float* frustum, int num_cameras,
+                                      const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
+                                      const float* target, float loss_weight, const float* bricked, void* record,
+                                      size_t record_bytes, float* output, float* loss, ngt_stream_t stream);
 size_t ngt_render_density_l2_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
                                                 const float* matrix_v, const float* matrix_p, const float* frustum,
                                                 int num_cameras, const int32_t* output_dhw, int interpolation,

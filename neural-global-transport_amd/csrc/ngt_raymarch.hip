@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace ngt {
 namespace {
@@ -1521,6 +1523,46 @@ int render_density_fwd_impl(const float* density, const int64_t* density_shape, 
 }  // namespace
 }  // namespace ngt
 
+// ---- allocation-free fused render + L2 forward ----------------------------
+// The scratch of ngt_render_density_l2_fwd (the haloed brick copy of the
+// density and the per-work-group loss partials) lives in a file-scope device
+// array instead of a caller workspace, for callers that cannot allocate
+// device memory.  Each GPU the library runs on holds its own copy of the
+// array until the process ends; host code addresses it only through the
+// pointer hipGetSymbolAddress returns for the current device (the array's
+// name is a host-side stand-in).  Every call on a GPU shares that GPU's
+// array: calls that overlap on one GPU race.  Nothing in it needs clearing:
+// the brick copy and every partial are written before they are read.
+constexpr size_t kStaticScratchBytes = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace ngt {
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch` (looked up once per device).
+int scratch_for_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  NGT_REQUIRE(e == hipSuccess, NGT_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  NGT_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, NGT_ERR_HIP, "device index out of range for the static scratch");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    NGT_REQUIRE(e == hipSuccess && p != nullptr, NGT_ERR_HIP,
+                std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NGT_OK;
+}
+}  // namespace
+}  // namespace ngt
+
+
 using namespace ngt;
 
 extern "C" {
@@ -1751,6 +1793,29 @@ int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape
   hipLaunchKernelGGL(l2_finish_kernel, dim3(1), dim3(256), 0, as_stream(stream), partial,
                      static_cast<int>(l2_partials(density_shape, num_cameras, output_dhw)), loss_weight, loss);
   return check_launch("ngt_render_density_l2_fwd");
+}
+
+size_t ngt_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+int ngt_render_density_l2_fwd_noalloc(const float* density, const int64_t* density_shape, const float* light_rgb,
+                                      float depth_step, const float* matrix_m, const float* matrix_v,
+                                      const float* matrix_p, const float* frustum, int num_cameras,
+                                      const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
+                                      const float* target, float loss_weight, const float* bricked, void* record,
+                                      size_t record_bytes, float* output, float* loss, ngt_stream_t stream) {
+  NGT_REQUIRE(output_dhw != nullptr && num_cameras > 0 && density_shape != nullptr, NGT_ERR_INVALID_ARGUMENT,
+              "Invalid output_shape");
+  const size_t need = ngt_render_density_l2_fwd_workspace_size(density_shape, interpolation, num_cameras, output_dhw);
+  NGT_REQUIRE(need <= kStaticScratchBytes, NGT_ERR_WORKSPACE,
+              "ngt_render_density_l2_fwd_noalloc: the call needs " + std::to_string(need) +
+                  " bytes of scratch, more than the static " + std::to_string(kStaticScratchBytes) + " bytes");
+  void* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc) return rc;
+  return ngt_render_density_l2_fwd(density, density_shape, light_rgb, depth_step, matrix_m, matrix_v, matrix_p,
+                                   frustum, num_cameras, output_dhw, interpolation, boundary, blending_mode, target,
+                                   loss_weight, bricked, record, record_bytes, output, loss, ws, kStaticScratchBytes,
+                                   stream);
 }
 
 size_t ngt_render_density_l2_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
