@@ -416,7 +416,8 @@ struct alignas(16) RayRec {
   float sx, sy, sz, dx, dy, dz;
   float g_scale, g_off;
   int steps;    // samples recorded (at most the plan's kcap)
-  int pad[3];   // pad[0]: the ray's own sample count; steps < pad[0] would mean a cut ray
+  int pad[3];   // pad[0]: the ray's own sample count (steps < pad[0] would mean a cut ray);
+                // pad[1]: float bits of a bound on |g_scale * (G - g_off)| over the ray's samples
 };
 
 // Image gradient of the fused L2 loss at one pixel (img: the pixel's C
@@ -464,12 +465,18 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
     float* gp;
     int64_t pixel;
     int kcap;
+    float gmax;
     NGT_DEVICE void one(int k, float, float, float, const Vec<C>& gc) {
-      vstore<CG>(gp + g_offset(k, pixel, kcap) * CG, fold_cell_grad<C, LIGHT>(a, gc));
+      const Vec<CG> g = fold_cell_grad<C, LIGHT>(a, gc);
+      vstore<CG>(gp + g_offset(k, pixel, kcap) * CG, g);
+#pragma unroll
+      for (int i = 0; i < CG; ++i) gmax = fmaxf(gmax, fabsf(g.v[i])) + (g.v[i] - g.v[i]);  // NaN stays NaN
     }
-  } sink{a, G + static_cast<int64_t>(pair) * kcap * g_pixels(npix) * CG, pixel, kcap};
+  } sink{a, G + static_cast<int64_t>(pair) * kcap * g_pixels(npix) * CG, pixel, kcap, 0.f};
   const Vec<C> g_img = a.l2_target ? l2_image_grad<C>(a, out + pix, pix / C * 3) : vload<C>(out_grad + pix);
   walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT, BRICKED>(a, src, r, g_img, out + pix, sink);
+  // the gather's fixed-point unit comes from this bound
+  rays[pair * npix + pixel].pad[1] = __float_as_int(sink.gmax);
 }
 
 template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
@@ -512,6 +519,7 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   const int rec_steps = min(r.steps, kcap);
   Vec<4> acc = vzero<4>();
   float q = 0.f;  // Q_{k-1}
+  float hmax = 0.f;
   for (int k = 0; k < r.steps; k += kUnroll) {
     float qx[kUnroll], qy[kUnroll], qz[kUnroll];
     Vec<4> cells[kUnroll];
@@ -540,6 +548,7 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
         for (int i = 0; i < 4; ++i) acc.v[i] += cells[u].v[i];
       }
       if (k + u < rec_steps) gp[static_cast<int64_t>(k + u) * kGPix] = h;
+      hmax = fmaxf(hmax, fabsf(h)) + (h - h);  // NaN stays NaN
     }
   }
   const int64_t pix = (static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x;
@@ -553,6 +562,8 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   rec.pad[0] = r.steps;
   rec.g_scale = a.depth_step * 2.f * a.l2_scale * (a.light_r * d0 + a.light_g * d1 + a.light_b * d2);
   rec.g_off = MODE == BEER_LAMBERT ? q : 0.f;
+  // bound on |g_scale (h_k - g_off)|: the gather's fixed-point unit comes from it
+  rec.pad[1] = __float_as_int(fabsf(rec.g_scale) * (hmax + fabsf(rec.g_off)));
   rays[static_cast<int64_t>(pair) * npix + pixel] = rec;
 }
 
@@ -603,10 +614,21 @@ NGT_DEVICE int block_inclusive_scan(int v, int* s_wtot) {
 // projects the brick's sample-support box (positions whose trilinear corners
 // can fall in the brick: p in [b0 - 0.5, b1 + 1.5)) to a pixel rectangle,
 // walks the rays of those pixels over the k interval inside the box, and adds
-// each sample's stored gradient times its corner weights to the brick's voxels
-// in LDS.  The brick owns its voxels, so the result is written with plain
+// each sample's stored gradient times its corner weights to the brick's
+// voxels.  The brick owns its voxels, so the result is written with plain
 // stores; a sample on a brick boundary is visited by each brick it touches
 // and contributes only that brick's corners.
+//
+// The sums are kept in LDS as 64-bit fixed point and added with integer LDS
+// atomics (ds_add_u64, at the LDS's full rate; float LDS atomics run some
+// 30x slower here).  The brick's unit is 2^-S with S chosen from the largest
+// per-ray gradient bound among its rays (RayRec::pad[1], written by the walk
+// that recorded the gradients), so that every corner contribution rounds to
+// an integer below 2^30 in magnitude: the rounding error of a contribution is
+// at most 2^-30 of the brick's largest sample gradient, and integer sums are
+// exact and independent of the order of the atomics (the gradient is
+// reproducible bit for bit).  A non-finite bound makes the brick's gradient
+// NaN.
 template <int CG, int BX, int BY, int BZ>
 NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
                                                                  const RayRec* __restrict__ rays,
@@ -614,13 +636,13 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
                                                                  float* __restrict__ in_grad, int accumulate,
                                                                  const float* __restrict__ grad_scale) {
   constexpr int NV = BX * BY * BZ;
-  // each of the 256 work-items owns a run of OX consecutive x-voxels of one
-  // (y, z) row of the brick
-  constexpr int OX = NV / 256;
-  static_assert(NV % 256 == 0 && BX % OX == 0 && (BX / OX) * BY * BZ == 256, "brick / owner layout");
-  __shared__ float acc[NV * CG];
+  // accumulator: the brick grown by one cell on every side, so that the 8
+  // corners of every sample whose floor cell lies in b0-1 .. b1 land in it
+  constexpr int PX = BX + 2, PY = BY + 2, PZ = BZ + 2, NP = PX * PY * PZ;
+  __shared__ unsigned long long acc[CG][NP];
   __shared__ int rect[kMaxCamsPerLaunch][4];  // x0, y0, nx, ny (nx = 0: view skipped)
   __shared__ int ray_begin[kMaxCamsPerLaunch + 1];
+  __shared__ float s_wmax[4];
   const int tid = threadIdx.x;
   const int bricks_x = (a.din.x + BX - 1) / BX, bricks_y = (a.din.y + BY - 1) / BY;
   const int bricks_z = (a.din.z + BZ - 1) / BZ;
@@ -631,6 +653,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   const int b0y = ((brem / bricks_x) % bricks_y) * BY;
   const int b0z = (brem / (bricks_x * bricks_y)) * BZ;
   const int b1x = min(b0x + BX, a.din.x) - 1, b1y = min(b0y + BY, a.din.y) - 1, b1z = min(b0z + BZ, a.din.z) - 1;
+  for (int i = tid; i < CG * NP; i += 256) (&acc[0][0])[i] = 0ull;
 
   // pixel rectangles of the views: wavefront 0 projects the 8 corners of the
   // support box for up to 8 views at once (lane = view * 8 + corner), then
@@ -693,44 +716,61 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   __syncthreads();
 
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
+  const int total = ray_begin[cb.count];
+  // the brick's fixed-point unit from the largest gradient bound of its rays
+  auto ray_index = [&](int r, int& px, int& py) {
+    int ci = 0;
+    while (ci + 1 < cb.count && r >= ray_begin[ci + 1]) ++ci;
+    const int local = r - ray_begin[ci];
+    px = rect[ci][0] + local % rect[ci][2];
+    py = rect[ci][1] + local / rect[ci][2];
+    return ci;
+  };
+  {
+    float m = 0.f;
+    for (int r = tid; r < total; r += 256) {
+      int px, py;
+      const int ci = ray_index(r, px, py);
+      const float bnd = __int_as_float(rays[(pair_base + ci) * static_cast<int>(npix) + py * a.out_w + px].pad[1]);
+      m = (bnd > m || bnd != bnd) ? bnd : m;  // NaN propagates
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float t = __shfl_xor(m, o, 64);
+      m = (t > m || t != t) ? t : m;
+    }
+    if ((tid & 63) == 0) s_wmax[tid >> 6] = m;
+  }
+  __syncthreads();
+  float wmax = s_wmax[0];
+#pragma unroll
+  for (int w = 1; w < 4; ++w) wmax = (s_wmax[w] > wmax || s_wmax[w] != s_wmax[w]) ? s_wmax[w] : wmax;
+  const bool finite = wmax < 3.0e38f;  // false for +inf and NaN
+  // |contribution| <= wmax < 2^e  ->  |contribution * 2^S| < 2^30 with S = 30 - e
+  const int S = wmax > 0.f && finite ? 30 - ilogbf(wmax) - 1 : 0;
+
   const float fdx = static_cast<float>(a.din.x), fdy = static_cast<float>(a.din.y), fdz = static_cast<float>(a.din.z);
-  const float fb0x = static_cast<float>(b0x), fb0y = static_cast<float>(b0y), fb0z = static_cast<float>(b0z);
-  const float fb1x = static_cast<float>(b1x), fb1y = static_cast<float>(b1y), fb1z = static_cast<float>(b1z);
-  const float lo3[3] = {static_cast<float>(b0x) - 0.5f, static_cast<float>(b0y) - 0.5f, static_cast<float>(b0z) - 0.5f};
-  const float hi3[3] = {static_cast<float>(b1x) + 1.5f, static_cast<float>(b1y) + 1.5f, static_cast<float>(b1z) + 1.5f};
+  // the support box widened by 1e-3 cells: far above the rounding of the
+  // slab test, so no sample in the box is lost; the per-sample test below
+  // decides exactly
+  const float lo3[3] = {static_cast<float>(b0x) - 0.501f, static_cast<float>(b0y) - 0.501f,
+                        static_cast<float>(b0z) - 0.501f};
+  const float hi3[3] = {static_cast<float>(b1x) + 1.501f, static_cast<float>(b1y) + 1.501f,
+                        static_cast<float>(b1z) + 1.501f};
   // Rays are taken 256 at a time: each work-item clips one ray to the support
-  // box (slab test with a one-sample margin) and counts its samples there;
+  // box (slab test) and counts its samples there;
   // after a block-wide scan the chunk's samples are dealt out one per
-  // work-item in sub-chunks of kItems.  A sub-chunk is counting-sorted in LDS
-  // by the sample's floor cell (bins = floor cells b0-1 .. b1 per axis; one
-  // integer LDS atomic per sample gives its rank in the bin), and then every
-  // work-item sums, for each voxel it owns, the samples of the 8 bins whose
-  // trilinear footprint holds
-  // that voxel, in registers: no floating-point atomics.  (A corner at
-  // floor+1 where the reference uses ceil = floor at integer positions carries
-  // weight exactly 0.)
-  constexpr int NB = (BX + 1) * (BY + 1) * (BZ + 1);
+  // work-item in sub-chunks of kItems, and each sample adds its 8 corner
+  // contributions to the accumulator.  (A corner at floor+1 where the
+  // reference uses ceil = floor at integer positions carries weight exactly 0.)
   constexpr int kItems = 1024;
   __shared__ int s_off[257];
-  __shared__ int s_k0[256];
-  __shared__ int s_rec[256];
+  __shared__ int s_kb[256];        // each ray's first k in the box minus its first item
   __shared__ int64_t s_gofs[256];  // G offset of each ray's sample 0
   __shared__ float4 s_ps[256];     // each ray's start and step x
-  __shared__ float2 s_pd[256];     // each ray's step y, z
-  // per bin: the sample count, then (scanned in place) the first slot
-  __shared__ int bin_start[NB + 1];
-  int* const bin_cnt = bin_start;
-  __shared__ float4 srec[kItems];
-  __shared__ float sg[CG > 1 ? kItems * CG : 1];
+  __shared__ float4 s_pd[256];     // each ray's step y, z and loss-gradient scale, offset
   __shared__ int s_wtot[4];
   __shared__ unsigned char s_ray[kItems];
-  // owned voxels: the x-pair of work-item tid (NV == 512)
-  float accv[OX][CG];
-#pragma unroll
-  for (int o = 0; o < OX; ++o)
-#pragma unroll
-    for (int ch = 0; ch < CG; ++ch) accv[o][ch] = 0.f;
-  const int total = ray_begin[cb.count];
   const int fb0m1x = b0x - 1, fb0m1y = b0y - 1, fb0m1z = b0z - 1;
   for (int base_r = 0; base_r < total; base_r += 256) {
     const int r = base_r + tid;
@@ -747,7 +787,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
              g_offset(0, static_cast<int64_t>(py) * a.out_w + px, kcap);
       const RayRec rr = rays[rec_index];
       s_ps[tid] = make_float4(rr.sx, rr.sy, rr.sz, rr.dx);
-      s_pd[tid] = make_float2(rr.dy, rr.dz);
+      s_pd[tid] = make_float4(rr.dy, rr.dz, rr.g_scale, rr.g_off);
       if (rr.steps > 0) {
         const float sv[3] = {rr.sx, rr.sy, rr.sz};
         const float dv[3] = {rr.dx, rr.dy, rr.dz};
@@ -764,48 +804,40 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
             miss = true;
           }
         }
-        if (!miss && lo <= hi + 2.f) {
+        if (!miss && lo <= hi) {
           const float fsteps = static_cast<float>(rr.steps);
-          k0 = max(0, static_cast<int>(floorf(fminf(fmaxf(lo, -1.f), fsteps))) - 1);
-          const int k1 = min(rr.steps - 1, static_cast<int>(floorf(fminf(fmaxf(hi, -2.f), fsteps))) + 1);
+          k0 = max(0, static_cast<int>(ceilf(fminf(fmaxf(lo, -1.f), fsteps))));
+          const int k1 = min(rr.steps - 1, static_cast<int>(floorf(fminf(fmaxf(hi, -2.f), fsteps))));
           cnt = max(0, k1 - k0 + 1);
         }
       }
     }
     // inclusive block scan of the counts
-    s_off[tid + 1] = block_inclusive_scan(cnt, s_wtot);
+    const int incl = block_inclusive_scan(cnt, s_wtot);
+    s_off[tid + 1] = incl;
     if (tid == 0) s_off[0] = 0;
-    s_k0[tid] = k0;
-    s_rec[tid] = rec_index;
+    s_kb[tid] = k0 - (incl - cnt);
     s_gofs[tid] = gofs;
     __syncthreads();
     const int items = s_off[256];
     for (int i0 = 0; i0 < items; i0 += kItems) {
       const int i_end = min(i0 + kItems, items);
-      for (int b = tid; b < NB; b += 256) bin_cnt[b] = 0;
       {  // the ray of every sample of this sub-chunk: each ray marks its own range
         const int r0 = max(s_off[tid], i0), r1 = min(s_off[tid + 1], i_end);
         for (int i = r0; i < r1; ++i) s_ray[i - i0] = static_cast<unsigned char>(tid);
       }
       __syncthreads();
-      // sample i -> (bin, weights, gradient); bin < 0 when it does not touch the brick
-      auto locate = [&](int i, float& wx, float& wy, float& wz, int64_t& gofs, int& k, float& gs, float& go) -> int {
+      for (int i = i0 + tid; i < i_end; i += 256) {
         const int lo_j = s_ray[i - i0];
-        k = s_k0[lo_j] + (i - s_off[lo_j]);
-        const int rec = s_rec[lo_j];
-        gofs = s_gofs[lo_j];
-        // the loss-gradient factors are first used at placement: their load
-        // completes during the bin scan
-        gs = rays[rec].g_scale;
-        go = rays[rec].g_off;
+        const int k = s_kb[lo_j] + i;
         const float4 ps = s_ps[lo_j];
-        const float2 pd = s_pd[lo_j];
+        const float4 pd = s_pd[lo_j];
         const float kf = static_cast<float>(k);
         const float qx = fmaf(kf, ps.w, ps.x) - 0.5f;
         const float qy = fmaf(kf, pd.x, ps.y) - 0.5f;
         const float qz = fmaf(kf, pd.y, ps.z) - 0.5f;
         const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
-        wx = qx - fx; wy = qy - fy; wz = qz - fz;
+        const float wx = qx - fx, wy = qy - fy, wz = qz - fz;
         const int lx = static_cast<int>(fmaxf(fminf(fx, 1e8f), -1e8f)) - fb0m1x;
         const int ly = static_cast<int>(fmaxf(fminf(fy, 1e8f), -1e8f)) - fb0m1y;
         const int lz = static_cast<int>(fmaxf(fminf(fz, 1e8f), -1e8f)) - fb0m1z;
@@ -813,113 +845,31 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
         const bool ok = (qx + 0.5f >= 0.f) & (qx + 0.5f < fdx) & (qy + 0.5f >= 0.f) & (qy + 0.5f < fdy) &
                         (qz + 0.5f >= 0.f) & (qz + 0.5f < fdz) & (lx >= 0) & (lx <= b1x - fb0m1x) & (ly >= 0) &
                         (ly <= b1y - fb0m1y) & (lz >= 0) & (lz <= b1z - fb0m1z);
-        return ok ? (lz * (BY + 1) + ly) * (BX + 1) + lx : -1;
-      };
-      // each work-item locates its (at most kPer) samples once: bin, rank in
-      // the bin (from the counting atomic), weights and gradient stay in
-      // registers until they are placed
-      constexpr int kPer = kItems / 256;
-      int ibin[kPer], irank[kPer];
-      float iwx[kPer], iwy[kPer], iwz[kPer], igs[kPer], igo[kPer];
-      Vec<CG> ig[kPer];  // raw G: scaled at placement, so the loads complete during the bin scan
+        if (!ok) continue;
+        const float gs = pd.z, go = pd.w;
+        const Vec<CG> gv = vload<CG>(G + (s_gofs[lo_j] + static_cast<int64_t>(k) * kGPix) * CG);
+        // corner weights: z, then y, then the x pair
+        const float wz0 = 1.f - wz, wy0 = 1.f - wy, wx0 = 1.f - wx;
+        const float w00 = wz0 * wy0, w01 = wz0 * wy, w10 = wz * wy0, w11 = wz * wy;
+        const float wyz[4] = {w00, w01, w10, w11};
+        unsigned long long* const p0 = &acc[0][(lz * PY + ly) * PX + lx];
 #pragma unroll
-      for (int j = 0; j < kPer; ++j) {
-        const int i = i0 + tid + 256 * j;
-        ibin[j] = -1;
-        irank[j] = 0;
-        if (i < i_end) {
-          int64_t gofs;
-          int k;
-          float gs, go;
-          const int bin = locate(i, iwx[j], iwy[j], iwz[j], gofs, k, gs, go);
-          if (bin >= 0) {
-            ig[j] = vload<CG>(G + (gofs + static_cast<int64_t>(k) * kGPix) * CG);
-            igs[j] = gs;
-            igo[j] = go;
-            irank[j] = atomicAdd(&bin_cnt[bin], 1);
-            ibin[j] = bin;
+        for (int ch = 0; ch < CG; ++ch) {
+          const float g = ldexpf(gs * (gv.v[ch] - go), S);
+#pragma unroll
+          for (int yz = 0; yz < 4; ++yz) {
+            const float t = g * wyz[yz];
+            const int off = ch * NP + (yz >> 1) * PX * PY + (yz & 1) * PX;
+            // truncation: at most one unit (2^-S) off, far below the sum's rounding
+            const long long c0 = static_cast<long long>(static_cast<int>(t * wx0));
+            const long long c1 = static_cast<long long>(static_cast<int>(t * wx));
+            atomicAdd(p0 + off, static_cast<unsigned long long>(c0));
+            atomicAdd(p0 + off + 1, static_cast<unsigned long long>(c1));
           }
         }
       }
-      __syncthreads();
-      // exclusive scan of the bins: 3 per work-item, then across work-items
-      {
-        constexpr int kPer = (NB + 255) / 256;
-        int local[kPer];
-        int sum = 0;
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-          const int b = tid * kPer + j;
-          local[j] = b < NB ? bin_cnt[b] : 0;
-          sum += local[j];
-        }
-        const int incl = block_inclusive_scan(sum, s_wtot);
-        int run = incl - sum;
-#pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-          const int b = tid * kPer + j;
-          if (b < NB) bin_start[b] = run;
-          run += local[j];
-        }
-        if (tid == 255) bin_start[NB] = incl;
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < kPer; ++j) {
-        if (ibin[j] < 0) continue;
-#pragma unroll
-        for (int ch = 0; ch < CG; ++ch) ig[j].v[ch] = igs[j] * (ig[j].v[ch] - igo[j]);
-        const int pos = bin_start[ibin[j]] + irank[j];
-        srec[pos] = make_float4(iwx[j], iwy[j], iwz[j], ig[j].v[0]);
-        if constexpr (CG > 1) {
-#pragma unroll
-          for (int ch = 0; ch < CG; ++ch) sg[pos * CG + ch] = ig[j].v[ch];
-        }
-      }
-      __syncthreads();
-      // owners: work-item tid holds the x-run ox .. ox + OX - 1 of row (oy,
-      // oz); voxel v is corner d of the samples whose floor cell is v - d, so
-      // per (dy, dz) the run reads the OX + 1 consecutive bins ox .. ox + OX
-      // (bin x index = floor x - b0x + 1): bin ox + b feeds voxel ox + b - 1
-      // as its floor corner (b >= 1) and voxel ox + b as its ceil corner
-      // (b < OX)
-      {
-        const int ox = OX * (tid % (BX / OX)), oy = (tid / (BX / OX)) % BY, oz = tid / (BX / OX * BY);
-#pragma unroll
-        for (int dyz = 0; dyz < 4; ++dyz) {
-          const int dy = dyz & 1, dz = dyz >> 1;
-          const int row = ((oz + 1 - dz) * (BY + 1) + (oy + 1 - dy)) * (BX + 1) + ox;
-          int sb[OX + 2];
-#pragma unroll
-          for (int b = 0; b < OX + 2; ++b) sb[b] = bin_start[row + b];
-          // sample gradient times its y / z weight for this (dy, dz)
-          auto tyz = [&](int q, const float4& rq, int ch) {
-            const float g = CG == 1 ? rq.w : sg[q * CG + ch];
-            return g * ((dy ? rq.y : 1.f - rq.y) * (dz ? rq.z : 1.f - rq.z));
-          };
-#pragma unroll
-          for (int b = 0; b <= OX; ++b) {
-            for (int q = sb[b]; q < sb[b + 1]; ++q) {
-              const float4 rq = srec[q];
-#pragma unroll
-              for (int ch = 0; ch < CG; ++ch) {
-                const float t = tyz(q, rq, ch);
-                if (b >= 1) accv[b - 1][ch] += t * (1.f - rq.x);
-                if (b < OX) accv[b][ch] += t * rq.x;
-              }
-            }
-          }
-        }
-      }
-      __syncthreads();  // bins and records are rewritten by the next sub-chunk
+      __syncthreads();  // s_ray is rewritten by the next sub-chunk
     }
-  }
-  {
-    const int ox = OX * (tid % (BX / OX)), oy = (tid / (BX / OX)) % BY, oz = tid / (BX / OX * BY);
-#pragma unroll
-    for (int o = 0; o < OX; ++o)
-#pragma unroll
-      for (int ch = 0; ch < CG; ++ch) acc[((oz * BY + oy) * BX + ox + o) * CG + ch] = accv[o][ch];
   }
   __syncthreads();
   const int64_t base = static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z;
@@ -929,9 +879,11 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
     const int x = b0x + lx, y = b0y + ly, z = b0z + lz;
     if (x > b1x || y > b1y || z > b1z) continue;
     float* d = in_grad + (base + cell_offset(a.din, x, y, z)) * CG;
+    const int pi = ((lz + 1) * PY + ly + 1) * PX + lx + 1;
 #pragma unroll
     for (int c = 0; c < CG; ++c) {
-      const float v = acc[i * CG + c] * scale;
+      const long long q = static_cast<long long>(acc[c][pi]);
+      const float v = finite ? ldexpf(static_cast<float>(q), -S) * scale : __int_as_float(0x7fc00000);
       d[c] = accumulate ? d[c] + v : v;
     }
   }
@@ -1317,13 +1269,24 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
           hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false, true>), rgrid, dim3(256), 0, s, a, cb,
                              base, output, output_grad, gp.G, gp.rays, gp.kcap)));
     }
+    // bricks: larger ones visit fewer samples twice (the support box is one
+    // cell wider than the brick), but the 64-bit accumulators of 2 or 4
+    // channels must fit the LDS
     constexpr int BX = 16, BY = 8, BZ = 8;
-    const unsigned bricks = blocks_for(a.din.x, BX) * blocks_for(a.din.y, BY) * blocks_for(a.din.z, BZ);
+    constexpr int BX1 = 16, BY1 = 16, BZ1 = 8;
     const int accumulate = b > 0 ? 1 : 0;
     const int CG = LIGHT ? 1 : C;
-    NGT_SWITCH_C(CG,
-        hipLaunchKernelGGL((raymarch_bwd_brick_kernel<CC, BX, BY, BZ>), dim3(bricks * n_data), dim3(256), 0, s, a, cb,
-                           base, gp.rays, gp.G, gp.kcap, in_grad, accumulate, recorded ? grad_scale : nullptr));
+    if (CG == 1) {
+      const unsigned bricks = blocks_for(a.din.x, BX1) * blocks_for(a.din.y, BY1) * blocks_for(a.din.z, BZ1);
+      hipLaunchKernelGGL((raymarch_bwd_brick_kernel<1, BX1, BY1, BZ1>), dim3(bricks * n_data), dim3(256), 0, s, a,
+                         cb, base, gp.rays, gp.G, gp.kcap, in_grad, accumulate, recorded ? grad_scale : nullptr);
+    } else {
+      const unsigned bricks = blocks_for(a.din.x, BX) * blocks_for(a.din.y, BY) * blocks_for(a.din.z, BZ);
+      NGT_SWITCH_C(CG,
+          hipLaunchKernelGGL((raymarch_bwd_brick_kernel<CC, BX, BY, BZ>), dim3(bricks * n_data), dim3(256), 0, s, a,
+                             cb, base, gp.rays, gp.G, gp.kcap, in_grad, accumulate,
+                             recorded ? grad_scale : nullptr));
+    }
   }
 }
 
