@@ -448,21 +448,16 @@ def batch_bench(steps, warmup):
     frame runs one MacCormack step (MC_SMOOTH, BORDER, MAC velocity) and the
     fused render with L2 loss, forward and backward.  Batched: one persistent
     launch per kernel over all 128 instances (eager and captured in a HIP
-    graph); per-instance: the same ops launched instance by instance."""
+    graph); per-instance: the library's per-instance ops (advection, then
+    the fused render + L2 op) launched instance by instance."""
     import torch
     from ngt import synth, batch as B, ops, _lib
     from ngt.renderer import Renderer
     _lib.load()
-    rng = np.random.default_rng(5)
-    edges = (48, 64, 96, 128, 160)
-    images = ((72, 128), (144, 256), (270, 480))
     dens, vels, cams, tgts = [], [], [], []
     cells = ray_samples = 0
-    for q in range(32):
-        e = int(rng.choice(edges))
-        nv = int(rng.integers(1, 9))
-        hw = images[int(rng.integers(0, 3))]
-        grid_t, cs = synth.ring_setup((e, e, e), hw, e, nv, )
+    for q, (e, nv, hw) in enumerate(synth.uneven_batch_layout(5, 32)):
+        grid_t, cs = synth.ring_setup((e, e, e), hw, e, nv)
         M, V, P, F = Renderer()._transform_params([grid_t], cs)
         vel = synth.curl_noise_staggered_torch((e, e, e), seed=1000 + q, vmax=2.0)
         for f in range(4):
@@ -505,13 +500,14 @@ def batch_bench(steps, warmup):
     ms_graph = timed(graph.replay, steps)
 
     def per_instance():
+        # the fastest per-instance composition: the fused render + L2 op
         for i in range(len(dens)):
             M, V, P, F, ds, out_dhw = cams[i]
             d = dens[i][None].requires_grad_()
             v = tuple(c[None].requires_grad_() for c in vels[i])
             adv = ops.advect_grid_semi_langrange(d, v, -1.0, 2, "MC_SMOOTH", "BORDER", False).squeeze(1)
-            img = ops.render_density_fused(adv, (1.0, 1.0, 1.0), ds, M, V, P, F, out_dhw)[0]
-            ((img[..., :3] - tgts[i]) ** 2).sum().backward()
+            loss, _ = ops.render_density_fused_l2(adv, tgts[i][None], (1.0, 1.0, 1.0), ds, M, V, P, F, out_dhw)
+            loss.backward()
 
     print("batch: batched %.2f ms eager, %.2f ms graph" % (ms_eager, ms_graph), file=sys.stderr, flush=True)
     per_instance()
@@ -526,7 +522,11 @@ def batch_bench(steps, warmup):
             "ms_per_step_per_instance": ms_plain, "speedup_vs_per_instance": ms_plain / best,
             "voxel_updates_per_s": cells / (best * 1e-3), "ray_samples_per_s": ray_samples / (best * 1e-3),
             "workspace_bytes": int(step.adv.workspace.numel() + step.ren.workspace.numel()),
-            "autotuned_blocks_per_cu": {"/".join(str(x) for x in (k[0], k[-1])): v for k, v in tuner.best.items()},
+            "render_record_chunks": step.ren.num_chunks, "render_record_bytes": step.ren.record_bytes,
+            "autotuned": {"/".join(str(x) for x in (k[0], k[-1])): (
+                v if k[0] == "advect" or k[-1] == "fwd" else {"blocks_per_cu": v[0],
+                                                              "gather_brick": B.GATHER_BRICKS[v[1]]})
+                for k, v in tuner.best.items()},
             "autotune_ms": {"/".join(str(x) for x in (k[0], k[-1])): {str(c): round(t, 3) for c, t in v.items()}
                             for k, v in tuner.timings.items()}}
 

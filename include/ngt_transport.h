@@ -366,22 +366,41 @@ typedef struct {
   int32_t image_dhw[3];     /* samples per ray D', image H', W' (all views of the instance) */
   float light_rgb[3];
   float depth_step;
-  float* images;            /* fwd: [V,H',W',4] (RGB, optical depth) */
+  float* images;            /* fwd / l2: [V,H',W',4] (RGB, optical depth) */
   const float* images_grad; /* bwd */
-  float* density_grad;      /* bwd: [D,H,W,1] */
+  float* density_grad;      /* bwd / l2: [D,H,W,1] */
+  const float* l2_target;   /* l2: [V,H',W',3] target images */
+  float* loss;              /* l2: one float, sum over pixels and RGB of (image - target)^2 */
 } ngt_render_instance;
 
 /* No counterpart in the reference; batched fused density render
  * (Renderer.render_density, renderer.py:718-827), LINEAR + BORDER, exact
  * gradients by the brick-gather adjoint.  The plan holds the FP64-inverted
- * camera matrices. */
+ * camera matrices.  record_budget: bytes for the gather adjoint's per-sample
+ * records (0: 4 GiB); consecutive instances are grouped into chunks whose
+ * records fit it (an instance whose records alone exceed it is a chunk of its
+ * own), and the backward runs chunk by chunk over one record region, so the
+ * workspace is the haloed density copies, one float per 16x16 ray tile and
+ * the largest chunk's records.  ngt_render_batch_plan_info reports the chunk
+ * count and the record region's size.
+ * ngt_render_batch_fwd: images.  ngt_render_batch_bwd: density_grad for the
+ * given images_grad; reads the images and the haloed copies of the last
+ * ngt_render_batch_fwd on this workspace (the densities unchanged since).
+ * ngt_render_batch_l2: images, loss and density_grad = d loss / d density in
+ * one call (each chunk's recording walk is followed by its gather).
+ * gather_brick: the gather's voxel brick, 0 (8x8x8), 1 (16x8x8) or
+ * 2 (16x16x8); the results of different bricks agree to the fixed-point
+ * rounding (2^-30 of the brick's largest sample gradient per sample). */
 size_t ngt_render_batch_plan_bytes(int num_instances);
-int ngt_render_batch_plan(const ngt_render_instance* instances, int num_instances, int blending_mode, void* plan,
-                          size_t plan_bytes, size_t* workspace_bytes);
+int ngt_render_batch_plan(const ngt_render_instance* instances, int num_instances, int blending_mode,
+                          size_t record_budget, void* plan, size_t plan_bytes, size_t* workspace_bytes);
+int ngt_render_batch_plan_info(const void* plan_host, int* num_chunks, size_t* record_bytes);
 int ngt_render_batch_fwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
                          int blocks_per_cu, ngt_stream_t stream);
 int ngt_render_batch_bwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
-                         int blocks_per_cu, ngt_stream_t stream);
+                         int blocks_per_cu, int gather_brick, ngt_stream_t stream);
+int ngt_render_batch_l2(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                        int blocks_per_cu, int gather_brick, ngt_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -192,6 +192,6 @@ extern "C" {
 
 const char* ngt_last_error(void) { return ngt::g_last_error.c_str(); }
 
-int ngt_version(void) { return 6; }
+int ngt_version(void) { return 7; }
 
 }  // extern "C"

@@ -237,27 +237,37 @@ NGT_DEVICE void raymarch_fwd_pixel(const RayArgs& a, const CamBatch& cb, const V
   }
 }
 
+// Slot of a work-group's L2 loss partial in a launch whose grid is the tile
+// grid (the batched kernels pass their flat tile index instead).
+NGT_DEVICE int64_t grid_slot(const VBlock& vb) {
+  return (static_cast<int64_t>(vb.z) * gridDim.y + vb.y) * gridDim.x + vb.x;
+}
+
+// The work-group's partial of the fused L2 loss (a shuffle tree per
+// wavefront, then the 4 wavefronts in order) into a.l2_partial[slot].
+NGT_DEVICE void store_l2_partial(const RayArgs& a, float l2, int64_t slot) {
+  __shared__ float s_w[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) l2 += __shfl_down(l2, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = l2;
+  __syncthreads();
+  if (threadIdx.x == 0) a.l2_partial[slot] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
 template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
-NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, float* __restrict__ out) {
+NGT_DEVICE void raymarch_fwd_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, float* __restrict__ out,
+                                         int64_t slot) {
   int x, y;
   pixel_of_thread(vb, x, y);
   float l2 = 0.f;
   if (x < a.out_w && y < a.out_h) raymarch_fwd_pixel<C, FM, BM, MODE, LIGHT, BRICKED>(a, cb, vb, x, y, out, l2);
-  if (a.l2_partial) {  // uniform: the work-group's partial of the fused L2 loss
-    __shared__ float s_w[4];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) l2 += __shfl_down(l2, o, 64);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = l2;
-    __syncthreads();
-    if (threadIdx.x == 0)
-      a.l2_partial[(static_cast<int64_t>(vb.z) * gridDim.y + vb.y) * gridDim.x + vb.x] =
-          (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
-  }
+  if (a.l2_partial) store_l2_partial(a, l2, slot);  // uniform
 }
 
 template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
 __global__ __launch_bounds__(256) void raymarch_fwd_kernel(RayArgs a, CamBatch cb, float* __restrict__ out) {
-  raymarch_fwd_kernel_body<C, FM, BM, MODE, LIGHT, BRICKED>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, out);
+  const VBlock vb{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)};
+  raymarch_fwd_kernel_body<C, FM, BM, MODE, LIGHT, BRICKED>(vb, a, cb, out, grid_slot(vb));
 }
 
 // Per-sample cell gradients of one ray (exact adjoint of the forward walk),
@@ -568,27 +578,27 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void raymarch_fwd_record_kernel(RayArgs a, CamBatch cb, float* __restrict__ out,
-                                                                  int pair_base, float* __restrict__ G,
-                                                                  RayRec* __restrict__ rays, int kcap) {
-  const VBlock vb{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)};
+NGT_DEVICE void raymarch_fwd_record_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb,
+                                                float* __restrict__ out, int pair_base, float* __restrict__ G,
+                                                RayRec* __restrict__ rays, int kcap, int64_t slot) {
   if constexpr (MODE == BEER_LAMBERT || MODE == ADDITIVE) {
     int x, y;
     pixel_of_thread(vb, x, y);
     float l2 = 0.f;
     if (x < a.out_w && y < a.out_h) raymarch_fwd_record_pixel<MODE>(a, cb, vb, x, y, out, l2, pair_base, G, rays, kcap);
-    __shared__ float s_w[4];  // the work-group's loss partial, as raymarch_fwd_kernel_body
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) l2 += __shfl_down(l2, o, 64);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = l2;
-    __syncthreads();
-    if (threadIdx.x == 0)
-      a.l2_partial[(static_cast<int64_t>(vb.z) * gridDim.y + vb.y) * gridDim.x + vb.x] =
-          (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+    store_l2_partial(a, l2, slot);
   } else {  // ALPHA blends: the image first, then the adjoint walk of each ray
-    raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(vb, a, cb, out);
+    raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(vb, a, cb, out, slot);
     raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(vb, a, cb, pair_base, out, nullptr, G, rays, kcap);
   }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void raymarch_fwd_record_kernel(RayArgs a, CamBatch cb, float* __restrict__ out,
+                                                                  int pair_base, float* __restrict__ G,
+                                                                  RayRec* __restrict__ rays, int kcap) {
+  const VBlock vb{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)};
+  raymarch_fwd_record_kernel_body<MODE>(vb, a, cb, out, pair_base, G, rays, kcap, grid_slot(vb));
 }
 
 // Inclusive scan of one int per work-item over a 256-wide work-group: a
@@ -1319,12 +1329,23 @@ __global__ __launch_bounds__(256) void l2_image_grad_kernel(RayArgs a, const flo
 
 // ---------------------------------------------------------------------------
 // Uneven batches of fused density renders: every instance (one density grid,
-// 1-8 views of one image size) in one persistent launch per kernel.
+// 1-8 views of one image size) in one persistent launch per kernel.  The
+// per-sample records of the gather adjoint (RayRec per pixel, one gradient
+// factor per sample) are the only large buffer: instances are grouped into
+// consecutive chunks whose records fit a byte budget, and the recording and
+// gather kernels run chunk by chunk over one shared record region.
+
+// Gather brick shapes (voxels per work-group): 8x8x8, 16x8x8, 16x16x8.
+constexpr int kGatherShapes = 3;
+constexpr size_t kDefaultRecordBudget = size_t(4) << 30;
+
 struct RenBatchHeader {
-  int n, mode;
-  int tiles_ray, tiles_brickify, tiles_brick;
-  size_t off_items, off_ray, off_brickify, off_brick;
-  int64_t ws_bytes;
+  int n, mode, nchunks;
+  int tiles_ray, tiles_brickify;
+  int tiles_brick[kGatherShapes];
+  size_t off_items, off_ray, off_brickify, off_chunk;
+  size_t off_brick[kGatherShapes];
+  int64_t ws_partial, ws_record, record_bytes, ws_bytes;  // workspace byte offsets and sizes
 };
 
 struct RenBatchItem {
@@ -1332,11 +1353,14 @@ struct RenBatchItem {
   CamBatch cb;
   int ntx, nty;        // 16 x 16 pixel tiles per view
   int kcap;
-  int brick_tiles;     // brick rows of the bricked copy (one work-group each)
+  int brick_tiles;     // brick rows of the haloed copy (one work-group each)
   float* images;
   const float* images_grad;
   float* density_grad;
-  int64_t ws_bricked, ws_rays, ws_G;  // byte offsets into the workspace
+  const float* l2_target;
+  float* loss;
+  int64_t ws_bricked;        // byte offset of the haloed copy in the workspace
+  int64_t rec_rays, rec_G;   // byte offsets of the records in the record region
 };
 
 struct RenBatchView {
@@ -1344,7 +1368,6 @@ struct RenBatchView {
   const RenBatchItem* items;
   const int* ray;
   const int* brickify;
-  const int* brick;
 };
 
 NGT_DEVICE RenBatchView ren_batch_view(const void* plan) {
@@ -1354,7 +1377,6 @@ NGT_DEVICE RenBatchView ren_batch_view(const void* plan) {
   v.items = reinterpret_cast<const RenBatchItem*>(p + v.h->off_items);
   v.ray = reinterpret_cast<const int*>(p + v.h->off_ray);
   v.brickify = reinterpret_cast<const int*>(p + v.h->off_brickify);
-  v.brick = reinterpret_cast<const int*>(p + v.h->off_brick);
   return v;
 }
 
@@ -1362,6 +1384,19 @@ NGT_DEVICE VBlock ray_tile(const RenBatchItem& it, int local) {
   const int per_view = it.ntx * it.nty;
   const int view = local / per_view, r = local % per_view;
   return VBlock{r % it.ntx, r / it.ntx, view};
+}
+
+// The instance's ray arguments with its haloed copy (and, for the L2 step,
+// its target, weight 1 and the loss partials) filled in.
+NGT_DEVICE RayArgs batch_ray_args(const RenBatchView& v, const RenBatchItem& it, const char* ws, bool l2) {
+  RayArgs a = it.a;
+  a.bricked = reinterpret_cast<const float*>(ws + it.ws_bricked);
+  if (l2) {
+    a.l2_target = it.l2_target;
+    a.l2_scale = 1.f;
+    a.l2_partial = reinterpret_cast<float*>(const_cast<char*>(ws) + v.h->ws_partial);
+  }
+  return a;
 }
 
 __global__ __launch_bounds__(256) void render_brickify_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws) {
@@ -1381,38 +1416,78 @@ __global__ __launch_bounds__(256) void render_fwd_batch_kernel(const void* __res
   for (int t = blockIdx.x; t < v.h->tiles_ray; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.ray, v.h->n, t));
     const RenBatchItem& it = v.items[i];
-    RayArgs a = it.a;
-    a.bricked = reinterpret_cast<const float*>(ws + it.ws_bricked);
-    raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(ray_tile(it, t - v.ray[i]), a, it.cb, it.images);
+    raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(ray_tile(it, t - v.ray[i]),
+                                                                  batch_ray_args(v, it, ws, false), it.cb, it.images, t);
   }
 }
 
+// Ray tiles t0 .. t1 (one chunk): the backward's walk that records each
+// sample's gradient for the given image gradients.
 template <int MODE>
-__global__ __launch_bounds__(256) void render_prepass_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws) {
+__global__ __launch_bounds__(256) void render_prepass_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws,
+                                                                   int t0, int t1) {
   const RenBatchView v = ren_batch_view(plan);
-  for (int t = blockIdx.x; t < v.h->tiles_ray; t += gridDim.x) {
+  char* rec = ws + v.h->ws_record;
+  for (int t = t0 + static_cast<int>(blockIdx.x); t < t1; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.ray, v.h->n, t));
     const RenBatchItem& it = v.items[i];
-    RayArgs a = it.a;
-    a.bricked = reinterpret_cast<const float*>(ws + it.ws_bricked);
     raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(
-        ray_tile(it, t - v.ray[i]), a, it.cb, 0, it.images, it.images_grad,
-        reinterpret_cast<float*>(ws + it.ws_G), reinterpret_cast<RayRec*>(ws + it.ws_rays), it.kcap);
+        ray_tile(it, t - v.ray[i]), batch_ray_args(v, it, ws, false), it.cb, 0, it.images, it.images_grad,
+        reinterpret_cast<float*>(rec + it.rec_G), reinterpret_cast<RayRec*>(rec + it.rec_rays), it.kcap);
   }
 }
 
-__global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws) {
+// Ray tiles t0 .. t1 (one chunk) of the L2 step: the image, its loss partial
+// (slot t) and the recorded per-sample factors in one walk.
+template <int MODE>
+__global__ __launch_bounds__(256) void render_record_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws,
+                                                                  int t0, int t1) {
   const RenBatchView v = ren_batch_view(plan);
-  for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
-    const int i = __builtin_amdgcn_readfirstlane(find_item(v.brick, v.h->n, t));
+  char* rec = ws + v.h->ws_record;
+  for (int t = t0 + static_cast<int>(blockIdx.x); t < t1; t += gridDim.x) {
+    const int i = __builtin_amdgcn_readfirstlane(find_item(v.ray, v.h->n, t));
     const RenBatchItem& it = v.items[i];
-    const RayArgs a = it.a;
-    raymarch_bwd_brick_kernel_body<1, 8, 8, 8>(VBlock{t - v.brick[i], 0, 0}, a, it.cb, 0,
-                                               reinterpret_cast<const RayRec*>(ws + it.ws_rays),
-                                               reinterpret_cast<const float*>(ws + it.ws_G), it.kcap,
-                                               it.density_grad, 0, nullptr);
+    raymarch_fwd_record_kernel_body<MODE>(ray_tile(it, t - v.ray[i]), batch_ray_args(v, it, ws, true), it.cb,
+                                          it.images, 0, reinterpret_cast<float*>(rec + it.rec_G),
+                                          reinterpret_cast<RayRec*>(rec + it.rec_rays), it.kcap, t);
+    __syncthreads();  // the loss partial's LDS is written again by the next tile
+  }
+}
+
+// Brick tiles t0 .. t1 (one chunk) of gather shape SHAPE.
+template <int SHAPE>
+__global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* __restrict__ plan, char* __restrict__ ws,
+                                                                 int t0, int t1) {
+  constexpr int BX = SHAPE == 0 ? 8 : 16, BY = SHAPE == 2 ? 16 : 8, BZ = 8;
+  const RenBatchView v = ren_batch_view(plan);
+  const int* table = reinterpret_cast<const int*>(as_constant<char>(plan) + v.h->off_brick[SHAPE]);
+  const char* rec = ws + v.h->ws_record;
+  for (int t = t0 + static_cast<int>(blockIdx.x); t < t1; t += gridDim.x) {
+    const int i = __builtin_amdgcn_readfirstlane(find_item(table, v.h->n, t));
+    const RenBatchItem& it = v.items[i];
+    raymarch_bwd_brick_kernel_body<1, BX, BY, BZ>(VBlock{t - table[i], 0, 0}, it.a, it.cb, 0,
+                                                  reinterpret_cast<const RayRec*>(rec + it.rec_rays),
+                                                  reinterpret_cast<const float*>(rec + it.rec_G), it.kcap,
+                                                  it.density_grad, 0, nullptr);
     __syncthreads();  // the brick's LDS arrays are rewritten by the next brick
   }
+}
+
+// One work-group per instance: its loss, the fixed-order double sum of its
+// ray tiles' partials.
+__global__ __launch_bounds__(256) void render_l2_finish_batch_kernel(const void* __restrict__ plan,
+                                                                     const char* __restrict__ ws) {
+  const RenBatchView v = ren_batch_view(plan);
+  const int i = blockIdx.x;
+  const float* partial = reinterpret_cast<const float*>(ws + v.h->ws_partial);
+  __shared__ double s_w[4];
+  double sum = 0.0;
+  for (int t = v.ray[i] + static_cast<int>(threadIdx.x); t < v.ray[i + 1]; t += 256) sum += static_cast<double>(partial[t]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) *v.items[i].loss = static_cast<float>((s_w[0] + s_w[1]) + (s_w[2] + s_w[3]));
 }
 
 // Pixels of the N x V images of a fused render.
@@ -1849,30 +1924,38 @@ size_t ngt_render_batch_plan_bytes(int num_instances) {
   BlobLayout L;
   L.add<RenBatchHeader>(1);
   L.add<RenBatchItem>(num_instances);
-  for (int k = 0; k < 3; ++k) L.add<int>(num_instances + 1);
+  for (int k = 0; k < 3 + kGatherShapes; ++k) L.add<int>(num_instances + 1);
   return L.bytes;
 }
 
-int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_mode, void* plan, size_t plan_bytes,
-                          size_t* workspace_bytes) {
+int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_mode, size_t record_budget, void* plan,
+                          size_t plan_bytes, size_t* workspace_bytes) {
   NGT_REQUIRE(inst != nullptr && n > 0, NGT_ERR_INVALID_ARGUMENT, "batch must not be empty");
   NGT_REQUIRE(blending_mode == BEER_LAMBERT || blending_mode == ALPHA || blending_mode == ALPHA_ADDITIVE ||
                   blending_mode == ADDITIVE,
               NGT_ERR_INVALID_ARGUMENT, "invalid blending mode");
   NGT_REQUIRE(plan != nullptr && plan_bytes >= ngt_render_batch_plan_bytes(n), NGT_ERR_WORKSPACE,
               "plan buffer too small for ngt_render_batch_plan");
+  if (record_budget == 0) record_budget = kDefaultRecordBudget;
   BlobLayout L;
   const size_t off_h = L.add<RenBatchHeader>(1);
   const size_t off_items = L.add<RenBatchItem>(n);
-  const size_t off_ray = L.add<int>(n + 1), off_brickify = L.add<int>(n + 1), off_brick = L.add<int>(n + 1);
+  const size_t off_ray = L.add<int>(n + 1), off_brickify = L.add<int>(n + 1), off_chunk = L.add<int>(n + 1);
+  size_t off_brick[kGatherShapes];
+  for (int k = 0; k < kGatherShapes; ++k) off_brick[k] = L.add<int>(n + 1);
   RenBatchHeader* h = blob_at<RenBatchHeader>(plan, off_h);
   RenBatchItem* items = blob_at<RenBatchItem>(plan, off_items);
   int* ray = blob_at<int>(plan, off_ray);
   int* brickify = blob_at<int>(plan, off_brickify);
-  int* brick = blob_at<int>(plan, off_brick);
-  ray[0] = brickify[0] = brick[0] = 0;
-  int64_t tr = 0, tz = 0, tb = 0;
-  size_t ws = 0;
+  int* chunk = blob_at<int>(plan, off_chunk);
+  int* brick[kGatherShapes];
+  for (int k = 0; k < kGatherShapes; ++k) brick[k] = blob_at<int>(plan, off_brick[k]);
+  static const int kShapeX[kGatherShapes] = {8, 16, 16}, kShapeY[kGatherShapes] = {8, 8, 16};
+  ray[0] = brickify[0] = chunk[0] = 0;
+  for (int k = 0; k < kGatherShapes; ++k) brick[k][0] = 0;
+  int64_t tr = 0, tz = 0, tb[kGatherShapes] = {0, 0, 0};
+  size_t ws = 0, chunk_bytes = 0, region = 0;
+  int nchunks = 0;
   for (int i = 0; i < n; ++i) {
     const ngt_render_instance& q = inst[i];
     NGT_REQUIRE(q.density && q.matrix_m && q.matrix_v && q.matrix_p && q.frustum, NGT_ERR_INVALID_ARGUMENT,
@@ -1885,6 +1968,7 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     int rc = check_grid_shape(sh, "batch density");
     if (rc) return rc;
     RenBatchItem& it = items[i];
+    it = RenBatchItem{};
     it.a = make_ray_args(q.density, sh, q.image_dhw);
     it.a.light_r = q.light_rgb[0]; it.a.light_g = q.light_rgb[1]; it.a.light_b = q.light_rgb[2];
     it.a.depth_step = q.depth_step;
@@ -1901,33 +1985,94 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     NGT_REQUIRE(bricked_fits(it.a.din, 1), NGT_ERR_INVALID_ARGUMENT, "batch density grid too large");
     it.brick_tiles = static_cast<int>(brick_rows(it.a.bd));
     it.images = q.images; it.images_grad = q.images_grad; it.density_grad = q.density_grad;
-    const size_t npix = static_cast<size_t>(it.a.out_w) * it.a.out_h;
+    it.l2_target = q.l2_target; it.loss = q.loss;
     it.ws_bricked = static_cast<int64_t>(ws);
     ws += align_up(static_cast<size_t>(it.a.bd.slice_cells) * sizeof(float));
-    it.ws_rays = static_cast<int64_t>(ws);
-    ws += align_up(q.num_views * npix * sizeof(RayRec));
-    it.ws_G = static_cast<int64_t>(ws);
-    ws += align_up(q.num_views * static_cast<size_t>(it.kcap) * g_pixels(npix) * sizeof(float));
+    // records: chunk-relative offsets; a new chunk starts where the budget would be exceeded
+    const size_t npix = static_cast<size_t>(it.a.out_w) * it.a.out_h;
+    const size_t rays_bytes = align_up(q.num_views * npix * sizeof(RayRec));
+    const size_t g_bytes = align_up(q.num_views * static_cast<size_t>(it.kcap) * g_pixels(npix) * sizeof(float));
+    if (chunk_bytes > 0 && chunk_bytes + rays_bytes + g_bytes > record_budget) {
+      chunk[++nchunks] = i;
+      chunk_bytes = 0;
+    }
+    it.rec_rays = static_cast<int64_t>(chunk_bytes);
+    it.rec_G = static_cast<int64_t>(chunk_bytes + rays_bytes);
+    chunk_bytes += rays_bytes + g_bytes;
+    region = std::max(region, chunk_bytes);
     tr += static_cast<int64_t>(q.num_views) * it.ntx * it.nty;
     tz += it.brick_tiles;
-    tb += static_cast<int64_t>(blocks_for(it.a.din.x, 8)) * blocks_for(it.a.din.y, 8) * blocks_for(it.a.din.z, 8);
-    NGT_REQUIRE(tr < (int64_t(1) << 31) && tz < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT, "batch too large");
-    ray[i + 1] = static_cast<int>(tr); brickify[i + 1] = static_cast<int>(tz); brick[i + 1] = static_cast<int>(tb);
+    for (int k = 0; k < kGatherShapes; ++k)
+      tb[k] += static_cast<int64_t>(blocks_for(it.a.din.x, kShapeX[k])) * blocks_for(it.a.din.y, kShapeY[k]) *
+               blocks_for(it.a.din.z, 8);
+    NGT_REQUIRE(tr < (int64_t(1) << 31) && tz < (int64_t(1) << 31) && tb[0] < (int64_t(1) << 31),
+                NGT_ERR_INVALID_ARGUMENT, "batch too large");
+    ray[i + 1] = static_cast<int>(tr); brickify[i + 1] = static_cast<int>(tz);
+    for (int k = 0; k < kGatherShapes; ++k) brick[k][i + 1] = static_cast<int>(tb[k]);
   }
-  h->n = n; h->mode = blending_mode;
-  h->tiles_ray = static_cast<int>(tr); h->tiles_brickify = static_cast<int>(tz); h->tiles_brick = static_cast<int>(tb);
-  h->off_items = off_items; h->off_ray = off_ray; h->off_brickify = off_brickify; h->off_brick = off_brick;
+  chunk[++nchunks] = n;
+  h->n = n; h->mode = blending_mode; h->nchunks = nchunks;
+  h->tiles_ray = static_cast<int>(tr); h->tiles_brickify = static_cast<int>(tz);
+  for (int k = 0; k < kGatherShapes; ++k) {
+    h->tiles_brick[k] = static_cast<int>(tb[k]);
+    h->off_brick[k] = off_brick[k];
+  }
+  h->off_items = off_items; h->off_ray = off_ray; h->off_brickify = off_brickify; h->off_chunk = off_chunk;
+  h->ws_partial = static_cast<int64_t>(ws);
+  ws += align_up(static_cast<size_t>(tr) * sizeof(float));
+  h->ws_record = static_cast<int64_t>(ws);
+  h->record_bytes = static_cast<int64_t>(region);
+  ws += region;
   h->ws_bytes = static_cast<int64_t>(ws);
   if (workspace_bytes) *workspace_bytes = ws;
   return NGT_OK;
 }
 
-int ngt_render_batch_fwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
-                         int blocks_per_cu, ngt_stream_t stream) {
+int ngt_render_batch_plan_info(const void* plan_host, int* num_chunks, size_t* record_bytes) {
+  NGT_REQUIRE(plan_host != nullptr, NGT_ERR_INVALID_ARGUMENT, "plan must be given");
+  const RenBatchHeader& h = *static_cast<const RenBatchHeader*>(plan_host);
+  if (num_chunks) *num_chunks = h.nchunks;
+  if (record_bytes) *record_bytes = static_cast<size_t>(h.record_bytes);
+  return NGT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_batch_call(const void* plan_host, const void* plan_device, size_t workspace_bytes, int gather_brick,
+                     const char* who) {
   NGT_REQUIRE(plan_host && plan_device, NGT_ERR_INVALID_ARGUMENT, "plan must be given");
   const RenBatchHeader& h = *static_cast<const RenBatchHeader*>(plan_host);
-  NGT_REQUIRE(workspace != nullptr && workspace_bytes >= static_cast<size_t>(h.ws_bytes), NGT_ERR_WORKSPACE,
-              "workspace too small for ngt_render_batch_fwd");
+  NGT_REQUIRE(workspace_bytes >= static_cast<size_t>(h.ws_bytes), NGT_ERR_WORKSPACE,
+              (std::string("workspace too small for ") + who).c_str());
+  NGT_REQUIRE(gather_brick >= 0 && gather_brick < kGatherShapes, NGT_ERR_INVALID_ARGUMENT,
+              "gather_brick must be 0 (8x8x8), 1 (16x8x8) or 2 (16x16x8)");
+  return NGT_OK;
+}
+
+// The gather over the brick tiles of chunk c.
+void launch_batch_gather(const RenBatchHeader& h, const void* plan_host, const void* plan_device, char* ws, int c,
+                         int blocks_per_cu, int shape, hipStream_t s) {
+  const int* chunk = blob_at<int>(plan_host, h.off_chunk);
+  const int* table = blob_at<int>(plan_host, h.off_brick[shape]);
+  const int t0 = table[chunk[c]], t1 = table[chunk[c + 1]];
+  const dim3 g(persistent_grid(t1 - t0, blocks_per_cu));
+  if (shape == 0) hipLaunchKernelGGL((render_brick_batch_kernel<0>), g, dim3(256), 0, s, plan_device, ws, t0, t1);
+  else if (shape == 1) hipLaunchKernelGGL((render_brick_batch_kernel<1>), g, dim3(256), 0, s, plan_device, ws, t0, t1);
+  else hipLaunchKernelGGL((render_brick_batch_kernel<2>), g, dim3(256), 0, s, plan_device, ws, t0, t1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ngt_render_batch_fwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                         int blocks_per_cu, ngt_stream_t stream) {
+  int rc = check_batch_call(plan_host, plan_device, workspace_bytes, 0, "ngt_render_batch_fwd");
+  if (rc) return rc;
+  const RenBatchHeader& h = *static_cast<const RenBatchHeader*>(plan_host);
+  NGT_REQUIRE(workspace != nullptr, NGT_ERR_WORKSPACE, "workspace must be given");
   const RenBatchItem* items = blob_at<RenBatchItem>(plan_host, h.off_items);
   for (int i = 0; i < h.n; ++i) NGT_REQUIRE(items[i].images, NGT_ERR_INVALID_ARGUMENT, "batch instance without images");
   hipStream_t s = as_stream(stream);
@@ -1940,25 +2085,54 @@ int ngt_render_batch_fwd(const void* plan_host, const void* plan_device, void* w
 }
 
 int ngt_render_batch_bwd(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
-                         int blocks_per_cu, ngt_stream_t stream) {
-  NGT_REQUIRE(plan_host && plan_device, NGT_ERR_INVALID_ARGUMENT, "plan must be given");
+                         int blocks_per_cu, int gather_brick, ngt_stream_t stream) {
+  int rc = check_batch_call(plan_host, plan_device, workspace_bytes, gather_brick, "ngt_render_batch_bwd");
+  if (rc) return rc;
   const RenBatchHeader& h = *static_cast<const RenBatchHeader*>(plan_host);
-  NGT_REQUIRE(workspace != nullptr && workspace_bytes >= static_cast<size_t>(h.ws_bytes), NGT_ERR_WORKSPACE,
-              "workspace too small for ngt_render_batch_bwd");
+  NGT_REQUIRE(workspace != nullptr, NGT_ERR_WORKSPACE, "workspace must be given");
   const RenBatchItem* items = blob_at<RenBatchItem>(plan_host, h.off_items);
   for (int i = 0; i < h.n; ++i)
     NGT_REQUIRE(items[i].images && items[i].images_grad && items[i].density_grad, NGT_ERR_INVALID_ARGUMENT,
                 "batch instance without images, image gradients or density gradient");
   hipStream_t s = as_stream(stream);
   char* ws = static_cast<char*>(workspace);
+  const int* chunk = blob_at<int>(plan_host, h.off_chunk);
+  const int* ray = blob_at<int>(plan_host, h.off_ray);
+  for (int c = 0; c < h.nchunks; ++c) {
+    const int t0 = ray[chunk[c]], t1 = ray[chunk[c + 1]];
+    const dim3 g(persistent_grid(t1 - t0, blocks_per_cu));
+    NGT_SWITCH_MODE(h.mode,
+        hipLaunchKernelGGL((render_prepass_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws, t0, t1));
+    launch_batch_gather(h, plan_host, plan_device, ws, c, blocks_per_cu, gather_brick, s);
+  }
+  return check_launch("ngt_render_batch_bwd");
+}
+
+int ngt_render_batch_l2(const void* plan_host, const void* plan_device, void* workspace, size_t workspace_bytes,
+                        int blocks_per_cu, int gather_brick, ngt_stream_t stream) {
+  int rc = check_batch_call(plan_host, plan_device, workspace_bytes, gather_brick, "ngt_render_batch_l2");
+  if (rc) return rc;
+  const RenBatchHeader& h = *static_cast<const RenBatchHeader*>(plan_host);
+  NGT_REQUIRE(workspace != nullptr, NGT_ERR_WORKSPACE, "workspace must be given");
+  const RenBatchItem* items = blob_at<RenBatchItem>(plan_host, h.off_items);
+  for (int i = 0; i < h.n; ++i)
+    NGT_REQUIRE(items[i].images && items[i].l2_target && items[i].loss && items[i].density_grad,
+                NGT_ERR_INVALID_ARGUMENT, "batch instance without images, target, loss or density gradient");
+  hipStream_t s = as_stream(stream);
+  char* ws = static_cast<char*>(workspace);
   hipLaunchKernelGGL(render_brickify_batch_kernel, dim3(persistent_grid(h.tiles_brickify, blocks_per_cu)),
                      dim3(256), 0, s, plan_device, ws);
-  const dim3 g(persistent_grid(h.tiles_ray, blocks_per_cu));
-  NGT_SWITCH_MODE(h.mode,
-      hipLaunchKernelGGL((render_prepass_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws));
-  hipLaunchKernelGGL(render_brick_batch_kernel, dim3(persistent_grid(h.tiles_brick, blocks_per_cu)), dim3(256), 0, s,
-                     plan_device, ws);
-  return check_launch("ngt_render_batch_bwd");
+  const int* chunk = blob_at<int>(plan_host, h.off_chunk);
+  const int* ray = blob_at<int>(plan_host, h.off_ray);
+  for (int c = 0; c < h.nchunks; ++c) {
+    const int t0 = ray[chunk[c]], t1 = ray[chunk[c + 1]];
+    const dim3 g(persistent_grid(t1 - t0, blocks_per_cu));
+    NGT_SWITCH_MODE(h.mode,
+        hipLaunchKernelGGL((render_record_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws, t0, t1));
+    launch_batch_gather(h, plan_host, plan_device, ws, c, blocks_per_cu, gather_brick, s);
+  }
+  hipLaunchKernelGGL(render_l2_finish_batch_kernel, dim3(h.n), dim3(256), 0, s, plan_device, ws);
+  return check_launch("ngt_render_batch_l2");
 }
 
 }  // extern "C"

@@ -87,7 +87,7 @@ class RenderInstance(ctypes.Structure):
     _fields_ = [("dhw", ctypes.c_int32 * 3), ("density", _vp), ("num_views", ctypes.c_int32), ("matrix_m", _c_fp),
                 ("matrix_v", _c_fp), ("matrix_p", _c_fp), ("frustum", _c_fp), ("image_dhw", ctypes.c_int32 * 3),
                 ("light_rgb", ctypes.c_float * 3), ("depth_step", ctypes.c_float), ("images", _vp),
-                ("images_grad", _vp), ("density_grad", _vp)]
+                ("images_grad", _vp), ("density_grad", _vp), ("l2_target", _vp), ("loss", _vp)]
 
 
 _szp = ctypes.POINTER(ctypes.c_size_t)
@@ -97,9 +97,11 @@ _SIGNATURES.update({
     "ngt_advect_batch_fwd": (_i, [_vp, _vp, _i, _vp]),
     "ngt_advect_batch_bwd": (_i, [_vp, _vp, _vp, _sz, _i, _vp]),
     "ngt_render_batch_plan_bytes": (_sz, [_i]),
-    "ngt_render_batch_plan": (_i, [ctypes.POINTER(RenderInstance), _i, _i, _vp, _sz, _szp]),
+    "ngt_render_batch_plan": (_i, [ctypes.POINTER(RenderInstance), _i, _i, _sz, _vp, _sz, _szp]),
+    "ngt_render_batch_plan_info": (_i, [_vp, ctypes.POINTER(ctypes.c_int), _szp]),
     "ngt_render_batch_fwd": (_i, [_vp, _vp, _vp, _sz, _i, _vp]),
-    "ngt_render_batch_bwd": (_i, [_vp, _vp, _vp, _sz, _i, _vp]),
+    "ngt_render_batch_bwd": (_i, [_vp, _vp, _vp, _sz, _i, _i, _vp]),
+    "ngt_render_batch_l2": (_i, [_vp, _vp, _vp, _sz, _i, _i, _vp]),
 })
 
 SYMBOLS = tuple(_SIGNATURES)

@@ -5,9 +5,15 @@ kernel (ngt_advect_batch_* / ngt_render_batch_* in include/ngt_transport.h).
 A plan is built once per batch layout (instance sizes and buffers) on the
 host, copied to device memory and reused; the launches only read it, so a
 whole batched forward + backward step can be captured in a HIP graph
-(``torch.cuda.CUDAGraph``) and replayed.  The persistent grid size of each
-kernel family (work-groups per compute unit) is chosen by ``Autotuner`` from
-timed candidates on first use and cached per batch layout.
+(``torch.cuda.CUDAGraph``) and replayed.  ``Autotuner`` times candidate
+launch shapes on first use and caches the fastest per batch layout: the
+persistent grid size of each kernel family (work-groups per compute unit,
+0 = one work-group per tile) and, for the render's gather adjoint, the voxel
+brick of a work-group (8x8x8, 16x8x8 or 16x16x8).
+
+The render's per-sample records (the gather adjoint's only large buffer) are
+bounded by a byte budget: instances are grouped into chunks whose records
+fit it and the backward runs chunk by chunk over one record region.
 
 The reference processes every instance and camera with its own launches
 (resample_grid.cu.cc:748-790, raymarch_grid.cu.cc:399-430); the batched path
@@ -22,6 +28,10 @@ import torch
 from . import _lib, ops
 
 CANDIDATE_BLOCKS_PER_CU = (0, 2, 4, 8, 16)
+GATHER_BRICKS = ("8x8x8", "16x8x8", "16x16x8")
+# (work-groups per compute unit, gather brick index)
+CANDIDATE_RENDER = tuple((b, g) for b in CANDIDATE_BLOCKS_PER_CU for g in range(len(GATHER_BRICKS)))
+DEFAULT_GATHER_BRICK = 2
 
 
 class Autotuner:
@@ -34,12 +44,12 @@ class Autotuner:
         self.best = {}
         self.timings = {}
 
-    def pick(self, key, run):
+    def pick(self, key, run, candidates=None):
         if key in self.best:
             return self.best[key]
         s = torch.cuda.current_stream()
         times = {}
-        for c in self.candidates:
+        for c in (self.candidates if candidates is None else candidates):
             run(c)  # warm
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(s)
@@ -148,20 +158,26 @@ class RenderBatch:
     own grid transform and 1-8 cameras of one image size."""
 
     def __init__(self, densities, cameras, light_rgb=(1.0, 1.0, 1.0), blending_mode="BEER_LAMBERT",
-                 tuner=DEFAULT_TUNER, density_grads=None):
+                 tuner=DEFAULT_TUNER, density_grads=None, targets=None, record_budget=0):
         """densities: list of [D,H,W,1] (or [1,D,H,W,1]) device tensors;
         cameras: list of (matrix_m, matrix_v, matrix_p, frustum, depth_step,
-        output_dhw) in the op layout of ``ngt.ops.render_density_fused``."""
+        output_dhw) in the op layout of ``ngt.ops.render_density_fused``;
+        targets: optional [V,H',W',3] target images per instance (for
+        ``l2_step``); record_budget: bytes for the gather adjoint's records
+        (0: the library's default, 4 GiB)."""
         lib = _lib.load()
         self.n = len(densities)
         if self.n == 0 or len(cameras) != self.n:
             raise ValueError("one camera set per density")
+        if targets is not None and len(targets) != self.n:
+            raise ValueError("one target per density")
         self.tuner = tuner
         self.mode = ops._enum(ops.BLEND, blending_mode, "blend mode")
         self.densities = [_lib.device_tensor(d, "density").reshape(d.shape[-4:-1] + (1,)) for d in densities]
         self._keep = []
         arr = (_lib.RenderInstance * self.n)()
-        self.images, self.image_grads, self.density_grads = [], [], []
+        self.images, self.image_grads, self.density_grads, self.targets = [], [], [], []
+        self.losses = torch.zeros(self.n, device="cuda")
         for i, (M, V, P, F, ds, out_dhw) in enumerate(cameras):
             q = arr[i]
             D, H, W = self.densities[i].shape[:3]
@@ -178,7 +194,8 @@ class RenderBatch:
             for c in range(3):
                 q.light_rgb[c] = float(light_rgb[c])
             q.depth_step = float(ds)
-            img = torch.empty((nv, int(out_dhw[1]), int(out_dhw[2]), 4), device="cuda")
+            hw = (int(out_dhw[1]), int(out_dhw[2]))
+            img = torch.empty((nv,) + hw + (4,), device="cuda")
             self.images.append(img)
             self.image_grads.append(torch.zeros_like(img))
             self.density_grads.append(torch.empty_like(self.densities[i]) if density_grads is None
@@ -186,14 +203,23 @@ class RenderBatch:
             q.images = _ptr(img)
             q.images_grad = _ptr(self.image_grads[-1])
             q.density_grad = _ptr(self.density_grads[-1])
+            if targets is not None:
+                tgt = _lib.device_tensor(targets[i], "target").reshape((nv,) + hw + (3,))
+                self.targets.append(tgt)
+                q.l2_target = _ptr(tgt)
+                q.loss = self.losses.data_ptr() + 4 * i
         nbytes = lib.ngt_render_batch_plan_bytes(self.n)
         blob = (ctypes.c_uint8 * nbytes)()
         ws = ctypes.c_size_t(0)
-        _lib.check(lib.ngt_render_batch_plan(arr, self.n, self.mode, blob, nbytes, ctypes.byref(ws)))
+        _lib.check(lib.ngt_render_batch_plan(arr, self.n, self.mode, int(record_budget), blob, nbytes,
+                                             ctypes.byref(ws)))
+        chunks, rbytes = ctypes.c_int(0), ctypes.c_size_t(0)
+        _lib.check(lib.ngt_render_batch_plan_info(blob, ctypes.byref(chunks), ctypes.byref(rbytes)))
+        self.num_chunks, self.record_bytes = int(chunks.value), int(rbytes.value)
         self.plan_host, self.plan_dev = _upload(bytes(blob))
         self.workspace = torch.empty(max(int(ws.value), 1), dtype=torch.uint8, device="cuda")
         self.key = ("render", tuple(tuple(d.shape) for d in self.densities),
-                    tuple(tuple(i.shape) for i in self.images), self.mode)
+                    tuple(tuple(i.shape) for i in self.images), self.mode, self.num_chunks)
 
     def _fwd(self, bpc):
         lib = _lib.load()
@@ -201,42 +227,69 @@ class RenderBatch:
                                             self.workspace.data_ptr(), self.workspace.numel(), bpc,
                                             _lib.stream(torch.device("cuda"))))
 
-    def _bwd(self, bpc):
+    def _bwd(self, shape):
+        bpc, brick = shape
         lib = _lib.load()
         _lib.check(lib.ngt_render_batch_bwd(self.plan_host.ctypes.data, self.plan_dev.data_ptr(),
-                                            self.workspace.data_ptr(), self.workspace.numel(), bpc,
+                                            self.workspace.data_ptr(), self.workspace.numel(), bpc, brick,
                                             _lib.stream(torch.device("cuda"))))
+
+    def _l2(self, shape):
+        bpc, brick = shape
+        lib = _lib.load()
+        _lib.check(lib.ngt_render_batch_l2(self.plan_host.ctypes.data, self.plan_dev.data_ptr(),
+                                           self.workspace.data_ptr(), self.workspace.numel(), bpc, brick,
+                                           _lib.stream(torch.device("cuda"))))
 
     def forward(self):
         self._fwd(self.tuner.pick(self.key + ("fwd",), self._fwd) if self.tuner else 0)
         return self.images
 
-    def backward(self):
-        """Reads ``image_grads``; fills ``density_grads``."""
-        self._bwd(self.tuner.pick(self.key + ("bwd",), self._bwd) if self.tuner else 0)
+    def backward(self, shape=None):
+        """Reads ``image_grads`` and the images of the last ``forward``;
+        fills ``density_grads``.  shape: (work-groups per CU, gather brick
+        index) instead of the tuned one."""
+        if shape is None:
+            shape = (self.tuner.pick(self.key + ("bwd",), self._bwd, CANDIDATE_RENDER) if self.tuner
+                     else (0, DEFAULT_GATHER_BRICK))
+        self._bwd(shape)
         return self.density_grads
+
+    def l2_step(self, shape=None):
+        """Images, per-instance losses sum((image - target)^2 over RGB) in
+        ``losses`` and their density gradients in ``density_grads``."""
+        if not self.targets:
+            raise ValueError("l2_step needs targets")
+        if shape is None:
+            shape = (self.tuner.pick(self.key + ("l2",), self._l2, CANDIDATE_RENDER) if self.tuner
+                     else (0, DEFAULT_GATHER_BRICK))
+        self._l2(shape)
+        return self.images, self.losses, self.density_grads
 
 
 class TransportRenderBatch:
     """One differentiable step for every instance: MacCormack advection of
     the density, fused render of the advected density, L2 loss against
     per-instance target images, and the full backward to the initial density
-    and velocity, as two batched ops forward and two backward."""
+    and velocity: the advection forward, the render's L2 step (images, loss
+    and density gradient) and the advection backward."""
 
     def __init__(self, densities, velocities, cameras, targets, light_rgb=(1.0, 1.0, 1.0), timestep=-1.0,
-                 clamp="MC_SMOOTH", boundary="BORDER", tuner=DEFAULT_TUNER):
+                 clamp="MC_SMOOTH", boundary="BORDER", tuner=DEFAULT_TUNER, record_budget=0):
         self.adv = AdvectBatch(densities, velocities, timestep, 2, clamp, boundary, tuner)
         # the render's density gradient is written straight into the advection's output gradient
-        self.ren = RenderBatch(self.adv.outputs, cameras, light_rgb, tuner=tuner, density_grads=self.adv.out_grads)
-        self.targets = [t.reshape(img.shape[:3] + (3,)) for t, img in zip(targets, self.ren.images)]
+        self.ren = RenderBatch(self.adv.outputs, cameras, light_rgb, tuner=tuner, density_grads=self.adv.out_grads,
+                               targets=targets, record_budget=record_budget)
+        self.targets = self.ren.targets
 
     def step(self):
         self.adv.forward()
-        images = self.ren.forward()
-        for img, tgt, g in zip(images, self.targets, self.ren.image_grads):
-            torch.mul(img[..., :3] - tgt, 2.0, out=g[..., :3])
-        self.ren.backward()
+        self.ren.l2_step()
         return self.adv.backward()
 
+    def losses(self):
+        """Per-instance losses of the last step (device tensor [N])."""
+        return self.ren.losses
+
     def loss(self):
-        return sum(float(((img[..., :3] - t) ** 2).double().sum()) for img, t in zip(self.ren.images, self.targets))
+        return float(self.ren.losses.double().sum())

@@ -11,6 +11,7 @@
   to a unit cube centred at the origin, cameras on a horizontal ring of
   radius 3 grid extents, 72 degrees apart, horizontal FOV 40 degrees, looking
   at the centre, near / far = R -+ sqrt(3)/2.
+* ``uneven_batch_layout``: the uneven-batch workload's per-sequence sizes.
 """
 import numpy as np
 from scipy.ndimage import gaussian_filter
@@ -82,6 +83,25 @@ def ring_setup(grid_dhw, image_hw=(144, 256), depth=128, views=5, fov=40.0, dist
     far = distance + np.sqrt(3.0) / 2.0
     cams = ring_cameras(views, distance, fov, image_hw, depth, near, far)
     return grid, cams
+
+
+UNEVEN_EDGES = (48, 64, 96, 128, 160)
+UNEVEN_IMAGES = ((72, 128), (144, 256), (270, 480))
+
+
+def uneven_batch_layout(seed=5, sequences=32):
+    """Per sequence (cubic grid edge, views, (image H, W)): the edge drawn
+    from 48, 64, 96, 128, 160, 1-8 ring views, the image from 128x72,
+    256x144, 480x270 (in that order per sequence, from one seeded stream);
+    each view marches edge-many depth slices."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(sequences):
+        e = int(rng.choice(UNEVEN_EDGES))
+        nv = int(rng.integers(1, 9))
+        hw = UNEVEN_IMAGES[int(rng.integers(0, 3))]
+        out.append((e, nv, hw))
+    return out
 
 
 # --- device-side generators (large batches) ----------------------------------------

@@ -56,14 +56,18 @@ def test_advect_batch_matches_per_instance(batch):
             assert rel(a, b.grad[0]) < 1e-6, i
 
 
-def test_render_batch_matches_per_instance(batch):
+@pytest.mark.parametrize("brick", [0, 1, 2])
+def test_render_batch_matches_per_instance(batch, brick):
+    """Every gather brick (8x8x8, 16x8x8, 16x16x8), each instance in a
+    record chunk of its own (record_budget 1 byte)."""
     B, ops = batch["B"], batch["ops"]
-    rb = B.RenderBatch(batch["dens"], batch["cams"], (0.9, 0.8, 0.7), tuner=B.Autotuner())
+    rb = B.RenderBatch(batch["dens"], batch["cams"], (0.9, 0.8, 0.7), tuner=B.Autotuner(), record_budget=1)
+    assert rb.num_chunks == len(SPECS)
     imgs = rb.forward()
     g = torch.Generator("cuda").manual_seed(4)
     for ig in rb.image_grads:
         ig.copy_(torch.randn(ig.shape, device="cuda", generator=g))
-    dg = rb.backward()
+    dg = rb.backward(shape=(0, brick))
     torch.cuda.synchronize()
     for i in range(len(SPECS)):
         M, V, P, F, ds, out_dhw = batch["cams"][i]
@@ -72,6 +76,36 @@ def test_render_batch_matches_per_instance(batch):
         assert torch.equal(imgs[i], ref), i
         ref.backward(rb.image_grads[i])
         assert rel(dg[i], d.grad[0]) < 1e-5, i
+
+
+@pytest.mark.parametrize("blend", ["BEER_LAMBERT", "ALPHA"])
+def test_render_batch_l2_step_matches_fused_l2(batch, blend):
+    """The batched L2 step (recording walk + gather per chunk) against the
+    per-instance fused render + L2 op; one chunk and one chunk per instance
+    give the same bits."""
+    B, ops = batch["B"], batch["ops"]
+    light = (0.9, 0.8, 0.7)
+    one = B.RenderBatch(batch["dens"], batch["cams"], light, blend, tuner=None, targets=batch["tgts"])
+    imgs, losses, dg = one.l2_step(shape=(0, 2))
+    torch.cuda.synchronize()
+    assert one.num_chunks == 1
+    got = ([t.clone() for t in imgs], losses.clone(), [t.clone() for t in dg])
+    many = B.RenderBatch(batch["dens"], batch["cams"], light, blend, tuner=None, targets=batch["tgts"],
+                         record_budget=1)
+    imgs2, losses2, dg2 = many.l2_step(shape=(4, 2))
+    torch.cuda.synchronize()
+    assert many.num_chunks == len(SPECS)
+    assert torch.equal(losses2, got[1])
+    for i in range(len(SPECS)):
+        assert torch.equal(imgs2[i], got[0][i]) and torch.equal(dg2[i], got[2][i]), i
+        M, V, P, F, ds, out_dhw = batch["cams"][i]
+        d = batch["dens"][i][None].clone().requires_grad_()
+        loss, img = ops.render_density_fused_l2(d, batch["tgts"][i][None], light, ds, M, V, P, F, out_dhw,
+                                                blending_mode=blend)
+        loss.backward()
+        assert torch.equal(got[0][i], img.detach()[0]), i
+        assert abs(float(got[1][i]) - float(loss)) <= 1e-6 * float(loss), i
+        assert rel(got[2][i], d.grad[0]) < 1e-5, i
 
 
 def _per_instance_step(batch):
