@@ -53,8 +53,12 @@ struct RayArgs {
 
 NGT_DEVICE int kMaxRaySteps(const RayArgs& a) { return 16 * (a.din.x + a.din.y + a.din.z + a.out_d); }
 
-// Ray samples whose gathers are issued together.
-constexpr int kUnroll = 4;
+// Ray samples whose gathers are issued together.  2 measured fastest on the
+// 128^3 / 5-view render (recording forward 60.3 us; 1: 60.4, 3: 62.5,
+// 4: 64.9, 8: 72.2): with few rays per SIMD the walk is bound by its
+// per-sample recurrence more than by the latency of its corner loads, and
+// wider rounds only add registers and fetched past-the-end samples.
+constexpr int kUnroll = 2;
 
 struct Ray {
   float sx, sy, sz;  // start
@@ -1188,8 +1192,7 @@ int ray_step_cap(const std::vector<Pair>& pairs, const float* M, const float* V,
   if (!(min_step < 1e300)) return -1;
   const double k = std::ceil((diag + 2.0) / (0.99 * min_step)) + 3.0;
   if (!(k <= hard_cap)) return -1;
-  // a multiple of kUnroll: the per-sample gradient rows are written kUnroll
-  // samples (16 bytes) at a time
+  // a multiple of kUnroll (the walks fetch kUnroll samples per round)
   const int cap = static_cast<int>(k);
   return (cap + kUnroll - 1) / kUnroll * kUnroll;
 }
