@@ -907,19 +907,20 @@ struct AdvBatchHeader {
 };
 
 struct AdvBatchItem {
-  AdvArgs a;
+  AdvArgs a;  // its velocity pointers are read from vx, vy, vz below
   BrickGrid bg;
   int nbx, nby;  // 64 x 4 cell tiles in x / y
   int64_t cells;
-  const float* input;
-  float* output;
-  float* sl;
-  const float* g_out;
-  float* g_in;
-  float* gvx;
-  float* gvy;
-  float* gvz;
-  int64_t ws_gsl, ws_gvc, ws_t2, ws_rdir, ws_ovf, ws_t1;  // workspace offsets (floats)
+  gptr<const float> vx, vy, vz;
+  gptr<const float> input;
+  gptr<float> output;
+  gptr<float> sl;
+  gptr<const float> g_out;
+  gptr<float> g_in;
+  gptr<float> gvx;
+  gptr<float> gvy;
+  gptr<float> gvz;
+  int64_t ws_gsl, ws_gvc, ws_t2, ws_ovf, ws_t1;  // workspace offsets (floats)
 };
 
 constexpr int kFillChunk = 1024;
@@ -945,8 +946,51 @@ NGT_DEVICE AdvBatchView adv_batch_view(const void* plan) {
   return v;
 }
 
+// An instance's arguments and buffers as generic pointers converted from
+// their global-typed plan fields (gptr), so that their accesses stay global.
+struct AdvBatchBufs {
+  AdvArgs a;
+  const float* input;
+  float* output;
+  float* sl;
+  const float* g_out;
+  float* g_in;
+  float* gvx;
+  float* gvy;
+  float* gvz;
+};
+
+NGT_DEVICE AdvBatchBufs batch_bufs(const AdvBatchItem& it) {
+  AdvBatchBufs b;
+  b.a = it.a;
+  b.a.vx = (const float*)it.vx;
+  b.a.vy = (const float*)it.vy;
+  b.a.vz = (const float*)it.vz;
+  b.input = (const float*)it.input;
+  b.output = (float*)it.output;
+  b.sl = (float*)it.sl;
+  b.g_out = (const float*)it.g_out;
+  b.g_in = (float*)it.g_in;
+  b.gvx = (float*)it.gvx;
+  b.gvy = (float*)it.gvy;
+  b.gvz = (float*)it.gvz;
+  return b;
+}
+
+// A cell tile of the batched per-cell kernels is 64 x 4 cells in x / y over
+// kCellZ consecutive z planes (the work-group visits the planes in turn, so
+// that finding the tile's instance is paid once per kCellZ planes).
+constexpr int kCellZ = 4;
+
 NGT_DEVICE VBlock cell_tile(const AdvBatchItem& it, int local) {
-  return VBlock{local % it.nbx, (local / it.nbx) % it.nby, local / (it.nbx * it.nby)};
+  return VBlock{local % it.nbx, (local / it.nbx) % it.nby, local / (it.nbx * it.nby) * kCellZ};
+}
+
+// Calls f(VBlock) for the planes z0 .. z0 + kCellZ - 1 of the tile inside the grid.
+template <typename F>
+NGT_DEVICE void for_tile_planes(const AdvBatchItem& it, const VBlock& tile, F&& f) {
+  const int z1 = min(tile.z + kCellZ, it.a.d.z);
+  for (int z = tile.z; z < z1; ++z) f(VBlock{tile.x, tile.y, z});
 }
 
 template <int BM, bool NEG, bool TO_SL>
@@ -955,7 +999,10 @@ __global__ __launch_bounds__(256) void advect_sl_batch_kernel(const void* __rest
   for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.cell, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
-    advect_sl_kernel_body<1, BM, NEG>(cell_tile(it, t - v.cell[i]), it.a, it.input, TO_SL ? it.sl : it.output);
+    const AdvBatchBufs b = batch_bufs(it);
+    for_tile_planes(it, cell_tile(it, t - v.cell[i]), [&](const VBlock& vb) {
+      advect_sl_kernel_body<1, BM, NEG>(vb, b.a, b.input, TO_SL ? b.sl : b.output);
+    });
   }
 }
 
@@ -965,23 +1012,24 @@ __global__ __launch_bounds__(256) void advect_mc_batch_kernel(const void* __rest
   for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.cell, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
-    advect_mc_kernel_body<1, BM, CLAMPM>(cell_tile(it, t - v.cell[i]), it.a, it.input, it.sl, it.output);
+    const AdvBatchBufs b = batch_bufs(it);
+    for_tile_planes(it, cell_tile(it, t - v.cell[i]), [&](const VBlock& vb) {
+      advect_mc_kernel_body<1, BM, CLAMPM>(vb, b.a, b.input, b.sl, b.output);
+    });
   }
 }
 
-// zero input_grad (range 0) and ovf1 (range 1, order 2) of every instance
+// zero what the adjoint's atomics add into: the overflow field of the first
+// brick kernel (order 2; the first kernel then stores the direct rho term
+// into input_grad, every cell once, before the second kernel's overflow adds
+// to it), or input_grad itself (order 1)
 __global__ __launch_bounds__(256) void advect_fill_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
-  const int ranges = v.h->order == 2 ? 2 : 1;
   for (int t = blockIdx.x; t < v.h->tiles_fill; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.fill, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
-    const int local = t - v.fill[i];
-    const int chunks = static_cast<int>((it.cells + kFillChunk - 1) / kFillChunk);
-    const int range = local / chunks;
-    if (range >= ranges) continue;
-    float* dst = range == 0 ? it.g_in : ws + it.ws_ovf;
-    const int64_t b = static_cast<int64_t>(local % chunks) * kFillChunk;
+    float* dst = v.h->order == 2 ? ws + it.ws_ovf : (float*)it.g_in;
+    const int64_t b = static_cast<int64_t>(t - v.fill[i]) * kFillChunk;
     for (int64_t e = b + threadIdx.x; e < min(b + kFillChunk, it.cells); e += blockDim.x) dst[e] = 0.f;
   }
 }
@@ -992,10 +1040,14 @@ __global__ __launch_bounds__(kBrickThreads) void advect_mc_bwd_batch_kernel(cons
   for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.brick, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
-    float* g_vc = v.h->staggered ? ws + it.ws_gvc : it.gvx;
-    advect_mc_bwd_brick_kernel_body<1, BM, CLAMPM>(VBlock{t - v.brick[i], 0, 0}, it.a, it.bg, it.input, it.sl,
-                                                   it.g_out, ws + it.ws_gsl, ws + it.ws_rdir, ws + it.ws_ovf,
-                                                   ws + it.ws_t1, g_vc, 1);
+    const AdvBatchBufs b = batch_bufs(it);
+    float* g_vc = v.h->staggered ? ws + it.ws_gvc : b.gvx;
+    // the direct rho term goes straight to input_grad; the masked output
+    // gradient is kept only where NEGATIVE changes it
+    advect_mc_bwd_brick_kernel_body<1, BM, CLAMPM>(VBlock{t - v.brick[i], 0, 0}, b.a, it.bg, b.input, b.sl,
+                                                   b.g_out,
+                                                   CLAMPM == NGT_CLAMP_NEGATIVE ? ws + it.ws_gsl : nullptr,
+                                                   b.g_in, ws + it.ws_ovf, ws + it.ws_t1, g_vc, 1);
     __syncthreads();  // the LDS tile is cleared again by the next brick
   }
 }
@@ -1006,10 +1058,12 @@ __global__ __launch_bounds__(kBrickThreads) void advect_sl_bwd_batch_kernel(cons
   for (int t = blockIdx.x; t < v.h->tiles_brick; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.brick, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
-    float* g_vc = v.h->staggered ? ws + it.ws_gvc : it.gvx;
-    advect_sl_bwd_brick_kernel_body<1, BM, NEG, ORDER2>(VBlock{t - v.brick[i], 0, 0}, it.a, it.bg, it.input,
-                                                        ORDER2 ? ws + it.ws_gsl : it.g_out, ws + it.ws_ovf,
-                                                        ws + it.ws_t1, it.g_in, ws + it.ws_t2, g_vc, ORDER2 ? 1 : 0);
+    const AdvBatchBufs b = batch_bufs(it);
+    float* g_vc = v.h->staggered ? ws + it.ws_gvc : b.gvx;
+    const float* g = ORDER2 && v.h->clamp == NGT_CLAMP_NEGATIVE ? ws + it.ws_gsl : b.g_out;
+    advect_sl_bwd_brick_kernel_body<1, BM, NEG, ORDER2>(VBlock{t - v.brick[i], 0, 0}, b.a, it.bg, b.input,
+                                                        g, ws + it.ws_ovf,
+                                                        ws + it.ws_t1, b.g_in, ws + it.ws_t2, g_vc, ORDER2 ? 1 : 0);
     __syncthreads();
   }
 }
@@ -1019,8 +1073,10 @@ __global__ __launch_bounds__(256) void advect_rho_gather_batch_kernel(const void
   for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.cell, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
-    advect_rho_gather_kernel_body<1>(cell_tile(it, t - v.cell[i]), it.a, it.bg,
-                                     v.h->order == 2 ? ws + it.ws_rdir : nullptr, ws + it.ws_t2, it.g_in);
+    float* g_in = (float*)it.g_in;
+    for_tile_planes(it, cell_tile(it, t - v.cell[i]), [&](const VBlock& vb) {
+      advect_rho_gather_kernel_body<1>(vb, it.a, it.bg, nullptr, ws + it.ws_t2, g_in);
+    });
   }
 }
 
@@ -1029,8 +1085,8 @@ __global__ __launch_bounds__(256) void advect_faces_batch_kernel(const void* __r
   for (int t = blockIdx.x; t < v.h->tiles_face; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.face, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
-    faces_from_centered_grad_kernel_body(VBlock{t - v.face[i], 0, 0}, it.a.d, 1, ws + it.ws_gvc, it.gvx, it.gvy,
-                                         it.gvz);
+    faces_from_centered_grad_kernel_body(VBlock{t - v.face[i], 0, 0}, it.a.d, 1, ws + it.ws_gvc, (float*)it.gvx,
+                                         (float*)it.gvy, (float*)it.gvz);
   }
 }
 
@@ -1266,27 +1322,27 @@ int ngt_advect_batch_plan(const ngt_advect_instance* inst, int n, int order, int
     it.nbx = static_cast<int>(blocks_for(it.a.d.x, kBX));
     it.nby = static_cast<int>(blocks_for(it.a.d.y, kBY));
     it.cells = static_cast<int64_t>(q.dhw[0]) * q.dhw[1] * q.dhw[2];
-    it.input = q.input; it.output = q.output; it.sl = q.sl;
-    it.g_out = q.output_grad; it.g_in = q.input_grad;
-    it.gvx = q.vel_x_grad; it.gvy = q.vel_y_grad; it.gvz = q.vel_z_grad;
+    it.vx = (gptr<const float>)q.vel_x; it.vy = (gptr<const float>)q.vel_y; it.vz = (gptr<const float>)q.vel_z;
+    it.input = (gptr<const float>)q.input; it.output = (gptr<float>)q.output; it.sl = (gptr<float>)q.sl;
+    it.g_out = (gptr<const float>)q.output_grad; it.g_in = (gptr<float>)q.input_grad;
+    it.gvx = (gptr<float>)q.vel_x_grad; it.gvy = (gptr<float>)q.vel_y_grad; it.gvz = (gptr<float>)q.vel_z_grad;
     const int64_t tiles = static_cast<int64_t>(it.bg.count()) * kTileCells;
     auto take = [&](int64_t floats) {
       const int64_t off = ws;
       ws += (floats + 63) / 64 * 64;  // 256-byte aligned regions
       return off;
     };
-    it.ws_gsl = order == 2 ? take(it.cells) : 0;
+    it.ws_gsl = order == 2 && clamp == NGT_CLAMP_NEGATIVE ? take(it.cells) : 0;
     it.ws_gvc = staggered ? take(it.cells * 3) : 0;
     it.ws_t2 = take(tiles);
-    it.ws_rdir = order == 2 ? take(it.cells) : 0;
     it.ws_ovf = order == 2 ? take(it.cells) : 0;
     it.ws_t1 = order == 2 ? take(tiles) : 0;
-    tc += static_cast<int64_t>(it.nbx) * it.nby * q.dhw[0];
+    tc += static_cast<int64_t>(it.nbx) * it.nby * ((q.dhw[0] + kCellZ - 1) / kCellZ);
     tb += it.bg.count();
     if (staggered) {
       tf += face_tiles(it.a.d);
     }
-    tl += static_cast<int64_t>(order == 2 ? 2 : 1) * ((it.cells + kFillChunk - 1) / kFillChunk);
+    tl += (it.cells + kFillChunk - 1) / kFillChunk;
     NGT_REQUIRE(tc < (int64_t(1) << 31) && tl < (int64_t(1) << 31) && tf < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT,
                 "batch too large");
     cell[i + 1] = static_cast<int>(tc); brick[i + 1] = static_cast<int>(tb);

@@ -113,6 +113,15 @@ NGT_DEVICE const T* as_constant(const void* p) {
   return (const T*)((const __attribute__((address_space(4))) char*)p);
 }
 
+// A device pointer stored in a plan blob, typed as global memory.  The
+// compiler cannot tell the address space of a generic pointer loaded from
+// memory and would access it with flat instructions, whose completion a
+// wavefront can only wait for together with its LDS and scalar traffic; a
+// pointer loaded with this type and converted to a generic one where it is
+// used keeps its global accesses.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
 template <typename T>
 NGT_HOST_DEVICE_INLINE T* blob_at(void* base, size_t off) {
   return reinterpret_cast<T*>(static_cast<char*>(base) + off);

@@ -1352,16 +1352,17 @@ struct RenBatchHeader {
 };
 
 struct RenBatchItem {
-  RayArgs a;
+  RayArgs a;  // its density pointer is read from `density` below
   CamBatch cb;
   int ntx, nty;        // 16 x 16 pixel tiles per view
   int kcap;
   int brick_tiles;     // brick rows of the haloed copy (one work-group each)
-  float* images;
-  const float* images_grad;
-  float* density_grad;
-  const float* l2_target;
-  float* loss;
+  gptr<const float> density;
+  gptr<float> images;
+  gptr<const float> images_grad;
+  gptr<float> density_grad;
+  gptr<const float> l2_target;
+  gptr<float> loss;
   int64_t ws_bricked;        // byte offset of the haloed copy in the workspace
   int64_t rec_rays, rec_G;   // byte offsets of the records in the record region
 };
@@ -1393,9 +1394,13 @@ NGT_DEVICE VBlock ray_tile(const RenBatchItem& it, int local) {
 // its target, weight 1 and the loss partials) filled in.
 NGT_DEVICE RayArgs batch_ray_args(const RenBatchView& v, const RenBatchItem& it, const char* ws, bool l2) {
   RayArgs a = it.a;
+  a.input = (const float*)it.density;
   a.bricked = reinterpret_cast<const float*>(ws + it.ws_bricked);
+  a.l2_target = nullptr;
+  a.l2_loss_grad = nullptr;
+  a.l2_partial = nullptr;
   if (l2) {
-    a.l2_target = it.l2_target;
+    a.l2_target = (const float*)it.l2_target;
     a.l2_scale = 1.f;
     a.l2_partial = reinterpret_cast<float*>(const_cast<char*>(ws) + v.h->ws_partial);
   }
@@ -1408,7 +1413,7 @@ __global__ __launch_bounds__(256) void render_brickify_batch_kernel(const void* 
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.brickify, v.h->n, t));
     const RenBatchItem& it = v.items[i];
     const int local = t - v.brickify[i];
-    to_bricked_kernel_body<1, BORDER>(VBlock{local, 0, 0}, it.a.input, it.a.din, it.a.bd,
+    to_bricked_kernel_body<1, BORDER>(VBlock{local, 0, 0}, (const float*)it.density, it.a.din, it.a.bd,
                                       reinterpret_cast<float*>(ws + it.ws_bricked));
   }
 }
@@ -1420,7 +1425,8 @@ __global__ __launch_bounds__(256) void render_fwd_batch_kernel(const void* __res
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.ray, v.h->n, t));
     const RenBatchItem& it = v.items[i];
     raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(ray_tile(it, t - v.ray[i]),
-                                                                  batch_ray_args(v, it, ws, false), it.cb, it.images, t);
+                                                                  batch_ray_args(v, it, ws, false), it.cb,
+                                                                  (float*)it.images, t);
   }
 }
 
@@ -1435,7 +1441,8 @@ __global__ __launch_bounds__(256) void render_prepass_batch_kernel(const void* _
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.ray, v.h->n, t));
     const RenBatchItem& it = v.items[i];
     raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(
-        ray_tile(it, t - v.ray[i]), batch_ray_args(v, it, ws, false), it.cb, 0, it.images, it.images_grad,
+        ray_tile(it, t - v.ray[i]), batch_ray_args(v, it, ws, false), it.cb, 0, (float*)it.images,
+        (const float*)it.images_grad,
         reinterpret_cast<float*>(rec + it.rec_G), reinterpret_cast<RayRec*>(rec + it.rec_rays), it.kcap);
   }
 }
@@ -1451,7 +1458,7 @@ __global__ __launch_bounds__(256) void render_record_batch_kernel(const void* __
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.ray, v.h->n, t));
     const RenBatchItem& it = v.items[i];
     raymarch_fwd_record_kernel_body<MODE>(ray_tile(it, t - v.ray[i]), batch_ray_args(v, it, ws, true), it.cb,
-                                          it.images, 0, reinterpret_cast<float*>(rec + it.rec_G),
+                                          (float*)it.images, 0, reinterpret_cast<float*>(rec + it.rec_G),
                                           reinterpret_cast<RayRec*>(rec + it.rec_rays), it.kcap, t);
     __syncthreads();  // the loss partial's LDS is written again by the next tile
   }
@@ -1471,7 +1478,7 @@ __global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* __r
     raymarch_bwd_brick_kernel_body<1, BX, BY, BZ>(VBlock{t - table[i], 0, 0}, it.a, it.cb, 0,
                                                   reinterpret_cast<const RayRec*>(rec + it.rec_rays),
                                                   reinterpret_cast<const float*>(rec + it.rec_G), it.kcap,
-                                                  it.density_grad, 0, nullptr);
+                                                  (float*)it.density_grad, 0, nullptr);
     __syncthreads();  // the brick's LDS arrays are rewritten by the next brick
   }
 }
@@ -1490,7 +1497,7 @@ __global__ __launch_bounds__(256) void render_l2_finish_batch_kernel(const void*
   for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
   if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = sum;
   __syncthreads();
-  if (threadIdx.x == 0) *v.items[i].loss = static_cast<float>((s_w[0] + s_w[1]) + (s_w[2] + s_w[3]));
+  if (threadIdx.x == 0) *(float*)v.items[i].loss = static_cast<float>((s_w[0] + s_w[1]) + (s_w[2] + s_w[3]));
 }
 
 // Pixels of the N x V images of a fused render.
@@ -1987,8 +1994,10 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     it.nty = static_cast<int>(blocks_for(it.a.out_h, 16));
     NGT_REQUIRE(bricked_fits(it.a.din, 1), NGT_ERR_INVALID_ARGUMENT, "batch density grid too large");
     it.brick_tiles = static_cast<int>(brick_rows(it.a.bd));
-    it.images = q.images; it.images_grad = q.images_grad; it.density_grad = q.density_grad;
-    it.l2_target = q.l2_target; it.loss = q.loss;
+    it.density = (gptr<const float>)q.density;
+    it.images = (gptr<float>)q.images; it.images_grad = (gptr<const float>)q.images_grad;
+    it.density_grad = (gptr<float>)q.density_grad;
+    it.l2_target = (gptr<const float>)q.l2_target; it.loss = (gptr<float>)q.loss;
     it.ws_bricked = static_cast<int64_t>(ws);
     ws += align_up(static_cast<size_t>(it.a.bd.slice_cells) * sizeof(float));
     // records: chunk-relative offsets; a new chunk starts where the budget would be exceeded
