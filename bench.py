@@ -449,7 +449,9 @@ def batch_bench(steps, warmup):
     fused render with L2 loss, forward and backward.  Batched: one persistent
     launch per kernel over all 128 instances (eager and captured in a HIP
     graph); per-instance: the library's per-instance ops (advection, then
-    the fused render + L2 op) launched instance by instance."""
+    the fused render + L2 op) launched instance by instance; per-instance
+    plain kernels: the reference's composition (LUT-sampled MacCormack warp,
+    one raymarch per camera, scatter-form adjoints) instance by instance."""
     import torch
     from ngt import synth, batch as B, ops, _lib
     from ngt.renderer import Renderer
@@ -509,10 +511,40 @@ def batch_bench(steps, warmup):
             loss, _ = ops.render_density_fused_l2(adv, tgts[i][None], (1.0, 1.0, 1.0), ds, M, V, P, F, out_dhw)
             loss.backward()
 
+    def per_instance_plain():
+        # the reference's composition per instance on the same library's per-op
+        # entry points (LUT-sampled warp, one raymarch per camera, scatter-form
+        # adjoints), as the single-step bench's plain path
+        from ngt.data_structures import VelocityGrid
+        from ngt.renderer import Renderer
+        old = ops.use_gather_adjoint
+        ops.use_gather_adjoint = False
+        try:
+            for i in range(len(dens)):
+                M, V, P, F, ds, out_dhw = cams[i]
+                e = int(dens[i].shape[0])
+                r = Renderer(filter_mode="LINEAR", boundary_mode="BORDER", mipmapping="NONE",
+                             blend_mode="BEER_LAMBERT", fused=True)
+                d = dens[i][None].requires_grad_()
+                v = [c[None].requires_grad_() for c in vels[i]]
+                vg = VelocityGrid([e, e, e], x=v[0], y=v[1], z=v[2], warp_renderer=r)
+                adv = vg.warp(d, order=2, dt=1.0, clamp="MC_SMOOTH", fused=False)
+                data = torch.cat([adv, adv, adv, adv], dim=-1) * ds
+                imgs = [ops.raymarch_grid_transform(data, M, V[k:k + 1], P[k:k + 1], F[k:k + 1], out_dhw)
+                        for k in range(len(V))]
+                ((torch.cat(imgs, dim=1)[..., :3] - tgts[i][None]) ** 2).sum().backward()
+        finally:
+            ops.use_gather_adjoint = old
+
     print("batch: batched %.2f ms eager, %.2f ms graph" % (ms_eager, ms_graph), file=sys.stderr, flush=True)
     per_instance()
     torch.cuda.synchronize()
     ms_plain = timed(per_instance, max(1, steps // 4))
+    print("batch: per-instance fused ops %.2f ms" % ms_plain, file=sys.stderr, flush=True)
+    per_instance_plain()
+    torch.cuda.synchronize()
+    ms_plain_kernels = timed(per_instance_plain, 1)
+    print("batch: per-instance plain kernels %.2f ms" % ms_plain_kernels, file=sys.stderr, flush=True)
     best = min(ms_eager, ms_graph)
     tuner = B.DEFAULT_TUNER
     return {"workload": "uneven batch: 32 sequences x 4 frames (grid edge 48-160, 1-8 views, 128x72 / 256x144 / "
@@ -520,6 +552,8 @@ def batch_bench(steps, warmup):
             "instances": len(dens), "voxel_updates_per_step": cells, "ray_samples_per_step": ray_samples,
             "ms_per_step_batched_eager": ms_eager, "ms_per_step_batched_graph": ms_graph,
             "ms_per_step_per_instance": ms_plain, "speedup_vs_per_instance": ms_plain / best,
+            "ms_per_step_per_instance_plain_kernels": ms_plain_kernels,
+            "speedup_vs_per_instance_plain_kernels": ms_plain_kernels / best,
             "voxel_updates_per_s": cells / (best * 1e-3), "ray_samples_per_s": ray_samples / (best * 1e-3),
             "workspace_bytes": int(step.adv.workspace.numel() + step.ren.workspace.numel()),
             "render_record_chunks": step.ren.num_chunks, "render_record_bytes": step.ren.record_bytes,
