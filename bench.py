@@ -384,14 +384,21 @@ def cpu_baseline(reps=10, warmup=2):
 
 
 def _cpu_model():
+    """The first processor's /proc/cpuinfo model name, with its family /
+    model / stepping numbers (a virtualised host may report a short name)."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break
+                k, _, v = line.partition(":")
+                info[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    name = info.get("model name")
+    ids = ", ".join("%s %s" % (k, info[k]) for k in ("cpu family", "model", "stepping") if k in info)
+    return "%s (%s)" % (name, ids) if name and ids else name
 
 
 def chain_bench(steps, warmup):
