@@ -273,8 +273,7 @@ def op_calls(torch, prob):
     loss = torch.empty((), device=rho.device)
     loss_grad = torch.ones((), device=rho.device)
     g_adv = torch.empty_like(rho)
-    rws_b = lib.ngt_render_density_l2_bwd_workspace_size(s_rho, cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw, 1,
-                                                         0)
+    rws_b = lib.ngt_render_density_l2_record_size(s_rho, cams.m, cams.v, cams.p, cams.f, cams.nv, out_dhw)
     rws = torch.empty(max(rws_b, 1), dtype=torch.uint8, device=rho.device)
     fws_b = lib.ngt_render_density_l2_fwd_workspace_size(s_rho, 1, cams.nv, out_dhw)
     fws = torch.empty(max(fws_b, 1), dtype=torch.uint8, device=rho.device)

@@ -252,13 +252,15 @@ int ngt_bricked_grid(const float* grid, const int64_t* shape, int boundary, floa
  * instead of being read from memory.  `bricked` (nullable): the haloed brick
  * copy of density made by ngt_bricked_grid with the same boundary; when
  * given, neither call makes its own copy (the workspace sizes still assume
- * NULL).  `record` (nullable, LINEAR and BORDER only): a buffer of
- * ngt_render_density_l2_bwd_workspace_size bytes in which the forward also
+ * NULL).  `record` (nullable, LINEAR and BORDER only, needs `bricked`): a
+ * buffer of ngt_render_density_l2_record_size bytes in which the forward also
  * records each ray and its per-sample gradients for a unit loss gradient
  * (BEER_LAMBERT and ADDITIVE within the same walk, the ALPHA blends in a
  * second walk of each ray right after the first); _l2_bwd with
- * `recorded` = 1 and that buffer as its workspace then only gathers them
- * (scaled by *loss_grad). */
+ * `recorded` = 1, the same `bricked` and that buffer as its workspace then
+ * only gathers them (scaled by *loss_grad).  The record size is 0 when the
+ * cameras' rays are too long for a record (the callers then take the
+ * two-walk backward). */
 size_t ngt_render_density_l2_fwd_workspace_size(const int64_t* density_shape, int interpolation, int num_cameras,
                                                 const int32_t* output_dhw);
 int ngt_render_density_l2_fwd(const float* density, const int64_t* density_shape, const float* light_rgb,
@@ -283,6 +285,9 @@ int ngt_render_density_l2_fwd_noalloc(const float* density, const int64_t* densi
                                       const int32_t* output_dhw, int interpolation, int boundary, int blending_mode,
                                       const float* target, float loss_weight, const float* bricked, void* record,
                                       size_t record_bytes, float* output, float* loss, ngt_stream_t stream);
+size_t ngt_render_density_l2_record_size(const int64_t* density_shape, const float* matrix_m, const float* matrix_v,
+                                          const float* matrix_p, const float* frustum, int num_cameras,
+                                          const int32_t* output_dhw);
 size_t ngt_render_density_l2_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
                                                 const float* matrix_v, const float* matrix_p, const float* frustum,
                                                 int num_cameras, const int32_t* output_dhw, int interpolation,

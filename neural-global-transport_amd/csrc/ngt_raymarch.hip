@@ -1224,9 +1224,12 @@ size_t gather_workspace_bytes(const int64_t* shape, const float* M, const float*
          align_up(bricked_bytes(shape, grad_channels));
 }
 
+// with_bricked: the workspace also holds a haloed brick copy for the prepass
+// walk (not when the per-sample gradients are recorded by the forward, whose
+// gather reads no grid values).
 bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V, const float* P, const float* F,
                  const RayArgs& a, int interpolation, int boundary, int grad_channels, void* workspace,
-                 size_t workspace_bytes, GatherPlan& gp) {
+                 size_t workspace_bytes, GatherPlan& gp, bool with_bricked = true) {
   if (interpolation != LINEAR || boundary != BORDER || workspace == nullptr || !bricked_fits(a.din, grad_channels))
     return false;
   gp.kcap = ray_step_cap(pairs, M, V, P, F, a);
@@ -1235,11 +1238,12 @@ bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V,
   const size_t rays_bytes = align_up(pairs.size() * npix * sizeof(RayRec));
   const size_t g_bytes =
       align_up(pairs.size() * static_cast<size_t>(gp.kcap) * g_pixels(npix) * grad_channels * sizeof(float));
-  const size_t b_bytes = align_up(static_cast<size_t>(a.n_data) * a.bd.slice_cells * grad_channels * sizeof(float));
+  const size_t b_bytes =
+      with_bricked ? align_up(static_cast<size_t>(a.n_data) * a.bd.slice_cells * grad_channels * sizeof(float)) : 0;
   if (workspace_bytes < rays_bytes + g_bytes + b_bytes) return false;
   gp.rays = static_cast<RayRec*>(workspace);
   gp.G = reinterpret_cast<float*>(static_cast<char*>(workspace) + rays_bytes);
-  gp.bricked = reinterpret_cast<float*>(static_cast<char*>(workspace) + rays_bytes + g_bytes);
+  gp.bricked = with_bricked ? reinterpret_cast<float*>(static_cast<char*>(workspace) + rays_bytes + g_bytes) : nullptr;
   return true;
 }
 
@@ -1540,8 +1544,8 @@ int render_density_fwd_impl(const float* density, const int64_t* density_shape, 
     NGT_REQUIRE(l2_target != nullptr && bricked && interpolation == LINEAR && boundary == BORDER,
                 NGT_ERR_INVALID_ARGUMENT, "recording the per-sample gradients needs LINEAR, BORDER and the L2 loss");
     NGT_REQUIRE(plan_gather(pairs, matrix_m, matrix_v, matrix_p, frustum, a, interpolation, boundary, 1, record,
-                            record_bytes, gp),
-                NGT_ERR_WORKSPACE, "record buffer too small (ngt_render_density_l2_bwd_workspace_size)");
+                            record_bytes, gp, false),
+                NGT_ERR_WORKSPACE, "record buffer too small (ngt_render_density_l2_record_size)");
   }
   int64_t partial_base = 0;
   int pair_base = 0;
@@ -1866,6 +1870,21 @@ int ngt_render_density_l2_fwd_noalloc(const float* density, const int64_t* densi
                                    stream);
 }
 
+size_t ngt_render_density_l2_record_size(const int64_t* density_shape, const float* matrix_m, const float* matrix_v,
+                                          const float* matrix_p, const float* frustum, int num_cameras,
+                                          const int32_t* output_dhw) {
+  if (!density_shape || !matrix_m || !matrix_v || !matrix_p || !frustum || !output_dhw || num_cameras <= 0) return 0;
+  if (density_shape[4] != 1 || bricked_bytes(density_shape, 1) == 0) return 0;
+  std::vector<Pair> pairs;
+  if (make_pairs(static_cast<int>(density_shape[0]), num_cameras, true, pairs) != NGT_OK) return 0;
+  const RayArgs a = make_ray_args(nullptr, density_shape, output_dhw);
+  const int kcap = ray_step_cap(pairs, matrix_m, matrix_v, matrix_p, frustum, a);
+  if (kcap <= 0) return 0;
+  const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
+  return align_up(pairs.size() * npix * sizeof(RayRec)) +
+         align_up(pairs.size() * static_cast<size_t>(kcap) * g_pixels(npix) * sizeof(float));
+}
+
 size_t ngt_render_density_l2_bwd_workspace_size(const int64_t* density_shape, const float* matrix_m,
                                                 const float* matrix_v, const float* matrix_p, const float* frustum,
                                                 int num_cameras, const int32_t* output_dhw, int interpolation,
@@ -1894,8 +1913,13 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
   // gradient buffer and the scatter adjoint
   const int64_t npix = render_pixels(density_shape, num_cameras, output_dhw);
   const size_t img_need = static_cast<size_t>(npix) * 4 * sizeof(float);
-  const size_t g_need = ngt_render_density_bwd_workspace_size(density_shape, matrix_m, matrix_v, matrix_p, frustum,
-                                                              num_cameras, output_dhw, interpolation, boundary);
+  // a record made by the forward holds no brick copy when the caller passes one
+  const bool own_bricks = !(recorded && bricked != nullptr);
+  const size_t g_need =
+      own_bricks ? ngt_render_density_bwd_workspace_size(density_shape, matrix_m, matrix_v, matrix_p, frustum,
+                                                         num_cameras, output_dhw, interpolation, boundary)
+                 : ngt_render_density_l2_record_size(density_shape, matrix_m, matrix_v, matrix_p, frustum,
+                                                     num_cameras, output_dhw);
   const bool gather = g_need > 0 && workspace != nullptr && workspace_bytes >= g_need;
   NGT_REQUIRE(gather || (workspace != nullptr && workspace_bytes >= img_need), NGT_ERR_WORKSPACE,
               "workspace too small for ngt_render_density_l2_bwd");
@@ -1908,7 +1932,7 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
   hipStream_t s = as_stream(stream);
   GatherPlan gp;
   if (gather && plan_gather(pairs, matrix_m, matrix_v, matrix_p, frustum, a, interpolation, boundary, 1, workspace,
-                            workspace_bytes, gp)) {
+                            workspace_bytes, gp, own_bricks)) {
     run_gather<true>(pairs, matrix_m, matrix_v, matrix_p, frustum, a, blending_mode, 4, output, nullptr, gp,
                      density_grad, static_cast<int>(density_shape[0]), s, bricked, recorded != 0, loss_grad);
     return check_launch("ngt_render_density_l2_bwd");

@@ -65,6 +65,7 @@ _SIGNATURES = {
     "ngt_render_density_l2_fwd_noalloc": (_i, [_vp, _c_i64p, _c_fp, _f, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i,
                                                _i, _vp, _f, _vp, _vp, _sz, _vp, _vp, _vp]),
     "ngt_render_density_l2_bwd_workspace_size": (_sz, [_c_i64p, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p, _i, _i]),
+    "ngt_render_density_l2_record_size": (_sz, [_c_i64p, _c_fp, _c_fp, _c_fp, _c_fp, _i, _c_i32p]),
     "ngt_render_density_l2_bwd": (_i, [_vp, _c_i64p, _c_fp, _f, _vp, _vp, _f, _vp, _c_fp, _c_fp, _c_fp, _c_fp, _i,
                                        _c_i32p, _i, _i, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "ngt_advect_workspace_size": (_sz, [_c_i64p, _i, _i, _i, _i]),

@@ -540,8 +540,8 @@ class _RenderDensityL2(torch.autograd.Function):
         record, rbytes = None, 0
         if (ctx.needs_input_grad[0] and use_gather_adjoint and bricked is not None
                 and boundary == BOUNDARY["BORDER"]):
-            rb = lib.ngt_render_density_l2_bwd_workspace_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv,
-                                                              _lib.dhw(out_shape), interp, boundary)
+            rb = lib.ngt_render_density_l2_record_size(s_d, cams.m, cams.v, cams.p, cams.f, cams.nv,
+                                                       _lib.dhw(out_shape))
             if rb > 0:
                 record, rbytes = _lib.workspace(rb, density.device)
         _lib.check(lib.ngt_render_density_l2_fwd(

@@ -122,3 +122,25 @@ def test_step_cap_of_the_benchmark_views(lib):
     most = max_steps((128, 128, 128), M, V, P, F, (128, 144, 256))
     print("benchmark views: longest ray %d samples, bound %d" % (most, cap))
     assert 0 < most <= cap <= 1.5 * most + 8
+
+
+def test_record_size_of_the_benchmark_views(lib):
+    """The fused render + L2 forward's record (ngt_render_density_l2_record_size)
+    holds a 48-byte ray record per pixel and one float per recorded sample
+    (kcap per ray, pixels in groups of 8), each region 256-byte aligned; the
+    two-walk backward's workspace adds the haloed brick copy to it."""
+    from ngt import _lib, synth
+    from ngt.renderer import Renderer
+    grid_t, cams = synth.ring_setup((128, 128, 128), (144, 256), 128, 5)
+    M, V, P, F = Renderer()._transform_params([grid_t], cams)
+    shape = (ctypes.c_int64 * 5)(1, 128, 128, 128, 1)
+    m, v, p, fr = (_lib.host_floats(x, n) for x, n in ((M, 16), (V, 16), (P, 16), (F, 6)))
+    odhw = (ctypes.c_int32 * 3)(128, 144, 256)
+    record = lib.ngt_render_density_l2_record_size(shape, m[0], v[0], p[0], fr[0], 5, odhw)
+    two_walk = lib.ngt_render_density_l2_bwd_workspace_size(shape, m[0], v[0], p[0], fr[0], 5, odhw, 1, 0)
+    cap = step_cap(lib, (128, 128, 128), M, V, P, F, (128, 144, 256))
+    up = lambda n: (n + 255) // 256 * 256
+    npix = 144 * 256
+    assert record == up(5 * npix * 48) + up(5 * cap * ((npix + 7) // 8 * 8) * 4)
+    assert two_walk == record + up(lib.ngt_bricked_grid_bytes(shape))
+    print("record %.1f MB, two-walk workspace %.1f MB" % (record / 1e6, two_walk / 1e6))

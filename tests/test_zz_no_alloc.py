@@ -46,7 +46,7 @@ def _call(env, rho, tgt, cams, ds, out_dhw, noalloc, use_brick, use_record):
         _lib.check(lib.ngt_bricked_grid(_lib.ptr(rho), s_d, 0, _lib.ptr(brk), st))
     rec, rbytes = None, 0
     if use_record:
-        rbytes = lib.ngt_render_density_l2_bwd_workspace_size(s_d, c.m, c.v, c.p, c.f, c.nv, od, 1, 0)
+        rbytes = lib.ngt_render_density_l2_record_size(s_d, c.m, c.v, c.p, c.f, c.nv, od)
         rec = torch.zeros(rbytes, dtype=torch.uint8, device="cuda")
     bp = None if brk is None else _lib.ptr(brk)
     rp = None if rec is None else _lib.ptr(rec)
