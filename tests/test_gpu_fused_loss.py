@@ -231,7 +231,7 @@ def test_recorded_rays_are_never_cut(ops):
 
 @pytest.mark.parametrize("path", ["record", "prepass", "scatter"])
 def test_gradient_behind_opaque_media(ops, path):
-    """A dense volume (per-sample optical depth about 2, so the far half of
+    """A dense volume (per-sample optical depth about 1.4, so the far half of
     every ray is hidden): the exact BEER_LAMBERT adjoint forms each sample's
     suffix sum as a difference of FP32 prefix sums, which loses relative
     accuracy where the gradient is tiny.  Every path (the recording forward,
