@@ -1030,7 +1030,13 @@ __global__ __launch_bounds__(256) void advect_fill_batch_kernel(const void* __re
     const AdvBatchItem& it = v.items[i];
     float* dst = v.h->order == 2 ? ws + it.ws_ovf : (float*)it.g_in;
     const int64_t b = static_cast<int64_t>(t - v.fill[i]) * kFillChunk;
-    for (int64_t e = b + threadIdx.x; e < min(b + kFillChunk, it.cells); e += blockDim.x) dst[e] = 0.f;
+    const int64_t e1 = min(b + kFillChunk, it.cells);
+    if ((reinterpret_cast<uintptr_t>(dst) & 15) == 0 && e1 - b == kFillChunk) {
+      static_assert(kFillChunk == 4 * 256, "one 16-byte store per work-item of a 256-wide work-group");
+      reinterpret_cast<float4*>(dst + b)[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      for (int64_t e = b + threadIdx.x; e < e1; e += blockDim.x) dst[e] = 0.f;
+    }
   }
 }
 

@@ -139,9 +139,28 @@ CamParams make_cam(const float* m16, const float* v16, const float* p16, const f
   return cp;
 }
 
+// 16-byte stores where the range starts 16-byte aligned (workspace regions
+// and torch allocations do), else 4-byte stores.
+__device__ __forceinline__ void fill_range(float* __restrict__ p, int64_t n, float value, int64_t tid,
+                                           int64_t stride) {
+  if ((reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+    const int64_t n4 = n >> 2;
+    float4* __restrict__ q = reinterpret_cast<float4*>(p);
+    const float4 v = make_float4(value, value, value, value);
+    for (int64_t i = tid; i < n4; i += stride) q[i] = v;
+    for (int64_t i = (n4 << 2) + tid; i < n; i += stride) p[i] = value;
+  } else {
+    for (int64_t i = tid; i < n; i += stride) p[i] = value;
+  }
+}
+
+inline unsigned fill_blocks(int64_t n) {
+  return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((n + 1023) / 1024, 4096)));
+}
+
 __global__ void fill_kernel(float* __restrict__ p, int64_t n, float value) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = value;
+  fill_range(p, n, value, static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x,
+             static_cast<int64_t>(gridDim.x) * blockDim.x);
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t value) {
@@ -151,23 +170,19 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t va
 
 void fill_zero(float* p, int64_t n, hipStream_t s) {
   if (n <= 0) return;
-  const unsigned b = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 4096));
-  hipLaunchKernelGGL(fill_kernel, dim3(b), dim3(256), 0, s, p, n, 0.f);
+  hipLaunchKernelGGL(fill_kernel, dim3(fill_blocks(n)), dim3(256), 0, s, p, n, 0.f);
 }
 
 // Two ranges in one launch: grid.y selects the range.
 __global__ void fill2_kernel(float* __restrict__ a, int64_t na, float* __restrict__ b, int64_t nb) {
-  float* p = blockIdx.y == 0 ? a : b;
-  const int64_t n = blockIdx.y == 0 ? na : nb;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.f;
+  fill_range(blockIdx.y == 0 ? a : b, blockIdx.y == 0 ? na : nb, 0.f,
+             static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x, static_cast<int64_t>(gridDim.x) * blockDim.x);
 }
 
 void fill_zero2(float* a, int64_t na, float* b, int64_t nb, hipStream_t s) {
   const int64_t n = std::max(na, nb);
   if (n <= 0) return;
-  const unsigned g = static_cast<unsigned>(std::min<int64_t>((n + 255) / 256, 4096));
-  hipLaunchKernelGGL(fill2_kernel, dim3(g, 2), dim3(256), 0, s, a, na, b, nb);
+  hipLaunchKernelGGL(fill2_kernel, dim3(fill_blocks(n), 2), dim3(256), 0, s, a, na, b, nb);
 }
 
 void fill_zero_u32(uint32_t* p, int64_t n, hipStream_t s) {
