@@ -1,7 +1,7 @@
 // RaymarchGridTransform (+Grad) and the fused density renderer on gfx950.
 //
-// One work-item per pixel; a 16x16 work-group is remapped so each wavefront
-// owns an 8x8 pixel tile, which keeps the 64 rays of a wavefront spatially
+// One work-item per pixel; each wavefront owns an 8x8 pixel tile (one
+// 64-wide work-group), which keeps the 64 rays of a wavefront spatially
 // coherent and their 8-tap gathers inside a few cache lines per step.  All
 // (batch, camera) pairs of a call run in one launch (grid.z), with camera
 // matrices passed by value as kernel arguments instead of __constant__
@@ -117,12 +117,28 @@ NGT_DEVICE Ray make_ray(const RayArgs& a, const CamParams& cp, int x, int y) {
   return r;
 }
 
-// Work-group 16x16 -> four 8x8 wavefront tiles.
+// The per-pixel ray kernels run one wavefront per 8x8 pixel tile: rays of
+// very different lengths (the image's edges miss the grid) balance over the
+// CUs in finer units than 16x16 work-groups (the 128^3 / 5-view render
+// forward 52 -> 43 us, the recording forward 60 -> 58 us).
+constexpr int kRayBlock = 64, kRayTile = 8;
+
+// Pixel of a work-item: a 64-wide work-group covers one 8x8 tile; a
+// 256-wide one covers a 16x16 tile as four 8x8 wavefront tiles.
 NGT_DEVICE void pixel_of_thread(const VBlock& vb, int& x, int& y) {
-  const int tid = threadIdx.x;  // 0..255
+  const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  x = vb.x * 16 + (wave & 1) * 8 + (lane & 7);
-  y = vb.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+  if (blockDim.x == 64) {
+    x = vb.x * 8 + (lane & 7);
+    y = vb.y * 8 + (lane >> 3);
+  } else {
+    x = vb.x * 16 + (wave & 1) * 8 + (lane & 7);
+    y = vb.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+  }
+}
+
+inline dim3 ray_grid_dims(int out_w, int out_h, int count) {
+  return dim3(blocks_for(out_w, kRayTile), blocks_for(out_h, kRayTile), count);
 }
 
 // Cell value read by a sample: the C-channel input (LIGHT = false) or the
@@ -255,7 +271,7 @@ NGT_DEVICE void store_l2_partial(const RayArgs& a, float l2, int64_t slot) {
   for (int o = 32; o > 0; o >>= 1) l2 += __shfl_down(l2, o, 64);
   if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = l2;
   __syncthreads();
-  if (threadIdx.x == 0) a.l2_partial[slot] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+  if (threadIdx.x == 0) a.l2_partial[slot] = blockDim.x == 64 ? s_w[0] : (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
 }
 
 template <int C, int FM, int BM, int MODE, bool LIGHT, bool BRICKED>
@@ -1273,17 +1289,17 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
       const Pair& p = pairs[b + i];
       cb.cam[i] = make_cam(M + 16 * p.n, V + 16 * p.v, P + 16 * p.v, F + 6 * p.v, p.n, p.out);
     }
-    const dim3 rgrid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    const dim3 rgrid = ray_grid_dims(a.out_w, a.out_h, cb.count);
     const int base = static_cast<int>(b);
     if (recorded) {
       // per-sample gradients recorded by the forward
     } else if constexpr (LIGHT) {
       NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<4, BORDER, MM, true, true>), rgrid, dim3(256), 0, s, a, cb,
+          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<4, BORDER, MM, true, true>), rgrid, dim3(kRayBlock), 0, s, a, cb,
                              base, output, output_grad, gp.G, gp.rays, gp.kcap));
     } else {
       NGT_SWITCH_C(C, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false, true>), rgrid, dim3(256), 0, s, a, cb,
+          hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false, true>), rgrid, dim3(kRayBlock), 0, s, a, cb,
                              base, output, output_grad, gp.G, gp.rays, gp.kcap)));
     }
     // bricks: larger ones visit fewer samples twice (the support box is one
@@ -1358,7 +1374,7 @@ struct RenBatchHeader {
 struct RenBatchItem {
   RayArgs a;  // its density pointer is read from `density` below
   CamBatch cb;
-  int ntx, nty;        // 16 x 16 pixel tiles per view
+  int ntx, nty;        // 8 x 8 pixel tiles per view (one wavefront each)
   int kcap;
   int brick_tiles;     // brick rows of the haloed copy (one work-group each)
   gptr<const float> density;
@@ -1550,22 +1566,22 @@ int render_density_fwd_impl(const float* density, const int64_t* density_shape, 
   int64_t partial_base = 0;
   int pair_base = 0;
   for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
-    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    const dim3 grid = ray_grid_dims(a.out_w, a.out_h, cb.count);
     a.l2_partial = l2_partial ? l2_partial + partial_base : nullptr;
     partial_base += static_cast<int64_t>(grid.x) * grid.y * grid.z;
     const int base = pair_base;
     pair_base += cb.count;
     if (record) {
       NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_fwd_record_kernel<MM>), grid, dim3(256), 0, s, a, cb, output, base, gp.G,
+          hipLaunchKernelGGL((raymarch_fwd_record_kernel<MM>), grid, dim3(kRayBlock), 0, s, a, cb, output, base, gp.G,
                              gp.rays, gp.kcap));
     } else if (bricked) {
       NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_fwd_kernel<4, LINEAR, BB, MM, true, true>), grid, dim3(256), 0, s, a, cb,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<4, LINEAR, BB, MM, true, true>), grid, dim3(kRayBlock), 0, s, a, cb,
                              output)));
     } else {
       NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_fwd_kernel<4, FF, BB, MM, true, false>), grid, dim3(256), 0, s, a, cb,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<4, FF, BB, MM, true, false>), grid, dim3(kRayBlock), 0, s, a, cb,
                              output))));
     }
   });
@@ -1643,14 +1659,14 @@ int ngt_raymarch_grid_transform_fwd(const float* input, const int64_t* input_sha
     a.bricked = static_cast<const float*>(workspace);
   }
   for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
-    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    const dim3 grid = ray_grid_dims(a.out_w, a.out_h, cb.count);
     if (bricked) {
       NGT_SWITCH_C(C, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_fwd_kernel<CC, LINEAR, BB, MM, false, true>), grid, dim3(256), 0, s, a, cb,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<CC, LINEAR, BB, MM, false, true>), grid, dim3(kRayBlock), 0, s, a, cb,
                              output))));
     } else {
       NGT_SWITCH_C(C, NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_fwd_kernel<CC, FF, BB, MM, false, false>), grid, dim3(256), 0, s, a, cb,
+          hipLaunchKernelGGL((raymarch_fwd_kernel<CC, FF, BB, MM, false, false>), grid, dim3(kRayBlock), 0, s, a, cb,
                              output)))));
     }
   });
@@ -1709,9 +1725,9 @@ int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_sha
       CamBatch cb;
       cb.count = 1;
       cb.cam[0] = make_cam(matrix_m + 16 * p.n, matrix_v + 16 * p.v, matrix_p + 16 * p.v, frustum + 6 * p.v, p.n, p.out);
-      const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), 1);
+      const dim3 grid = ray_grid_dims(a.out_w, a.out_h, 1);
       NGT_SWITCH_C(C, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_bwd_reference_kernel<CC, BB, MM>), grid, dim3(256), 0, s, a, cb, output,
+          hipLaunchKernelGGL((raymarch_bwd_reference_kernel<CC, BB, MM>), grid, dim3(kRayBlock), 0, s, a, cb, output,
                              output_grad, gbuf, wsum))));
       NGT_SWITCH_C(C,
           hipLaunchKernelGGL((normalize_by_weight_kernel<CC>), dim3(blocks_for(si.cells(), 256)), dim3(256), 0, s, gbuf,
@@ -1727,9 +1743,9 @@ int ngt_raymarch_grid_transform_bwd(const float* input, const int64_t* input_sha
   } else {
     fill_zero(input_grad, si.elems(), s);
     for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
-      const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+      const dim3 grid = ray_grid_dims(a.out_w, a.out_h, cb.count);
       NGT_SWITCH_C(C, NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_bwd_exact_kernel<CC, FF, BB, MM, false>), grid, dim3(256), 0, s, a, cb, output,
+          hipLaunchKernelGGL((raymarch_bwd_exact_kernel<CC, FF, BB, MM, false>), grid, dim3(kRayBlock), 0, s, a, cb, output,
                              output_grad, input_grad)))));
     });
   }
@@ -1781,9 +1797,9 @@ int ngt_render_density_bwd(const float* density, const int64_t* density_shape, c
   } else {
     fill_zero(density_grad, shape5(density_shape).elems(), s);
     for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
-      const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+      const dim3 grid = ray_grid_dims(a.out_w, a.out_h, cb.count);
       NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-          hipLaunchKernelGGL((raymarch_bwd_exact_kernel<4, FF, BB, MM, true>), grid, dim3(256), 0, s, a, cb, output,
+          hipLaunchKernelGGL((raymarch_bwd_exact_kernel<4, FF, BB, MM, true>), grid, dim3(kRayBlock), 0, s, a, cb, output,
                              output_grad, density_grad))));
     });
   }
@@ -1809,9 +1825,10 @@ int ngt_bricked_grid(const float* grid, const int64_t* shape, int boundary, floa
   return check_launch("ngt_bricked_grid");
 }
 
-// partial sums of the fused L2 loss: one per forward work-group (16 x 16 pixels)
+// partial sums of the fused L2 loss: one per forward work-group (an 8x8 pixel tile)
 static int64_t l2_partials(const int64_t* density_shape, int num_cameras, const int32_t* output_dhw) {
-  return density_shape[0] * num_cameras * static_cast<int64_t>((output_dhw[2] + 15) / 16) * ((output_dhw[1] + 15) / 16);
+  return density_shape[0] * num_cameras * static_cast<int64_t>(blocks_for(output_dhw[2], kRayTile)) *
+         blocks_for(output_dhw[1], kRayTile);
 }
 
 size_t ngt_render_density_l2_fwd_workspace_size(const int64_t* density_shape, int interpolation, int num_cameras,
@@ -1944,9 +1961,9 @@ int ngt_render_density_l2_bwd(const float* density, const int64_t* density_shape
   a.l2_target = nullptr;
   fill_zero(density_grad, shape5(density_shape).elems(), s);
   for_cam_batches(pairs, matrix_m, matrix_v, matrix_p, frustum, [&](const CamBatch& cb) {
-    const dim3 grid(blocks_for(a.out_w, 16), blocks_for(a.out_h, 16), cb.count);
+    const dim3 grid = ray_grid_dims(a.out_w, a.out_h, cb.count);
     NGT_SWITCH_FM2(interpolation, NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
-        hipLaunchKernelGGL((raymarch_bwd_exact_kernel<4, FF, BB, MM, true>), grid, dim3(256), 0, s, a, cb, output,
+        hipLaunchKernelGGL((raymarch_bwd_exact_kernel<4, FF, BB, MM, true>), grid, dim3(kRayBlock), 0, s, a, cb, output,
                            dimg, density_grad))));
   });
   return check_launch("ngt_render_density_l2_bwd");
@@ -2014,8 +2031,8 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     it.kcap = ray_step_cap(pairs, q.matrix_m, q.matrix_v, q.matrix_p, q.frustum, it.a);
     NGT_REQUIRE(it.kcap > 0, NGT_ERR_INVALID_ARGUMENT,
                 "batch render: rays too long for the gather adjoint's per-sample record");
-    it.ntx = static_cast<int>(blocks_for(it.a.out_w, 16));
-    it.nty = static_cast<int>(blocks_for(it.a.out_h, 16));
+    it.ntx = static_cast<int>(blocks_for(it.a.out_w, kRayTile));
+    it.nty = static_cast<int>(blocks_for(it.a.out_h, kRayTile));
     NGT_REQUIRE(bricked_fits(it.a.din, 1), NGT_ERR_INVALID_ARGUMENT, "batch density grid too large");
     it.brick_tiles = static_cast<int>(brick_rows(it.a.bd));
     it.density = (gptr<const float>)q.density;
@@ -2116,7 +2133,8 @@ int ngt_render_batch_fwd(const void* plan_host, const void* plan_device, void* w
   hipLaunchKernelGGL(render_brickify_batch_kernel, dim3(persistent_grid(h.tiles_brickify, blocks_per_cu)),
                      dim3(256), 0, s, plan_device, ws);
   const dim3 g(persistent_grid(h.tiles_ray, blocks_per_cu));
-  NGT_SWITCH_MODE(h.mode, hipLaunchKernelGGL((render_fwd_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws));
+  NGT_SWITCH_MODE(h.mode,
+      hipLaunchKernelGGL((render_fwd_batch_kernel<MM>), g, dim3(kRayBlock), 0, s, plan_device, ws));
   return check_launch("ngt_render_batch_fwd");
 }
 
@@ -2138,7 +2156,7 @@ int ngt_render_batch_bwd(const void* plan_host, const void* plan_device, void* w
     const int t0 = ray[chunk[c]], t1 = ray[chunk[c + 1]];
     const dim3 g(persistent_grid(t1 - t0, blocks_per_cu));
     NGT_SWITCH_MODE(h.mode,
-        hipLaunchKernelGGL((render_prepass_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws, t0, t1));
+        hipLaunchKernelGGL((render_prepass_batch_kernel<MM>), g, dim3(kRayBlock), 0, s, plan_device, ws, t0, t1));
     launch_batch_gather(h, plan_host, plan_device, ws, c, blocks_per_cu, gather_brick, s);
   }
   return check_launch("ngt_render_batch_bwd");
@@ -2164,7 +2182,7 @@ int ngt_render_batch_l2(const void* plan_host, const void* plan_device, void* wo
     const int t0 = ray[chunk[c]], t1 = ray[chunk[c + 1]];
     const dim3 g(persistent_grid(t1 - t0, blocks_per_cu));
     NGT_SWITCH_MODE(h.mode,
-        hipLaunchKernelGGL((render_record_batch_kernel<MM>), g, dim3(256), 0, s, plan_device, ws, t0, t1));
+        hipLaunchKernelGGL((render_record_batch_kernel<MM>), g, dim3(kRayBlock), 0, s, plan_device, ws, t0, t1));
     launch_batch_gather(h, plan_host, plan_device, ws, c, blocks_per_cu, gather_brick, s);
   }
   hipLaunchKernelGGL(render_l2_finish_batch_kernel, dim3(h.n), dim3(256), 0, s, plan_device, ws);
