@@ -446,9 +446,30 @@ struct alignas(16) RayRec {
   float sx, sy, sz, dx, dy, dz;
   float g_scale, g_off;
   int steps;    // samples recorded (at most the plan's kcap)
-  int pad[3];   // pad[0]: the ray's own sample count (steps < pad[0] would mean a cut ray);
-                // pad[1]: float bits of a bound on |g_scale * (G - g_off)| over the ray's samples
+  int pad[3];   // pad[0]: the ray's own sample count (steps < pad[0] would mean a cut ray)
 };
+// Beside the records, one float per 8x8 pixel tile of each pair: a bound on
+// |g_scale * (G - g_off)| over the samples of the tile's rays, from which the
+// gather takes its fixed-point unit (a brick reads the bounds of the tiles
+// its pixel rectangle touches, 1/64 of the loads of a per-ray bound).
+
+inline __host__ __device__ int bound_tiles(int out_w, int out_h) {
+  return ((out_w + kRayTile - 1) / kRayTile) * ((out_h + kRayTile - 1) / kRayTile);
+}
+
+// The wavefront's 8x8 tile bound: the largest of its lanes' bounds (NaN
+// propagates), stored by lane 0.  Every lane of the wavefront calls it.
+NGT_DEVICE void store_tile_bound(const RayArgs& a, float* __restrict__ bound, int pair, int x, int y, float b) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const float t = __shfl_xor(b, o, 64);
+    b = (t > b || t != t) ? t : b;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    const int ntx = (a.out_w + kRayTile - 1) / kRayTile;
+    bound[static_cast<int64_t>(pair) * bound_tiles(a.out_w, a.out_h) + (y / kRayTile) * ntx + x / kRayTile] = b;
+  }
+}
 
 // Image gradient of the fused L2 loss at one pixel (img: the pixel's C
 // channels, tpix: its first target value).
@@ -463,14 +484,10 @@ NGT_DEVICE Vec<C> l2_image_grad(const RayArgs& a, const float* __restrict__ img,
 }
 
 template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
-NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
-                                                                   const float* __restrict__ out,
-                                                                   const float* __restrict__ out_grad,
-                                                                   float* __restrict__ G, RayRec* __restrict__ rays,
-                                                                   int kcap) {
-  int x, y;
-  pixel_of_thread(vb, x, y);
-  if (x >= a.out_w || y >= a.out_h) return;
+NGT_DEVICE float raymarch_bwd_prepass_pixel(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
+                                            int x, int y, const float* __restrict__ out,
+                                            const float* __restrict__ out_grad, float* __restrict__ G,
+                                            RayRec* __restrict__ rays, int kcap) {
   const CamParams& cp = cb.cam[vb.z];
   constexpr int CIN = LIGHT ? 1 : C;
   constexpr int CG = GradChannels<C, LIGHT>::value;
@@ -488,7 +505,7 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
   rec.g_scale = 1.f;
   rec.g_off = 0.f;
   rays[pair * npix + pixel] = rec;
-  if (r.steps <= 0) return;
+  if (r.steps <= 0) return 0.f;
   const int64_t pix = ((static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x) * C;
   struct {
     const RayArgs& a;
@@ -505,8 +522,21 @@ NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs
   } sink{a, G + static_cast<int64_t>(pair) * kcap * g_pixels(npix) * CG, pixel, kcap, 0.f};
   const Vec<C> g_img = a.l2_target ? l2_image_grad<C>(a, out + pix, pix / C * 3) : vload<C>(out_grad + pix);
   walk_ray_gradient<C, LINEAR, BM, MODE, LIGHT, BRICKED>(a, src, r, g_img, out + pix, sink);
-  // the gather's fixed-point unit comes from this bound
-  rays[pair * npix + pixel].pad[1] = __float_as_int(sink.gmax);
+  return sink.gmax;  // the gather's fixed-point unit comes from this bound
+}
+
+template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
+NGT_DEVICE void raymarch_bwd_prepass_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
+                                                 const float* __restrict__ out, const float* __restrict__ out_grad,
+                                                 float* __restrict__ G, RayRec* __restrict__ rays,
+                                                 float* __restrict__ bound, int kcap) {
+  int x, y;
+  pixel_of_thread(vb, x, y);
+  float b = 0.f;
+  if (x < a.out_w && y < a.out_h)
+    b = raymarch_bwd_prepass_pixel<C, BM, MODE, LIGHT, BRICKED>(vb, a, cb, pair_base, x, y, out, out_grad, G, rays,
+                                                                 kcap);
+  store_tile_bound(a, bound, pair_base + static_cast<int>(vb.z), x, y, b);
 }
 
 template <int C, int BM, int MODE, bool LIGHT, bool BRICKED>
@@ -514,8 +544,8 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
                                                                    const float* __restrict__ out,
                                                                    const float* __restrict__ out_grad,
                                                                    float* __restrict__ G, RayRec* __restrict__ rays,
-                                                                   int kcap) {
-  raymarch_bwd_prepass_kernel_body<C, BM, MODE, LIGHT, BRICKED>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, out, out_grad, G, rays, kcap);
+                                                                   float* __restrict__ bound, int kcap) {
+  raymarch_bwd_prepass_kernel_body<C, BM, MODE, LIGHT, BRICKED>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, out, out_grad, G, rays, bound, kcap);
 }
 
 // The fused render + L2 forward that also records what the gather adjoint
@@ -537,7 +567,7 @@ __global__ __launch_bounds__(256) void raymarch_bwd_prepass_kernel(RayArgs a, Ca
 template <int MODE>
 NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, const VBlock& vb, int x, int y,
                                           float* __restrict__ out, float& l2, int pair_base, float* __restrict__ G,
-                                          RayRec* __restrict__ rays, int kcap) {
+                                          RayRec* __restrict__ rays, float& bnd, int kcap) {
   static_assert(MODE == BEER_LAMBERT || MODE == ADDITIVE, "one-walk recording: BEER_LAMBERT or ADDITIVE");
   const CamParams& cp = cb.cam[vb.z];
   const float* src = ray_source<1, true>(a, cp.data_index);
@@ -592,33 +622,38 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   rec.pad[0] = r.steps;
   rec.g_scale = a.depth_step * 2.f * a.l2_scale * (a.light_r * d0 + a.light_g * d1 + a.light_b * d2);
   rec.g_off = MODE == BEER_LAMBERT ? q : 0.f;
-  // bound on |g_scale (h_k - g_off)|: the gather's fixed-point unit comes from it
-  rec.pad[1] = __float_as_int(fabsf(rec.g_scale) * (hmax + fabsf(rec.g_off)));
   rays[static_cast<int64_t>(pair) * npix + pixel] = rec;
+  // bound on |g_scale (h_k - g_off)|: the gather's fixed-point unit comes from it
+  bnd = fabsf(rec.g_scale) * (hmax + fabsf(rec.g_off));
 }
 
 template <int MODE>
 NGT_DEVICE void raymarch_fwd_record_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb,
                                                 float* __restrict__ out, int pair_base, float* __restrict__ G,
-                                                RayRec* __restrict__ rays, int kcap, int64_t slot) {
+                                                RayRec* __restrict__ rays, float* __restrict__ bound, int kcap,
+                                                int64_t slot) {
   if constexpr (MODE == BEER_LAMBERT || MODE == ADDITIVE) {
     int x, y;
     pixel_of_thread(vb, x, y);
     float l2 = 0.f;
-    if (x < a.out_w && y < a.out_h) raymarch_fwd_record_pixel<MODE>(a, cb, vb, x, y, out, l2, pair_base, G, rays, kcap);
+    float b = 0.f;
+    if (x < a.out_w && y < a.out_h) raymarch_fwd_record_pixel<MODE>(a, cb, vb, x, y, out, l2, pair_base, G, rays, b, kcap);
+    store_tile_bound(a, bound, pair_base + vb.z, x, y, b);
     store_l2_partial(a, l2, slot);
   } else {  // ALPHA blends: the image first, then the adjoint walk of each ray
     raymarch_fwd_kernel_body<4, LINEAR, BORDER, MODE, true, true>(vb, a, cb, out, slot);
-    raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(vb, a, cb, pair_base, out, nullptr, G, rays, kcap);
+    raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(vb, a, cb, pair_base, out, nullptr, G, rays, bound,
+                                                                   kcap);
   }
 }
 
 template <int MODE>
 __global__ __launch_bounds__(256) void raymarch_fwd_record_kernel(RayArgs a, CamBatch cb, float* __restrict__ out,
                                                                   int pair_base, float* __restrict__ G,
-                                                                  RayRec* __restrict__ rays, int kcap) {
+                                                                  RayRec* __restrict__ rays, float* __restrict__ bound,
+                                                                  int kcap) {
   const VBlock vb{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)};
-  raymarch_fwd_record_kernel_body<MODE>(vb, a, cb, out, pair_base, G, rays, kcap, grid_slot(vb));
+  raymarch_fwd_record_kernel_body<MODE>(vb, a, cb, out, pair_base, G, rays, bound, kcap, grid_slot(vb));
 }
 
 // Inclusive scan of one int per work-item over a 256-wide work-group: a
@@ -652,7 +687,7 @@ NGT_DEVICE int block_inclusive_scan(int v, int* s_wtot) {
 // The sums are kept in LDS as 64-bit fixed point and added with integer LDS
 // atomics (ds_add_u64, at the LDS's full rate; float LDS atomics run some
 // 30x slower here).  The brick's unit is 2^-S with S chosen from the largest
-// per-ray gradient bound among its rays (RayRec::pad[1], written by the walk
+// per-ray gradient bound among its rays (the bound array, written by the walk
 // that recorded the gradients), so that every corner contribution rounds to
 // an integer below 2^30 in magnitude: the rounding error of a contribution is
 // at most 2^-30 of the brick's largest sample gradient, and integer sums are
@@ -662,6 +697,7 @@ NGT_DEVICE int block_inclusive_scan(int v, int* s_wtot) {
 template <int CG, int BX, int BY, int BZ>
 NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& a, const CamBatch& cb, int pair_base,
                                                                  const RayRec* __restrict__ rays,
+                                                                 const float* __restrict__ bound,
                                                                  const float* __restrict__ G, int kcap,
                                                                  float* __restrict__ in_grad, int accumulate,
                                                                  const float* __restrict__ grad_scale) {
@@ -758,11 +794,17 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   };
   {
     float m = 0.f;
-    for (int r = tid; r < total; r += 256) {
-      int px, py;
-      const int ci = ray_index(r, px, py);
-      const float bnd = __int_as_float(rays[(pair_base + ci) * static_cast<int>(npix) + py * a.out_w + px].pad[1]);
-      m = (bnd > m || bnd != bnd) ? bnd : m;  // NaN propagates
+    const int ntx = (a.out_w + kRayTile - 1) / kRayTile, per_pair = bound_tiles(a.out_w, a.out_h);
+    for (int ci = 0; ci < cb.count; ++ci) {
+      if (rect[ci][2] == 0) continue;
+      const int tx0 = rect[ci][0] / kRayTile, tx1 = (rect[ci][0] + rect[ci][2] - 1) / kRayTile;
+      const int ty0 = rect[ci][1] / kRayTile, ty1 = (rect[ci][1] + rect[ci][3] - 1) / kRayTile;
+      const int ntiles = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+      for (int t = tid; t < ntiles; t += 256) {
+        const int tx = tx0 + t % (tx1 - tx0 + 1), ty = ty0 + t / (tx1 - tx0 + 1);
+        const float bnd = bound[static_cast<int64_t>(pair_base + ci) * per_pair + ty * ntx + tx];
+        m = (bnd > m || bnd != bnd) ? bnd : m;  // NaN propagates
+      }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -922,10 +964,11 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
 template <int CG, int BX, int BY, int BZ>
 __global__ __launch_bounds__(256) void raymarch_bwd_brick_kernel(RayArgs a, CamBatch cb, int pair_base,
                                                                  const RayRec* __restrict__ rays,
+                                                                 const float* __restrict__ bound,
                                                                  const float* __restrict__ G, int kcap,
                                                                  float* __restrict__ in_grad, int accumulate,
                                                                  const float* __restrict__ grad_scale) {
-  raymarch_bwd_brick_kernel_body<CG, BX, BY, BZ>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, rays, G, kcap, in_grad, accumulate, grad_scale);
+  raymarch_bwd_brick_kernel_body<CG, BX, BY, BZ>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, cb, pair_base, rays, bound, G, kcap, in_grad, accumulate, grad_scale);
 }
 
 // Reference gradient of one camera: reverse walk from `end` with the
@@ -1216,6 +1259,7 @@ int ray_step_cap(const std::vector<Pair>& pairs, const float* M, const float* V,
 struct GatherPlan {
   int kcap;
   RayRec* rays;
+  float* bound;  // one per ray, beside the records
   float* G;
   float* bricked;  // bricked copy of the sampled grid (LINEAR: always, it follows G)
 };
@@ -1236,6 +1280,7 @@ size_t gather_workspace_bytes(const int64_t* shape, const float* M, const float*
   if (kcap <= 0) return 0;
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   return align_up(pairs.size() * npix * sizeof(RayRec)) +
+         align_up(pairs.size() * bound_tiles(a.out_w, a.out_h) * sizeof(float)) +
          align_up(pairs.size() * static_cast<size_t>(kcap) * g_pixels(npix) * grad_channels * sizeof(float)) +
          align_up(bricked_bytes(shape, grad_channels));
 }
@@ -1252,14 +1297,17 @@ bool plan_gather(const std::vector<Pair>& pairs, const float* M, const float* V,
   if (gp.kcap <= 0) return false;
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   const size_t rays_bytes = align_up(pairs.size() * npix * sizeof(RayRec));
+  const size_t bound_bytes = align_up(pairs.size() * bound_tiles(a.out_w, a.out_h) * sizeof(float));
   const size_t g_bytes =
       align_up(pairs.size() * static_cast<size_t>(gp.kcap) * g_pixels(npix) * grad_channels * sizeof(float));
   const size_t b_bytes =
       with_bricked ? align_up(static_cast<size_t>(a.n_data) * a.bd.slice_cells * grad_channels * sizeof(float)) : 0;
-  if (workspace_bytes < rays_bytes + g_bytes + b_bytes) return false;
-  gp.rays = static_cast<RayRec*>(workspace);
-  gp.G = reinterpret_cast<float*>(static_cast<char*>(workspace) + rays_bytes);
-  gp.bricked = with_bricked ? reinterpret_cast<float*>(static_cast<char*>(workspace) + rays_bytes + g_bytes) : nullptr;
+  if (workspace_bytes < rays_bytes + bound_bytes + g_bytes + b_bytes) return false;
+  char* w = static_cast<char*>(workspace);
+  gp.rays = reinterpret_cast<RayRec*>(w);
+  gp.bound = reinterpret_cast<float*>(w + rays_bytes);
+  gp.G = reinterpret_cast<float*>(w + rays_bytes + bound_bytes);
+  gp.bricked = with_bricked ? reinterpret_cast<float*>(w + rays_bytes + bound_bytes + g_bytes) : nullptr;
   return true;
 }
 
@@ -1296,11 +1344,11 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
     } else if constexpr (LIGHT) {
       NGT_SWITCH_MODE(blending_mode,
           hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<4, BORDER, MM, true, true>), rgrid, dim3(kRayBlock), 0, s, a, cb,
-                             base, output, output_grad, gp.G, gp.rays, gp.kcap));
+                             base, output, output_grad, gp.G, gp.rays, gp.bound, gp.kcap));
     } else {
       NGT_SWITCH_C(C, NGT_SWITCH_MODE(blending_mode,
           hipLaunchKernelGGL((raymarch_bwd_prepass_kernel<CC, BORDER, MM, false, true>), rgrid, dim3(kRayBlock), 0, s, a, cb,
-                             base, output, output_grad, gp.G, gp.rays, gp.kcap)));
+                             base, output, output_grad, gp.G, gp.rays, gp.bound, gp.kcap)));
     }
     // bricks: larger ones visit fewer samples twice (the support box is one
     // cell wider than the brick), but the 64-bit accumulators of 2 or 4
@@ -1312,12 +1360,13 @@ void run_gather(const std::vector<Pair>& pairs, const float* M, const float* V, 
     if (CG == 1) {
       const unsigned bricks = blocks_for(a.din.x, BX1) * blocks_for(a.din.y, BY1) * blocks_for(a.din.z, BZ1);
       hipLaunchKernelGGL((raymarch_bwd_brick_kernel<1, BX1, BY1, BZ1>), dim3(bricks * n_data), dim3(256), 0, s, a,
-                         cb, base, gp.rays, gp.G, gp.kcap, in_grad, accumulate, recorded ? grad_scale : nullptr);
+                         cb, base, gp.rays, gp.bound, gp.G, gp.kcap, in_grad, accumulate,
+                         recorded ? grad_scale : nullptr);
     } else {
       const unsigned bricks = blocks_for(a.din.x, BX) * blocks_for(a.din.y, BY) * blocks_for(a.din.z, BZ);
       NGT_SWITCH_C(CG,
           hipLaunchKernelGGL((raymarch_bwd_brick_kernel<CC, BX, BY, BZ>), dim3(bricks * n_data), dim3(256), 0, s, a,
-                             cb, base, gp.rays, gp.G, gp.kcap, in_grad, accumulate,
+                             cb, base, gp.rays, gp.bound, gp.G, gp.kcap, in_grad, accumulate,
                              recorded ? grad_scale : nullptr));
     }
   }
@@ -1384,7 +1433,7 @@ struct RenBatchItem {
   gptr<const float> l2_target;
   gptr<float> loss;
   int64_t ws_bricked;        // byte offset of the haloed copy in the workspace
-  int64_t rec_rays, rec_G;   // byte offsets of the records in the record region
+  int64_t rec_rays, rec_bound, rec_G;  // byte offsets of the records in the record region
 };
 
 struct RenBatchView {
@@ -1463,7 +1512,8 @@ __global__ __launch_bounds__(256) void render_prepass_batch_kernel(const void* _
     raymarch_bwd_prepass_kernel_body<4, BORDER, MODE, true, true>(
         ray_tile(it, t - v.ray[i]), batch_ray_args(v, it, ws, false), it.cb, 0, (float*)it.images,
         (const float*)it.images_grad,
-        reinterpret_cast<float*>(rec + it.rec_G), reinterpret_cast<RayRec*>(rec + it.rec_rays), it.kcap);
+        reinterpret_cast<float*>(rec + it.rec_G), reinterpret_cast<RayRec*>(rec + it.rec_rays),
+        reinterpret_cast<float*>(rec + it.rec_bound), it.kcap);
   }
 }
 
@@ -1479,7 +1529,8 @@ __global__ __launch_bounds__(256) void render_record_batch_kernel(const void* __
     const RenBatchItem& it = v.items[i];
     raymarch_fwd_record_kernel_body<MODE>(ray_tile(it, t - v.ray[i]), batch_ray_args(v, it, ws, true), it.cb,
                                           (float*)it.images, 0, reinterpret_cast<float*>(rec + it.rec_G),
-                                          reinterpret_cast<RayRec*>(rec + it.rec_rays), it.kcap, t);
+                                          reinterpret_cast<RayRec*>(rec + it.rec_rays),
+                                          reinterpret_cast<float*>(rec + it.rec_bound), it.kcap, t);
     __syncthreads();  // the loss partial's LDS is written again by the next tile
   }
 }
@@ -1497,6 +1548,7 @@ __global__ __launch_bounds__(256) void render_brick_batch_kernel(const void* __r
     const RenBatchItem& it = v.items[i];
     raymarch_bwd_brick_kernel_body<1, BX, BY, BZ>(VBlock{t - table[i], 0, 0}, it.a, it.cb, 0,
                                                   reinterpret_cast<const RayRec*>(rec + it.rec_rays),
+                                                  reinterpret_cast<const float*>(rec + it.rec_bound),
                                                   reinterpret_cast<const float*>(rec + it.rec_G), it.kcap,
                                                   (float*)it.density_grad, 0, nullptr);
     __syncthreads();  // the brick's LDS arrays are rewritten by the next brick
@@ -1574,7 +1626,7 @@ int render_density_fwd_impl(const float* density, const int64_t* density_shape, 
     if (record) {
       NGT_SWITCH_MODE(blending_mode,
           hipLaunchKernelGGL((raymarch_fwd_record_kernel<MM>), grid, dim3(kRayBlock), 0, s, a, cb, output, base, gp.G,
-                             gp.rays, gp.kcap));
+                             gp.rays, gp.bound, gp.kcap));
     } else if (bricked) {
       NGT_SWITCH_BM(boundary, NGT_SWITCH_MODE(blending_mode,
           hipLaunchKernelGGL((raymarch_fwd_kernel<4, LINEAR, BB, MM, true, true>), grid, dim3(kRayBlock), 0, s, a, cb,
@@ -1899,6 +1951,7 @@ size_t ngt_render_density_l2_record_size(const int64_t* density_shape, const flo
   if (kcap <= 0) return 0;
   const size_t npix = static_cast<size_t>(a.out_w) * a.out_h;
   return align_up(pairs.size() * npix * sizeof(RayRec)) +
+         align_up(pairs.size() * bound_tiles(a.out_w, a.out_h) * sizeof(float)) +
          align_up(pairs.size() * static_cast<size_t>(kcap) * g_pixels(npix) * sizeof(float));
 }
 
@@ -2044,14 +2097,18 @@ int ngt_render_batch_plan(const ngt_render_instance* inst, int n, int blending_m
     // records: chunk-relative offsets; a new chunk starts where the budget would be exceeded
     const size_t npix = static_cast<size_t>(it.a.out_w) * it.a.out_h;
     const size_t rays_bytes = align_up(q.num_views * npix * sizeof(RayRec));
+    const size_t bound_bytes = align_up(q.num_views * static_cast<size_t>(bound_tiles(it.a.out_w, it.a.out_h)) *
+                                        sizeof(float));
     const size_t g_bytes = align_up(q.num_views * static_cast<size_t>(it.kcap) * g_pixels(npix) * sizeof(float));
-    if (chunk_bytes > 0 && chunk_bytes + rays_bytes + g_bytes > record_budget) {
+    const size_t inst_bytes = rays_bytes + bound_bytes + g_bytes;
+    if (chunk_bytes > 0 && chunk_bytes + inst_bytes > record_budget) {
       chunk[++nchunks] = i;
       chunk_bytes = 0;
     }
     it.rec_rays = static_cast<int64_t>(chunk_bytes);
-    it.rec_G = static_cast<int64_t>(chunk_bytes + rays_bytes);
-    chunk_bytes += rays_bytes + g_bytes;
+    it.rec_bound = static_cast<int64_t>(chunk_bytes + rays_bytes);
+    it.rec_G = static_cast<int64_t>(chunk_bytes + rays_bytes + bound_bytes);
+    chunk_bytes += inst_bytes;
     region = std::max(region, chunk_bytes);
     tr += static_cast<int64_t>(q.num_views) * it.ntx * it.nty;
     tz += it.brick_tiles;

@@ -126,9 +126,10 @@ def test_step_cap_of_the_benchmark_views(lib):
 
 def test_record_size_of_the_benchmark_views(lib):
     """The fused render + L2 forward's record (ngt_render_density_l2_record_size)
-    holds a 48-byte ray record per pixel and one float per recorded sample
-    (kcap per ray, pixels in groups of 8), each region 256-byte aligned; the
-    two-walk backward's workspace adds the haloed brick copy to it."""
+    holds a 48-byte ray record per pixel, a 4-byte gradient bound per 8x8
+    pixel tile and one float per recorded sample (kcap per ray, pixels in
+    groups of 8), each region 256-byte aligned; the two-walk backward's workspace adds the haloed
+    brick copy to it."""
     from ngt import _lib, synth
     from ngt.renderer import Renderer
     grid_t, cams = synth.ring_setup((128, 128, 128), (144, 256), 128, 5)
@@ -141,6 +142,6 @@ def test_record_size_of_the_benchmark_views(lib):
     cap = step_cap(lib, (128, 128, 128), M, V, P, F, (128, 144, 256))
     up = lambda n: (n + 255) // 256 * 256
     npix = 144 * 256
-    assert record == up(5 * npix * 48) + up(5 * cap * ((npix + 7) // 8 * 8) * 4)
+    assert record == up(5 * npix * 48) + up(5 * (256 // 8) * (144 // 8) * 4) + up(5 * cap * ((npix + 7) // 8 * 8) * 4)
     assert two_walk == record + up(lib.ngt_bricked_grid_bytes(shape))
     print("record %.1f MB, two-walk workspace %.1f MB" % (record / 1e6, two_walk / 1e6))
