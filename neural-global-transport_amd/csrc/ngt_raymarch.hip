@@ -687,8 +687,9 @@ NGT_DEVICE int block_inclusive_scan(int v, int* s_wtot) {
 // The sums are kept in LDS as 64-bit fixed point and added with integer LDS
 // atomics (ds_add_u64, at the LDS's full rate; float LDS atomics run some
 // 30x slower here).  The brick's unit is 2^-S with S chosen from the largest
-// per-ray gradient bound among its rays (the bound array, written by the walk
-// that recorded the gradients), so that every corner contribution rounds to
+// gradient bound of the 8x8 pixel tiles its rays belong to (the bound array,
+// written by the walk that recorded the gradients), so that every corner
+// contribution rounds to
 // an integer below 2^30 in magnitude: the rounding error of a contribution is
 // at most 2^-30 of the brick's largest sample gradient, and integer sums are
 // exact and independent of the order of the atomics (the gradient is
@@ -783,7 +784,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
 
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
   const int total = ray_begin[cb.count];
-  // the brick's fixed-point unit from the largest gradient bound of its rays
+  // the brick's fixed-point unit from the largest gradient bound of its rays' tiles
   auto ray_index = [&](int r, int& px, int& py) {
     int ci = 0;
     while (ci + 1 < cb.count && r >= ray_begin[ci + 1]) ++ci;
