@@ -80,9 +80,12 @@ NGT_DEVICE void vstore(float* __restrict__ p, const Vec<C>& r) {
 
 template <int C>
 NGT_DEVICE Vec<C> vlerp(const Vec<C>& a, const Vec<C>& b, float t) {
+  // a + t (b - a) with one rounding of the product and sum (an explicit FMA,
+  // so that every kernel that interpolates forms the same bits, whatever the
+  // compiler's contraction choices in its context)
   Vec<C> r;
 #pragma unroll
-  for (int i = 0; i < C; ++i) r.v[i] = a.v[i] + t * (b.v[i] - a.v[i]);
+  for (int i = 0; i < C; ++i) r.v[i] = fmaf(t, b.v[i] - a.v[i], a.v[i]);
   return r;
 }
 template <int C>
@@ -801,8 +804,10 @@ NGT_DEVICE void blend_step(Vec<C>& acc, const Vec<C>& cell) {
   } else if constexpr (MODE == BEER_LAMBERT) {
     const float dd = fmaxf(acc.v[C - 1] + cell.v[C - 1], 0.f);
     const float t = exp_neg(dd);
+    // one explicit rounding per channel, so that every walk that blends
+    // (image, recording, adjoint) forms the same image bits
 #pragma unroll
-    for (int i = 0; i < C - 1; ++i) acc.v[i] += t * cell.v[i];
+    for (int i = 0; i < C - 1; ++i) acc.v[i] = fmaf(t, cell.v[i], acc.v[i]);
     acc.v[C - 1] = dd;
   } else if constexpr (MODE == ALPHA) {
     const float t = 1.f - acc.v[C - 1];

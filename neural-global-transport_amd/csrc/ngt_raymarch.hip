@@ -53,12 +53,14 @@ struct RayArgs {
 
 NGT_DEVICE int kMaxRaySteps(const RayArgs& a) { return 16 * (a.din.x + a.din.y + a.din.z + a.out_d); }
 
-// Ray samples whose gathers are issued together.  2 measured fastest on the
-// 128^3 / 5-view render (recording forward 60.3 us; 1: 60.4, 3: 62.5,
-// 4: 64.9, 8: 72.2): with few rays per SIMD the walk is bound by its
-// per-sample recurrence more than by the latency of its corner loads, and
-// wider rounds only add registers and fetched past-the-end samples.
+// Ray samples whose gathers are issued together, measured on the 128^3 /
+// 5-view render with one wavefront per 8x8 pixel tile: the image walk is
+// fastest with 2 (43.7 us; 1: 48.9, 4: 47.2), the recording walk (more work
+// per sample) with 4 (54.7 us; 1: 54.9, 2: 58.1).  With few rays per SIMD a
+// walk is bound by its per-sample recurrence as much as by the latency of its
+// corner loads, and wider rounds add registers and past-the-end samples.
 constexpr int kUnroll = 2;
+constexpr int kRecordUnroll = 4;
 
 struct Ray {
   float sx, sy, sz;  // start
@@ -177,18 +179,22 @@ NGT_DEVICE const float* ray_source(const RayArgs& a, int n) {
   else return a.input + static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z * CIN;
 }
 
-// kUnroll cell values at once; for LINEAR all their corner loads are issued
+// U cell values at once; for LINEAR all their corner loads are issued
 // before any is used.
-template <int C, int FM, int BM, bool LIGHT, bool BRICKED>
-NGT_DEVICE void sample_cells(const RayArgs& a, const float* __restrict__ src, const float (&qx)[kUnroll],
-                             const float (&qy)[kUnroll], const float (&qz)[kUnroll], Vec<C> (&cells)[kUnroll]) {
+template <int C, int FM, int BM, bool LIGHT, bool BRICKED, int U>
+NGT_DEVICE void sample_cells(const RayArgs& a, const float* __restrict__ src, const float (&qx)[U],
+                             const float (&qy)[U], const float (&qz)[U], Vec<C> (&cells)[U]) {
   if constexpr (FM == LINEAR) {
     constexpr int CIN = LIGHT ? 1 : C;
-    Vec<CIN> raw[kUnroll];
-    read_linear_batch<CIN, BM, kUnroll, BRICKED>(src, a.din, a.bd, qx, qy, qz, raw);
+    Vec<CIN> raw[U];
+    read_linear_batch<CIN, BM, U, BRICKED>(src, a.din, a.bd, qx, qy, qz, raw);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       if constexpr (LIGHT) {
+        // products rounded on their own (never fused into a later add), so
+        // that every walk, however its samples are unrolled, forms the same
+        // optical depths and images
+#pragma clang fp contract(off)
         const float s = raw[u].v[0] * a.depth_step;
         cells[u].v[0] = a.light_r * s; cells[u].v[1] = a.light_g * s; cells[u].v[2] = a.light_b * s;
         cells[u].v[3] = s;
@@ -199,7 +205,7 @@ NGT_DEVICE void sample_cells(const RayArgs& a, const float* __restrict__ src, co
     }
   } else {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) cells[u] = sample_cell<C, FM, BM, LIGHT>(a, src, qx[u], qy[u], qz[u]);
+    for (int u = 0; u < U; ++u) cells[u] = sample_cell<C, FM, BM, LIGHT>(a, src, qx[u], qy[u], qz[u]);
   }
 }
 
@@ -580,17 +586,17 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   Vec<4> acc = vzero<4>();
   float q = 0.f;  // Q_{k-1}
   float hmax = 0.f;
-  for (int k = 0; k < r.steps; k += kUnroll) {
-    float qx[kUnroll], qy[kUnroll], qz[kUnroll];
-    Vec<4> cells[kUnroll];
+  for (int k = 0; k < r.steps; k += kRecordUnroll) {
+    float qx[kRecordUnroll], qy[kRecordUnroll], qz[kRecordUnroll];
+    Vec<4> cells[kRecordUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < kRecordUnroll; ++u) {
       const float kf = static_cast<float>(k + u);
       qx[u] = fmaf(kf, r.dx, r.sx) - 0.5f; qy[u] = fmaf(kf, r.dy, r.sy) - 0.5f; qz[u] = fmaf(kf, r.dz, r.sz) - 0.5f;
     }
     sample_cells<4, LINEAR, BORDER, true, true>(a, src, qx, qy, qz, cells);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
+    for (int u = 0; u < kRecordUnroll; ++u) {
       const bool live = k + u < r.steps;
 #pragma unroll
       for (int i = 0; i < 4; ++i) cells[u].v[i] = live ? cells[u].v[i] : 0.f;
@@ -599,7 +605,7 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
         const float dd = fmaxf(acc.v[3] + cells[u].v[3], 0.f);
         const float e = exp_neg(dd);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) acc.v[i] += e * cells[u].v[i];
+        for (int i = 0; i < 3; ++i) acc.v[i] = fmaf(e, cells[u].v[i], acc.v[i]);  // as blend_step
         acc.v[3] = dd;
         h = e + q;
         q += e * cells[u].v[3];
