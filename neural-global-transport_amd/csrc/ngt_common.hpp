@@ -238,26 +238,34 @@ inline BrickDims brick_dims(const Dims& d) {
 
 // Whether a C-channel slice of the haloed layout is addressable with 32-bit
 // offsets (the bricked kernels are used only then).
+// (The brick index is formed with 24-bit multiplies: bricks per plane below
+// 2^24.)
 inline bool bricked_fits(const Dims& d, int C) {
   const BrickDims b = brick_dims(d);
-  return b.slice_cells != 0 && static_cast<uint64_t>(b.slice_cells) * C < (uint64_t(1) << 32);
+  return b.slice_cells != 0 && static_cast<uint64_t>(b.slice_cells) * C < (uint64_t(1) << 32) &&
+         b.nbxy < (1u << 24);
 }
 
 // Offset (cells) of the floor corner of a footprint in the haloed layout.
-// x0 etc. are floor(q); BORDER and CLAMP clamp them into [-1, d - 1] (for
-// BORDER that changes only positions outside the forward validity, whose
-// value is zeroed anyway; for CLAMP both corners of a clamped axis then read
-// the same edge value, as the clamped reads do), WRAP wraps them.
+// fx etc. are floor(q) as floats; BORDER and CLAMP clamp them into
+// [-1, d - 1] (for BORDER that changes only positions outside the forward
+// validity, whose value is zeroed anyway; for CLAMP both corners of a clamped
+// axis then read the same edge value, as the clamped reads do), WRAP wraps
+// them.  The clamp is one med3 on floor(q) + 1 (exact for |q| < 2^24, and
+// any larger or NaN q lands in range as well).
 template <int BM>
-NGT_DEVICE uint32_t brick_corner(const Dims& d, const BrickDims& b, int x0, int y0, int z0) {
+NGT_DEVICE uint32_t brick_corner(const Dims& d, const BrickDims& b, float fx, float fy, float fz) {
+  uint32_t xp, yp, zp;  // corner + 1, in [0, d]
   if constexpr (BM == WRAP) {
-    x0 = wrap_index(x0, d.x); y0 = wrap_index(y0, d.y); z0 = wrap_index(z0, d.z);
+    xp = static_cast<uint32_t>(wrap_index(static_cast<int>(fx), d.x) + 1);
+    yp = static_cast<uint32_t>(wrap_index(static_cast<int>(fy), d.y) + 1);
+    zp = static_cast<uint32_t>(wrap_index(static_cast<int>(fz), d.z) + 1);
   } else {
-    x0 = min(max(x0, -1), d.x - 1); y0 = min(max(y0, -1), d.y - 1); z0 = min(max(z0, -1), d.z - 1);
+    xp = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fx + 1.f, 0.f, static_cast<float>(d.x)));
+    yp = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fy + 1.f, 0.f, static_cast<float>(d.y)));
+    zp = static_cast<uint32_t>(__builtin_amdgcn_fmed3f(fz + 1.f, 0.f, static_cast<float>(d.z)));
   }
-  const uint32_t xp = static_cast<uint32_t>(x0 + 1), yp = static_cast<uint32_t>(y0 + 1),
-                 zp = static_cast<uint32_t>(z0 + 1);
-  const uint32_t brick = (zp >> 2) * b.nbxy + (yp >> 2) * b.nbx + (xp >> 2);
+  const uint32_t brick = __umul24(zp >> 2, b.nbxy) + __umul24(yp >> 2, b.nbx) + (xp >> 2);
   return brick * kBrickFloats + (xp & 3u) + (yp & 3u) * 5u + (zp & 3u) * 25u;
 }
 
@@ -312,41 +320,68 @@ NGT_DEVICE Vec<C> read_linear(const float* __restrict__ g, const Dims& d, float 
   return lerp8<C, BM>(v, c);
 }
 
+// The corner loads of U trilinear reads in the haloed brick layout, in
+// flight: bricked_taps_issue starts them, bricked_taps_finish interpolates
+// (a walk may issue the next round before it finishes this one).
+template <int C, int U>
+struct BrickedTaps {
+  Vec<C> v[U][8];
+  float wx[U], wy[U], wz[U];
+  bool inside[U];
+};
+
+template <int C, int BM, int U>
+NGT_DEVICE void bricked_taps_issue(const float* __restrict__ g, const Dims& d, const BrickDims& b,
+                                   const float (&qx)[U], const float (&qy)[U], const float (&qz)[U],
+                                   BrickedTaps<C, U>& t) {
+  // one address per sample, no per-corner masks
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const float fx = floorf(qx[u]), fy = floorf(qy[u]), fz = floorf(qz[u]);
+    t.wx[u] = qx[u] - fx; t.wy[u] = qy[u] - fy; t.wz[u] = qz[u] - fz;
+    t.inside[u] = true;
+    // q + 0.5 in [0, d), compared as q in [-0.5, d - 0.5): the same decision
+    // for every float q below 2^22 (q + 0.5 is exact there, or, below -1,
+    // negative either way), without the three adds
+    if constexpr (BM == BORDER)
+      t.inside[u] = (qx[u] >= -0.5f) & (qx[u] < static_cast<float>(d.x) - 0.5f) & (qy[u] >= -0.5f) &
+                    (qy[u] < static_cast<float>(d.y) - 0.5f) & (qz[u] >= -0.5f) &
+                    (qz[u] < static_cast<float>(d.z) - 0.5f);
+    const uint32_t base = brick_corner<BM>(d, b, fx, fy, fz);
+    const float* p = g + base * static_cast<uint32_t>(C);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t.v[u][k] = vload<C>(p + brick_corner_step(k) * C);
+  }
+}
+
+// keep[u] false zeroes sample u as well (one select with the BORDER mask).
+template <int C, int BM, int U>
+NGT_DEVICE void bricked_taps_finish(const BrickedTaps<C, U>& t, Vec<C> (&out)[U], const bool (&keep)[U]) {
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const Vec<C> v0 = vlerp(vlerp(t.v[u][0], t.v[u][1], t.wx[u]), vlerp(t.v[u][2], t.v[u][3], t.wx[u]), t.wy[u]);
+    const Vec<C> v1 = vlerp(vlerp(t.v[u][4], t.v[u][5], t.wx[u]), vlerp(t.v[u][6], t.v[u][7], t.wx[u]), t.wy[u]);
+    out[u] = vlerp(v0, v1, t.wz[u]);
+    const bool ok = (BM == BORDER ? t.inside[u] : true) && keep[u];
+#pragma unroll
+    for (int i = 0; i < C; ++i) out[u].v[i] = ok ? out[u].v[i] : 0.f;
+  }
+}
+
 // U trilinear reads whose 8*U loads are all issued before any is consumed;
 // BRICKED selects the haloed brick layout (BrickDims b).
 template <int C, int BM, int U, bool BRICKED>
 NGT_DEVICE void read_linear_batch(const float* __restrict__ g, const Dims& d, const BrickDims& b, const float (&qx)[U],
                                   const float (&qy)[U], const float (&qz)[U], Vec<C> (&out)[U]) {
-  Vec<C> v[U][8];
   if constexpr (BRICKED) {
-    // haloed bricks: one address per sample, no per-corner masks
-    float wx[U], wy[U], wz[U];
-    bool inside[U];
+    BrickedTaps<C, U> t;
+    bricked_taps_issue<C, BM, U>(g, d, b, qx, qy, qz, t);
+    bool keep[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const float fx = floorf(qx[u]), fy = floorf(qy[u]), fz = floorf(qz[u]);
-      wx[u] = qx[u] - fx; wy[u] = qy[u] - fy; wz[u] = qz[u] - fz;
-      inside[u] = true;
-      if constexpr (BM == BORDER)
-        inside[u] = (qx[u] + 0.5f >= 0.f) & (qx[u] + 0.5f < static_cast<float>(d.x)) & (qy[u] + 0.5f >= 0.f) &
-                    (qy[u] + 0.5f < static_cast<float>(d.y)) & (qz[u] + 0.5f >= 0.f) &
-                    (qz[u] + 0.5f < static_cast<float>(d.z));
-      const uint32_t base = brick_corner<BM>(d, b, static_cast<int>(fx), static_cast<int>(fy), static_cast<int>(fz));
-      const float* p = g + base * static_cast<uint32_t>(C);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) v[u][k] = vload<C>(p + brick_corner_step(k) * C);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const Vec<C> v0 = vlerp(vlerp(v[u][0], v[u][1], wx[u]), vlerp(v[u][2], v[u][3], wx[u]), wy[u]);
-      const Vec<C> v1 = vlerp(vlerp(v[u][4], v[u][5], wx[u]), vlerp(v[u][6], v[u][7], wx[u]), wy[u]);
-      out[u] = vlerp(v0, v1, wz[u]);
-      if constexpr (BM == BORDER) {
-#pragma unroll
-        for (int i = 0; i < C; ++i) out[u].v[i] = inside[u] ? out[u].v[i] : 0.f;
-      }
-    }
+    for (int u = 0; u < U; ++u) keep[u] = true;
+    bricked_taps_finish<C, BM, U>(t, out, keep);
   } else {
+    Vec<C> v[U][8];
     Corners c[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) c[u] = make_corners<BM>(d, qx[u], qy[u], qz[u]);

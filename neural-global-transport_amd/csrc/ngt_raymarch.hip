@@ -143,6 +143,7 @@ inline dim3 ray_grid_dims(int out_w, int out_h, int count) {
   return dim3(blocks_for(out_w, kRayTile), blocks_for(out_h, kRayTile), count);
 }
 
+
 // Cell value read by a sample: the C-channel input (LIGHT = false) or the
 // fused renderer's (rho*rgb*ds, rho*ds) from the 1-channel density.
 // BRICKED (LINEAR only): src is a slice of the bricked copy.
@@ -179,6 +180,46 @@ NGT_DEVICE const float* ray_source(const RayArgs& a, int n) {
   else return a.input + static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z * CIN;
 }
 
+// The light cell (rho rgb ds, rho ds) of a density sample.  Its products are
+// rounded on their own (never fused into a later add), so that every walk,
+// however its samples are unrolled or pipelined, forms the same optical
+// depths and images.
+NGT_DEVICE Vec<4> light_cell(const RayArgs& a, float rho) {
+#pragma clang fp contract(off)
+  const float s = rho * a.depth_step;
+  Vec<4> c;
+  c.v[0] = a.light_r * s; c.v[1] = a.light_g * s; c.v[2] = a.light_b * s;
+  c.v[3] = s;
+  return c;
+}
+
+// One sample of a light-cell walk with BEER_LAMBERT or ADDITIVE blending, in
+// scalar form.  The light is the same at every sample, so the colour is
+// light * Q with Q = sum_k e_k s_k (BEER_LAMBERT; e_k = exp(-dd_k), dd_k the
+// optical depth through sample k) or Q = sum_k s_k (ADDITIVE), where s_k =
+// rho_k ds: one accumulator in place of three.  Returns e_k (1 for ADDITIVE).
+template <int MODE>
+NGT_DEVICE float light_blend_step(float& dd, float& Q, float s) {
+  if constexpr (MODE == BEER_LAMBERT) {
+    dd = fmaxf(dd + s, 0.f);
+    const float e = exp_neg(dd);
+    Q = fmaf(e, s, Q);
+    return e;
+  } else {
+    Q += s;
+    return 1.f;
+  }
+}
+
+// The pixel (rgb, optical depth or summed depth) of a light-cell walk.
+template <int MODE>
+NGT_DEVICE Vec<4> light_pixel(const RayArgs& a, float dd, float Q) {
+  Vec<4> p;
+  p.v[0] = a.light_r * Q; p.v[1] = a.light_g * Q; p.v[2] = a.light_b * Q;
+  p.v[3] = MODE == BEER_LAMBERT ? dd : Q;
+  return p;
+}
+
 // U cell values at once; for LINEAR all their corner loads are issued
 // before any is used.
 template <int C, int FM, int BM, bool LIGHT, bool BRICKED, int U>
@@ -191,13 +232,7 @@ NGT_DEVICE void sample_cells(const RayArgs& a, const float* __restrict__ src, co
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if constexpr (LIGHT) {
-        // products rounded on their own (never fused into a later add), so
-        // that every walk, however its samples are unrolled, forms the same
-        // optical depths and images
-#pragma clang fp contract(off)
-        const float s = raw[u].v[0] * a.depth_step;
-        cells[u].v[0] = a.light_r * s; cells[u].v[1] = a.light_g * s; cells[u].v[2] = a.light_b * s;
-        cells[u].v[3] = s;
+        cells[u] = light_cell(a, raw[u].v[0]);
       } else {
 #pragma unroll
         for (int i = 0; i < C; ++i) cells[u].v[i] = raw[u].v[i];
@@ -235,6 +270,22 @@ NGT_DEVICE void raymarch_fwd_pixel(const RayArgs& a, const CamBatch& cb, const V
   // round's positions past the ray end are fetched (in-grid addresses) and
   // blended as empty cells, which leaves the accumulator unchanged in every
   // blend mode (no branch per sample, so no load is sunk into one).
+  if constexpr (LIGHT && (MODE == BEER_LAMBERT || MODE == ADDITIVE)) {
+    float dd = 0.f, Q = 0.f;
+    for (int k = 0; k < r.steps; k += kUnroll) {
+      float qx[kUnroll], qy[kUnroll], qz[kUnroll];
+      Vec<C> cells[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const float kf = static_cast<float>(k + u);
+        qx[u] = fmaf(kf, r.dx, r.sx) - 0.5f; qy[u] = fmaf(kf, r.dy, r.sy) - 0.5f; qz[u] = fmaf(kf, r.dz, r.sz) - 0.5f;
+      }
+      sample_cells<C, FM, BM, LIGHT, BRICKED>(a, src, qx, qy, qz, cells);
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) light_blend_step<MODE>(dd, Q, k + u < r.steps ? cells[u].v[3] : 0.f);
+    }
+    acc = light_pixel<MODE>(a, dd, Q);
+  } else {
   for (int k = 0; k < r.steps; k += kUnroll) {
     float qx[kUnroll], qy[kUnroll], qz[kUnroll];
     Vec<C> cells[kUnroll];
@@ -251,6 +302,7 @@ NGT_DEVICE void raymarch_fwd_pixel(const RayArgs& a, const CamBatch& cb, const V
       for (int i = 0; i < C; ++i) cells[u].v[i] = live ? cells[u].v[i] : 0.f;
       blend_step<C, MODE>(acc, cells[u]);
     }
+  }
   }
   const int64_t pix = (static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x;
   vstore<C>(out + pix * C, acc);
@@ -583,40 +635,34 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   const int pair = pair_base + vb.z;
   float* gp = G + static_cast<int64_t>(pair) * kcap * g_pixels(npix) + g_offset(0, pixel, kcap);
   const int rec_steps = min(r.steps, kcap);
-  Vec<4> acc = vzero<4>();
-  float q = 0.f;  // Q_{k-1}
+  float dd = 0.f, q = 0.f;  // optical depth, Q_{k-1}
   float hmax = 0.f;
-  for (int k = 0; k < r.steps; k += kRecordUnroll) {
-    float qx[kRecordUnroll], qy[kRecordUnroll], qz[kRecordUnroll];
-    Vec<4> cells[kRecordUnroll];
+  constexpr int U = kRecordUnroll;
+  for (int k = 0; k < r.steps; k += U) {
+    float qx[U], qy[U], qz[U];
+    bool live[U];
 #pragma unroll
-    for (int u = 0; u < kRecordUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       const float kf = static_cast<float>(k + u);
       qx[u] = fmaf(kf, r.dx, r.sx) - 0.5f; qy[u] = fmaf(kf, r.dy, r.sy) - 0.5f; qz[u] = fmaf(kf, r.dz, r.sz) - 0.5f;
+      live[u] = k + u < r.steps;
     }
-    sample_cells<4, LINEAR, BORDER, true, true>(a, src, qx, qy, qz, cells);
+    BrickedTaps<1, U> taps;
+    bricked_taps_issue<1, BORDER, U>(src, a.din, a.bd, qx, qy, qz, taps);
+    Vec<1> raw[U];
+    bricked_taps_finish<1, BORDER, U>(taps, raw, live);  // samples past the end are empty
 #pragma unroll
-    for (int u = 0; u < kRecordUnroll; ++u) {
-      const bool live = k + u < r.steps;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) cells[u].v[i] = live ? cells[u].v[i] : 0.f;
-      float h = 1.f;
-      if constexpr (MODE == BEER_LAMBERT) {
-        const float dd = fmaxf(acc.v[3] + cells[u].v[3], 0.f);
-        const float e = exp_neg(dd);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) acc.v[i] = fmaf(e, cells[u].v[i], acc.v[i]);  // as blend_step
-        acc.v[3] = dd;
-        h = e + q;
-        q += e * cells[u].v[3];
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc.v[i] += cells[u].v[i];
-      }
+    for (int u = 0; u < U; ++u) {
+      const float q_prev = q;
+      const float e = light_blend_step<MODE>(dd, q, light_cell(a, raw[u].v[0]).v[3]);
+      const float h = MODE == BEER_LAMBERT ? e + q_prev : 1.f;
       if (k + u < rec_steps) gp[static_cast<int64_t>(k + u) * kGPix] = h;
-      hmax = fmaxf(hmax, fabsf(h)) + (h - h);  // NaN stays NaN
+      // a NaN h needs no propagation here: h = e + Q_{k-1} with e in [0, 1],
+      // so it is NaN only when Q is, and the bound includes |Q_N|
+      hmax = fmaxf(hmax, fabsf(h));
     }
   }
+  const Vec<4> acc = light_pixel<MODE>(a, dd, q);
   const int64_t pix = (static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x;
   vstore<4>(out + pix * 4, acc);
   const float* t = a.l2_target + pix * 3;
