@@ -485,14 +485,18 @@ __global__ __launch_bounds__(256) void raymarch_bwd_exact_kernel(RayArgs a, CamB
 
 // Recorded per-sample gradients G of one (data, camera) pair: the image's
 // pixels (row-major) in groups of kGPix, each group holding its pixels'
-// samples k side by side, (k, pixel) at
-// (pixel / kGPix) * kcap * kGPix + k * kGPix + pixel % kGPix.  The gather
-// reads consecutive samples of one ray kGPix * 4 bytes apart; the recording
-// writes the kGPix pixels of a group at one k together.
-constexpr int kGPix = 8;
+// samples in quads of kGQuad consecutive k, (k, pixel) at
+//   (pixel / kGPix) * kcap * kGPix + (k / kGQuad) * kGPix * kGQuad
+//   + (pixel % kGPix) * kGQuad + k % kGQuad
+// (kcap is a multiple of kGQuad).  The recording walk stores a pixel's quad
+// with one 16-byte store, so the 8 pixels of a wavefront's tile row write one
+// 128-byte line; the gather reads a ray's consecutive samples 4 to a quad.
+constexpr int kGPix = 8, kGQuad = 4;
 inline __host__ __device__ int64_t g_pixels(int64_t npix) { return (npix + kGPix - 1) / kGPix * kGPix; }
+// offset of sample k from sample 0 of the same pixel
+NGT_DEVICE int64_t g_step(int k) { return static_cast<int64_t>(k / kGQuad) * (kGPix * kGQuad) + k % kGQuad; }
 NGT_DEVICE int64_t g_offset(int k, int64_t pixel, int kcap) {
-  return (pixel / kGPix) * kcap * kGPix + static_cast<int64_t>(k) * kGPix + pixel % kGPix;
+  return (pixel / kGPix) * kcap * kGPix + (pixel % kGPix) * kGQuad + g_step(k);
 }
 
 // A ray of the gather adjoint, and how its stored per-sample values become
@@ -638,6 +642,7 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
   float dd = 0.f, q = 0.f;  // optical depth, Q_{k-1}
   float hmax = 0.f;
   constexpr int U = kRecordUnroll;
+  static_assert(U == kGQuad, "a round of the recording walk stores one quad of samples");
   for (int k = 0; k < r.steps; k += U) {
     float qx[U], qy[U], qz[U];
     bool live[U];
@@ -651,16 +656,20 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
     bricked_taps_issue<1, BORDER, U>(src, a.din, a.bd, qx, qy, qz, taps);
     Vec<1> raw[U];
     bricked_taps_finish<1, BORDER, U>(taps, raw, live);  // samples past the end are empty
+    Vec<4> hq;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const float q_prev = q;
       const float e = light_blend_step<MODE>(dd, q, light_cell(a, raw[u].v[0]).v[3]);
       const float h = MODE == BEER_LAMBERT ? e + q_prev : 1.f;
-      if (k + u < rec_steps) gp[static_cast<int64_t>(k + u) * kGPix] = h;
+      hq.v[u] = h;
       // a NaN h needs no propagation here: h = e + Q_{k-1} with e in [0, 1],
       // so it is NaN only when Q is, and the bound includes |Q_N|
       hmax = fmaxf(hmax, fabsf(h));
     }
+    // the whole quad (k < rec_steps <= kcap, both multiples of the quad's
+    // start): its samples past rec_steps are never read
+    if (k < rec_steps) vstore<4>(gp + g_step(k), hq);
   }
   const Vec<4> acc = light_pixel<MODE>(a, dd, q);
   const int64_t pix = (static_cast<int64_t>(cp.out_index) * a.out_h + y) * a.out_w + x;
@@ -986,7 +995,7 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
                    (ly <= b1y - fb0m1y) & (lz >= 0) & (lz <= b1z - fb0m1z);
           cell[u] = (lz * PY + ly) * PX + lx;
           gss[u] = pd.z; gos[u] = pd.w;
-          if (oks[u]) gvs[u] = vload<CG>(G + (s_gofs[lo_j] + static_cast<int64_t>(k) * kGPix) * CG);
+          if (oks[u]) gvs[u] = vload<CG>(G + (s_gofs[lo_j] + g_step(k)) * CG);
         }
       }
 #pragma unroll
@@ -1323,9 +1332,11 @@ int ray_step_cap(const std::vector<Pair>& pairs, const float* M, const float* V,
   if (!(min_step < 1e300)) return -1;
   const double k = std::ceil((diag + 2.0) / (0.99 * min_step)) + 3.0;
   if (!(k <= hard_cap)) return -1;
-  // a multiple of kUnroll (the walks fetch kUnroll samples per round)
+  // a multiple of the record's sample quad (and of the walks' rounds)
+  constexpr int kRound = kGQuad > kUnroll ? kGQuad : kUnroll;
+  static_assert(kRound % kUnroll == 0 && kRound % kGQuad == 0, "cap granularity");
   const int cap = static_cast<int>(k);
-  return (cap + kUnroll - 1) / kUnroll * kUnroll;
+  return (cap + kRound - 1) / kRound * kRound;
 }
 
 struct GatherPlan {
