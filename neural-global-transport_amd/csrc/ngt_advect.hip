@@ -823,6 +823,42 @@ __global__ __launch_bounds__(256) void advect_rho_gather_kernel(AdvArgs a, Brick
   advect_rho_gather_kernel_body<C>(VBlock{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)}, a, bg, r_dir, tiles2, g_rho);
 }
 
+// The MAC face gradients that cell (x, y, z) owns, from the centered velocity
+// gradient g (3 per cell): its lower face on each axis, and the upper one at
+// the grid's far side.  The same sums as faces_from_centered_grad_kernel.
+NGT_DEVICE void faces_of_cell(const Dims& d, const float* __restrict__ g, float* __restrict__ ox,
+                              float* __restrict__ oy, float* __restrict__ oz, int x, int y, int z) {
+  const int64_t plane = static_cast<int64_t>(d.x) * d.y;
+  const int64_t c = cell_offset(d, x, y, z);
+  const float g0 = g[c * 3 + 0], g1 = g[c * 3 + 1], g2 = g[c * 3 + 2];
+  const float l0 = x > 0 ? g[(c - 1) * 3 + 0] : 0.f;
+  const float l1 = y > 0 ? g[(c - d.x) * 3 + 1] : 0.f;
+  const float l2 = z > 0 ? g[(c - plane) * 3 + 2] : 0.f;
+  const int64_t fx = (static_cast<int64_t>(z) * d.y + y) * (d.x + 1) + x;
+  ox[fx] = 0.5f * (l0 + g0);
+  if (x == d.x - 1) ox[fx + 1] = 0.5f * g0;
+  const int64_t fy = (static_cast<int64_t>(z) * (d.y + 1) + y) * d.x + x;
+  oy[fy] = 0.5f * (l1 + g1);
+  if (y == d.y - 1) oy[fy + d.x] = 0.5f * g1;
+  oz[c] = 0.5f * (l2 + g2);
+  if (z == d.z - 1) oz[c + plane] = 0.5f * g2;
+}
+
+// advect_rho_gather_kernel and the face gradients in one pass, for one data
+// grid with one staggered velocity (grid: one work-group row per z plane).
+template <int C>
+__global__ __launch_bounds__(256) void advect_rho_faces_kernel(AdvArgs a, BrickGrid bg, const float* __restrict__ r_dir,
+                                                               const float* __restrict__ tiles2,
+                                                               float* __restrict__ g_rho,
+                                                               const float* __restrict__ g_vc, float* __restrict__ gx,
+                                                               float* __restrict__ gy, float* __restrict__ gz) {
+  const VBlock vb{static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), static_cast<int>(blockIdx.z)};
+  advect_rho_gather_kernel_body<C>(vb, a, bg, r_dir, tiles2, g_rho);
+  const int x = vb.x * kBX + static_cast<int>(threadIdx.x), y = vb.y * kBY + static_cast<int>(threadIdx.y);
+  if (x >= a.d.x || y >= a.d.y) return;
+  faces_of_cell(a.d, g_vc, gx, gy, gz, x, y, vb.z);
+}
+
 #define NGT_SWITCH_C(C_, ...)                              \
   switch (C_) {                                            \
     case 1: { constexpr int CC = 1; __VA_ARGS__; } break;  \
@@ -901,8 +937,8 @@ void launch_faces(const AdvPlan& p, const float* g_vc, float* gx, float* gy, flo
 // what the kernels read.
 struct AdvBatchHeader {
   int n, order, clamp, boundary, staggered;
-  int tiles_cell, tiles_brick, tiles_face, tiles_fill;
-  size_t off_items, off_cell, off_brick, off_face, off_fill;
+  int tiles_cell, tiles_brick, tiles_fill;
+  size_t off_items, off_cell, off_brick, off_fill;
   int64_t ws_bytes;
 };
 
@@ -930,7 +966,6 @@ struct AdvBatchView {
   const AdvBatchItem* items;
   const int* cell;
   const int* brick;
-  const int* face;
   const int* fill;
 };
 
@@ -941,7 +976,6 @@ NGT_DEVICE AdvBatchView adv_batch_view(const void* plan) {
   v.items = reinterpret_cast<const AdvBatchItem*>(p + v.h->off_items);
   v.cell = reinterpret_cast<const int*>(p + v.h->off_cell);
   v.brick = reinterpret_cast<const int*>(p + v.h->off_brick);
-  v.face = reinterpret_cast<const int*>(p + v.h->off_face);
   v.fill = reinterpret_cast<const int*>(p + v.h->off_fill);
   return v;
 }
@@ -1074,25 +1108,21 @@ __global__ __launch_bounds__(kBrickThreads) void advect_sl_bwd_batch_kernel(cons
   }
 }
 
+// The density gather and, for staggered velocities, the face gradients of
+// the same cells.
 __global__ __launch_bounds__(256) void advect_rho_gather_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
   const AdvBatchView v = adv_batch_view(plan);
+  const bool staggered = v.h->staggered != 0;
   for (int t = blockIdx.x; t < v.h->tiles_cell; t += gridDim.x) {
     const int i = __builtin_amdgcn_readfirstlane(find_item(v.cell, v.h->n, t));
     const AdvBatchItem& it = v.items[i];
     float* g_in = (float*)it.g_in;
     for_tile_planes(it, cell_tile(it, t - v.cell[i]), [&](const VBlock& vb) {
       advect_rho_gather_kernel_body<1>(vb, it.a, it.bg, nullptr, ws + it.ws_t2, g_in);
+      const int x = vb.x * kBX + static_cast<int>(threadIdx.x), y = vb.y * kBY + static_cast<int>(threadIdx.y);
+      if (staggered && x < it.a.d.x && y < it.a.d.y)
+        faces_of_cell(it.a.d, ws + it.ws_gvc, (float*)it.gvx, (float*)it.gvy, (float*)it.gvz, x, y, vb.z);
     });
-  }
-}
-
-__global__ __launch_bounds__(256) void advect_faces_batch_kernel(const void* __restrict__ plan, float* __restrict__ ws) {
-  const AdvBatchView v = adv_batch_view(plan);
-  for (int t = blockIdx.x; t < v.h->tiles_face; t += gridDim.x) {
-    const int i = __builtin_amdgcn_readfirstlane(find_item(v.face, v.h->n, t));
-    const AdvBatchItem& it = v.items[i];
-    faces_from_centered_grad_kernel_body(VBlock{t - v.face[i], 0, 0}, it.a.d, 1, ws + it.ws_gvc, (float*)it.gvx,
-                                         (float*)it.gvy, (float*)it.gvz);
   }
 }
 
@@ -1230,9 +1260,15 @@ int ngt_advect_bwd(const float* input, const int64_t* input_shape, const float* 
                                   input, output_grad, nullptr, nullptr, input_grad, tiles2, g_vc, vel_mode)));
     }
     const dim3 dgrid(blocks_for(p.a.d.x, kBX), blocks_for(p.a.d.y, kBY), static_cast<unsigned>(p.a.d.z * p.a.n_data));
-    NGT_SWITCH_C12(p.C, hipLaunchKernelGGL((advect_rho_gather_kernel<CC>), dgrid, block, 0, s, p.a, bg,
-                                           order == 2 && !direct ? r_dir : nullptr, tiles2, input_grad));
-    if (p.a.staggered) launch_faces(p, g_vc, vel_x_grad, vel_y_grad, vel_z_grad, s);
+    const float* rd_in = order == 2 && !direct ? r_dir : nullptr;
+    if (p.a.staggered && p.a.n_data == 1 && p.a.n_vel == 1) {
+      NGT_SWITCH_C12(p.C, hipLaunchKernelGGL((advect_rho_faces_kernel<CC>), dgrid, block, 0, s, p.a, bg, rd_in, tiles2,
+                                             input_grad, g_vc, vel_x_grad, vel_y_grad, vel_z_grad));
+    } else {
+      NGT_SWITCH_C12(p.C, hipLaunchKernelGGL((advect_rho_gather_kernel<CC>), dgrid, block, 0, s, p.a, bg, rd_in, tiles2,
+                                             input_grad));
+      if (p.a.staggered) launch_faces(p, g_vc, vel_x_grad, vel_y_grad, vel_z_grad, s);
+    }
     return check_launch("ngt_advect_bwd");
   }
   fill_zero(g_vc, static_cast<int64_t>(p.a.n_vel) * p.cells * 3, s);
@@ -1283,7 +1319,7 @@ size_t ngt_advect_batch_plan_bytes(int num_instances) {
   BlobLayout L;
   L.add<AdvBatchHeader>(1);
   L.add<AdvBatchItem>(num_instances);
-  for (int k = 0; k < 4; ++k) L.add<int>(num_instances + 1);
+  for (int k = 0; k < 3; ++k) L.add<int>(num_instances + 1);
   return L.bytes;
 }
 
@@ -1298,17 +1334,15 @@ int ngt_advect_batch_plan(const ngt_advect_instance* inst, int n, int order, int
   BlobLayout L;
   const size_t off_h = L.add<AdvBatchHeader>(1);
   const size_t off_items = L.add<AdvBatchItem>(n);
-  const size_t off_cell = L.add<int>(n + 1), off_brick = L.add<int>(n + 1), off_face = L.add<int>(n + 1),
-               off_fill = L.add<int>(n + 1);
+  const size_t off_cell = L.add<int>(n + 1), off_brick = L.add<int>(n + 1), off_fill = L.add<int>(n + 1);
   AdvBatchHeader* h = blob_at<AdvBatchHeader>(plan, off_h);
   AdvBatchItem* items = blob_at<AdvBatchItem>(plan, off_items);
   int* cell = blob_at<int>(plan, off_cell);
   int* brick = blob_at<int>(plan, off_brick);
-  int* face = blob_at<int>(plan, off_face);
   int* fill = blob_at<int>(plan, off_fill);
   const int staggered = inst[0].vel_y != nullptr ? 1 : 0;
-  int64_t tc = 0, tb = 0, tf = 0, tl = 0, ws = 0;
-  cell[0] = brick[0] = face[0] = fill[0] = 0;
+  int64_t tc = 0, tb = 0, tl = 0, ws = 0;
+  cell[0] = brick[0] = fill[0] = 0;
   for (int i = 0; i < n; ++i) {
     const ngt_advect_instance& q = inst[i];
     NGT_REQUIRE(q.dhw[0] > 0 && q.dhw[1] > 0 && q.dhw[2] > 0, NGT_ERR_INVALID_ARGUMENT, "empty batch instance");
@@ -1345,20 +1379,14 @@ int ngt_advect_batch_plan(const ngt_advect_instance* inst, int n, int order, int
     it.ws_t1 = order == 2 ? take(tiles) : 0;
     tc += static_cast<int64_t>(it.nbx) * it.nby * ((q.dhw[0] + kCellZ - 1) / kCellZ);
     tb += it.bg.count();
-    if (staggered) {
-      tf += face_tiles(it.a.d);
-    }
     tl += (it.cells + kFillChunk - 1) / kFillChunk;
-    NGT_REQUIRE(tc < (int64_t(1) << 31) && tl < (int64_t(1) << 31) && tf < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT,
-                "batch too large");
-    cell[i + 1] = static_cast<int>(tc); brick[i + 1] = static_cast<int>(tb);
-    face[i + 1] = static_cast<int>(tf); fill[i + 1] = static_cast<int>(tl);
+    NGT_REQUIRE(tc < (int64_t(1) << 31) && tl < (int64_t(1) << 31), NGT_ERR_INVALID_ARGUMENT, "batch too large");
+    cell[i + 1] = static_cast<int>(tc); brick[i + 1] = static_cast<int>(tb); fill[i + 1] = static_cast<int>(tl);
   }
   h->n = n; h->order = order; h->clamp = clamp; h->boundary = boundary; h->staggered = staggered;
   h->tiles_cell = static_cast<int>(tc); h->tiles_brick = static_cast<int>(tb);
-  h->tiles_face = static_cast<int>(tf); h->tiles_fill = static_cast<int>(tl);
-  h->off_items = off_items; h->off_cell = off_cell; h->off_brick = off_brick; h->off_face = off_face;
-  h->off_fill = off_fill;
+  h->tiles_fill = static_cast<int>(tl);
+  h->off_items = off_items; h->off_cell = off_cell; h->off_brick = off_brick; h->off_fill = off_fill;
   h->ws_bytes = ws * static_cast<int64_t>(sizeof(float));
   if (workspace_bytes) *workspace_bytes = static_cast<size_t>(h->ws_bytes);
   return NGT_OK;
@@ -1415,9 +1443,6 @@ int ngt_advect_batch_bwd(const void* plan_host, const void* plan_device, void* w
   }
   hipLaunchKernelGGL(advect_rho_gather_batch_kernel, dim3(persistent_grid(h.tiles_cell, blocks_per_cu)), b, 0, s,
                      plan_device, ws);
-  if (h.staggered)
-    hipLaunchKernelGGL(advect_faces_batch_kernel, dim3(persistent_grid(h.tiles_face, blocks_per_cu)), dim3(256), 0, s,
-                       plan_device, ws);
   return check_launch("ngt_advect_batch_bwd");
 }
 
