@@ -67,8 +67,8 @@ def parse():
 # kernels launched by each op of the step (substring of the kernel name)
 OP_KERNELS = {
     "advect_fwd": ["advect_sl_kernel", "advect_mc_kernel"],
-    "advect_bwd": ["advect_mc_bwd_brick_kernel", "advect_sl_bwd_brick_kernel", "advect_rho_gather_kernel",
-                   "faces_from_centered_grad_kernel", "fill_kernel"],
+    "advect_bwd": ["advect_mc_bwd_brick_kernel", "advect_sl_bwd_brick_kernel", "advect_rho_faces_kernel",
+                   "advect_rho_gather_kernel", "faces_from_centered_grad_kernel", "fill_kernel"],
     "render_fwd": ["to_bricked_kernel", "raymarch_fwd_record_kernel", "l2_finish_kernel"],
     "render_bwd": ["raymarch_bwd_brick_kernel"],
 }
