@@ -193,9 +193,10 @@ size_t ngt_raymarch_grid_transform_bwd_workspace_size(const int64_t* input_shape
 /* Host query, no counterpart in the reference: the per-ray sample bound the
  * gather adjoint sizes its per-sample record with (kcap above the BORDER ray
  * lengths of every pixel of every camera pair: (cell-centre box diagonal + 2)
- * / the exact minimum step length over the image, + slack); -1 when rays
- * are too long for a record and the scatter adjoint runs instead.  Pure host
- * computation (no device memory is touched). */
+ * / the exact minimum step length over the image, + slack, rounded up to a
+ * multiple of 4); -1 when rays are too long for a record and the scatter
+ * adjoint runs instead.  Pure host computation (no device memory is
+ * touched). */
 int ngt_raymarch_step_cap(const int64_t* input_shape, const float* matrix_m, const float* matrix_v,
                           const float* matrix_p, const float* frustum, int num_cameras, const int32_t* output_dhw,
                           int separate_camera_batch);
