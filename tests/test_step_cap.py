@@ -86,6 +86,7 @@ def test_step_cap_bounds_every_ray(lib, name, grid_kind):
     print("%s/%s: longest ray %d samples, bound %d" % (name, grid_kind, most, cap))
     assert most > 0
     assert 0 < most <= cap
+    assert cap % 4 == 0  # the record stores samples in quads of 4
 
 
 def test_step_cap_orthographic_and_general_projection(lib):
