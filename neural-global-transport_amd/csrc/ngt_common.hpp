@@ -320,9 +320,8 @@ NGT_DEVICE Vec<C> read_linear(const float* __restrict__ g, const Dims& d, float 
   return lerp8<C, BM>(v, c);
 }
 
-// The corner loads of U trilinear reads in the haloed brick layout, in
-// flight: bricked_taps_issue starts them, bricked_taps_finish interpolates
-// (a walk may issue the next round before it finishes this one).
+// The corner loads of U trilinear reads in the haloed brick layout:
+// bricked_taps_issue starts them, bricked_taps_finish interpolates.
 template <int C, int U>
 struct BrickedTaps {
   Vec<C> v[U][8];
@@ -340,9 +339,10 @@ NGT_DEVICE void bricked_taps_issue(const float* __restrict__ g, const Dims& d, c
     const float fx = floorf(qx[u]), fy = floorf(qy[u]), fz = floorf(qz[u]);
     t.wx[u] = qx[u] - fx; t.wy[u] = qy[u] - fy; t.wz[u] = qz[u] - fz;
     t.inside[u] = true;
-    // q + 0.5 in [0, d), compared as q in [-0.5, d - 0.5): the same decision
-    // for every float q below 2^22 (q + 0.5 is exact there, or, below -1,
-    // negative either way), without the three adds
+    // q + 0.5 in [0, d), compared as q in [-0.5, d - 0.5) without the three
+    // adds: the same decision for every float q below 2^22 (the rounding of
+    // q + 0.5 flips neither comparison: near -0.5 the sum is exact, and for
+    // q < d - 0.5 it never rounds up to d)
     if constexpr (BM == BORDER)
       t.inside[u] = (qx[u] >= -0.5f) & (qx[u] < static_cast<float>(d.x) - 0.5f) & (qy[u] >= -0.5f) &
                     (qy[u] < static_cast<float>(d.y) - 0.5f) & (qz[u] >= -0.5f) &

@@ -56,9 +56,10 @@ NGT_DEVICE int kMaxRaySteps(const RayArgs& a) { return 16 * (a.din.x + a.din.y +
 // Ray samples whose gathers are issued together, measured on the 128^3 /
 // 5-view render with one wavefront per 8x8 pixel tile: the image walk is
 // fastest with 2 (43.7 us; 1: 48.9, 4: 47.2), the recording walk (more work
-// per sample) with 4 (54.7 us; 1: 54.9, 2: 58.1).  With few rays per SIMD a
-// walk is bound by its per-sample recurrence as much as by the latency of its
-// corner loads, and wider rounds add registers and past-the-end samples.
+// per sample) with 4, one quad of its sample record per round (54.7 us; 1:
+// 54.9, 2: 58.1, 8: 59.6).  With few rays per SIMD a walk is bound by its
+// per-sample recurrence as much as by the latency of its corner loads, and
+// wider rounds add registers and past-the-end samples.
 constexpr int kUnroll = 2;
 constexpr int kRecordUnroll = 4;
 
@@ -182,8 +183,8 @@ NGT_DEVICE const float* ray_source(const RayArgs& a, int n) {
 
 // The light cell (rho rgb ds, rho ds) of a density sample.  Its products are
 // rounded on their own (never fused into a later add), so that every walk,
-// however its samples are unrolled or pipelined, forms the same optical
-// depths and images.
+// however its samples are unrolled, forms the same optical depths and
+// images.
 NGT_DEVICE Vec<4> light_cell(const RayArgs& a, float rho) {
 #pragma clang fp contract(off)
   const float s = rho * a.depth_step;
@@ -667,8 +668,9 @@ NGT_DEVICE void raymarch_fwd_record_pixel(const RayArgs& a, const CamBatch& cb, 
       // so it is NaN only when Q is, and the bound includes |Q_N|
       hmax = fmaxf(hmax, fabsf(h));
     }
-    // the whole quad (k < rec_steps <= kcap, both multiples of the quad's
-    // start): its samples past rec_steps are never read
+    // the whole quad: k is a multiple of 4 below rec_steps <= kcap, and kcap
+    // is a multiple of 4, so the quad lies inside the pixel's kcap samples;
+    // its samples past rec_steps are never read
     if (k < rec_steps) vstore<4>(gp + g_step(k), hq);
   }
   const Vec<4> acc = light_pixel<MODE>(a, dd, q);
