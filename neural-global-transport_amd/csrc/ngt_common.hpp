@@ -329,24 +329,34 @@ struct BrickedTaps {
   bool inside[U];
 };
 
+// The smallest float t with t + 0.5f >= d in float arithmetic (d >= 1 below
+// 2^22), so that q + 0.5f < d exactly when q < t: d - 0.5, except for d = 1,
+// where the largest float below 0.5 plus 0.5 rounds (to even) up to 1.
+NGT_DEVICE float validity_upper(int d) {
+  const float t = static_cast<float>(d) - 0.5f;
+  const float below = __int_as_float(__float_as_int(t) - 1);
+  return below + 0.5f >= static_cast<float>(d) ? below : t;
+}
+
 template <int C, int BM, int U>
 NGT_DEVICE void bricked_taps_issue(const float* __restrict__ g, const Dims& d, const BrickDims& b,
                                    const float (&qx)[U], const float (&qy)[U], const float (&qz)[U],
                                    BrickedTaps<C, U>& t) {
+  // q + 0.5 in [0, d) per axis, compared without the three adds as
+  // q >= -0.5 (the sum is exact near -0.5) and q < validity_upper(d)
+  float hx = 0.f, hy = 0.f, hz = 0.f;
+  if constexpr (BM == BORDER) {
+    hx = validity_upper(d.x); hy = validity_upper(d.y); hz = validity_upper(d.z);
+  }
   // one address per sample, no per-corner masks
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const float fx = floorf(qx[u]), fy = floorf(qy[u]), fz = floorf(qz[u]);
     t.wx[u] = qx[u] - fx; t.wy[u] = qy[u] - fy; t.wz[u] = qz[u] - fz;
     t.inside[u] = true;
-    // q + 0.5 in [0, d), compared as q in [-0.5, d - 0.5) without the three
-    // adds: the same decision for every float q below 2^22 (the rounding of
-    // q + 0.5 flips neither comparison: near -0.5 the sum is exact, and for
-    // q < d - 0.5 it never rounds up to d)
     if constexpr (BM == BORDER)
-      t.inside[u] = (qx[u] >= -0.5f) & (qx[u] < static_cast<float>(d.x) - 0.5f) & (qy[u] >= -0.5f) &
-                    (qy[u] < static_cast<float>(d.y) - 0.5f) & (qz[u] >= -0.5f) &
-                    (qz[u] < static_cast<float>(d.z) - 0.5f);
+      t.inside[u] = (qx[u] >= -0.5f) & (qx[u] < hx) & (qy[u] >= -0.5f) & (qy[u] < hy) & (qz[u] >= -0.5f) &
+                    (qz[u] < hz);
     const uint32_t base = brick_corner<BM>(d, b, fx, fy, fz);
     const float* p = g + base * static_cast<uint32_t>(C);
 #pragma unroll
