@@ -86,6 +86,36 @@ def test_fused_l2_loss_and_gradient(ops, bound, gather, mode):
     assert rel(host(rt.grad), eg) < GRAD_TOL
 
 
+@pytest.mark.parametrize("dims", [(1, 11, 13), (12, 1, 13), (12, 11, 1)])
+def test_fused_l2_grid_of_extent_one(ops, dims):
+    """Grids one cell thick along an axis (the bricked walks' validity bound
+    there is the float below 0.5, not 0.5): images and loss against the
+    oracle, the recording walk's image against the image walk's bit for bit,
+    and the density gradient against the oracle adjoint."""
+    rng = np.random.default_rng(64 + sum(dims))
+    (M, V, P, F), ds, out_dhw = _scene(dims)
+    rho = f32(rng.uniform(0.0, 0.5, (1,) + dims + (1,)))
+    tgt = f32(rng.uniform(0.0, 0.3, (1, 3) + out_dhw[1:] + (3,)))
+    print("case: fused L2 on a %s grid" % (dims,), flush=True)
+    rt = dev(rho).requires_grad_()
+    loss, img = ops.render_density_fused_l2(rt, dev(tgt), (1, 1, 1), ds, M, V, P, F, out_dhw)
+    loss.backward()
+    img2 = ops.render_density_fused(dev(rho), (1, 1, 1), ds, M, V, P, F, out_dhw)
+    assert torch.equal(img.detach(), img2.detach())
+    cat = np.concatenate([rho, rho, rho, rho], -1) * ds
+    eimg = RM.raymarch_grid_transform(cat, M, V, P, F, out_dhw, LINEAR, BORDER, B.BEER_LAMBERT)
+    if dims[1] > 1:  # a slab one cell high lies edge-on to the horizontal ring of cameras
+        assert np.max(np.abs(eimg[..., 3])) > 0
+    assert rel(host(img), eimg) < FWD_TOL
+    eloss = float(np.sum((eimg[..., :3] - tgt) ** 2))
+    assert abs(float(loss) - eloss) / abs(eloss) < FWD_TOL
+    gimg = np.zeros_like(eimg)
+    gimg[..., :3] = 2.0 * (eimg[..., :3] - tgt)
+    g_cat = RM.raymarch_grid_transform_grad(cat, eimg, gimg, M, V, P, F, out_dhw, LINEAR, BORDER, B.BEER_LAMBERT)
+    eg = ds * np.sum(g_cat, axis=-1, keepdims=True)
+    assert rel(host(rt.grad), eg) < GRAD_TOL
+
+
 def test_fused_l2_matches_unfused_composition(ops):
     rng = np.random.default_rng(62)
     dims = (12, 11, 13)
