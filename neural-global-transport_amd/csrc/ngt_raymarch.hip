@@ -816,37 +816,58 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
       vmin = fminf(vmin, __shfl_xor(vmin, o, 64)); vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
       behind |= __shfl_xor(behind, o, 64);
     }
-    if (c == 0 && ci < cb.count) {
-      int x0 = 0, y0 = 0, nx = 0, ny = 0;
-      if (view) {
-        int x1 = a.out_w - 1, y1 = a.out_h - 1;
-        if (!behind) {
-          // clamp before the float -> int conversions (far-off corners project anywhere)
-          const float fw = static_cast<float>(a.out_w), fh = static_cast<float>(a.out_h);
-          umin = fminf(fmaxf(umin, -2.f), fw + 1.f); umax = fminf(fmaxf(umax, -2.f), fw + 1.f);
-          vmin = fminf(fmaxf(vmin, -2.f), fh + 1.f); vmax = fminf(fmaxf(vmax, -2.f), fh + 1.f);
-          x0 = max(0, static_cast<int>(ceilf(umin - 1e-3f))); x1 = min(x1, static_cast<int>(floorf(umax + 1e-3f)));
-          y0 = max(0, static_cast<int>(ceilf(vmin - 1e-3f))); y1 = min(y1, static_cast<int>(floorf(vmax + 1e-3f)));
-        }
-        nx = max(0, x1 - x0 + 1);
-        ny = max(0, y1 - y0 + 1);
+    // every lane of a view's group holds its rectangle; lane 0 of the group
+    // stores it, and the groups' ray counts are scanned with shuffles
+    int x0 = 0, y0 = 0, nx = 0, ny = 0;
+    if (view) {
+      int x1 = a.out_w - 1, y1 = a.out_h - 1;
+      if (!behind) {
+        // clamp before the float -> int conversions (far-off corners project anywhere)
+        const float fw = static_cast<float>(a.out_w), fh = static_cast<float>(a.out_h);
+        umin = fminf(fmaxf(umin, -2.f), fw + 1.f); umax = fminf(fmaxf(umax, -2.f), fw + 1.f);
+        vmin = fminf(fmaxf(vmin, -2.f), fh + 1.f); vmax = fminf(fmaxf(vmax, -2.f), fh + 1.f);
+        x0 = max(0, static_cast<int>(ceilf(umin - 1e-3f))); x1 = min(x1, static_cast<int>(floorf(umax + 1e-3f)));
+        y0 = max(0, static_cast<int>(ceilf(vmin - 1e-3f))); y1 = min(y1, static_cast<int>(floorf(vmax + 1e-3f)));
       }
+      nx = max(0, x1 - x0 + 1);
+      ny = max(0, y1 - y0 + 1);
+    }
+    const int cnt = nx * ny;  // 0 for the groups past cb.count
+    int begin = 0, all = 0;
+#pragma unroll
+    for (int g = 0; g < kMaxCamsPerLaunch; ++g) {
+      const int cg = __shfl(cnt, g * 8, 64);
+      begin += g < ci ? cg : 0;
+      all += cg;
+    }
+    if (c == 0 && ci < cb.count) {
       rect[ci][0] = x0; rect[ci][1] = y0; rect[ci][2] = nx; rect[ci][3] = ny;
+      ray_begin[ci] = begin;
     }
-  }
-  __syncthreads();
-  if (tid == 0) {
-    int total = 0;
-    for (int ci = 0; ci < cb.count; ++ci) {
-      ray_begin[ci] = total;
-      total += rect[ci][2] * rect[ci][3];
-    }
-    ray_begin[cb.count] = total;
+    if (tid == 0) ray_begin[cb.count] = all;
   }
   __syncthreads();
 
   const int64_t npix = static_cast<int64_t>(a.out_w) * a.out_h;
   const int total = ray_begin[cb.count];
+  const int fb0m1x = b0x - 1, fb0m1y = b0y - 1, fb0m1z = b0z - 1;
+  // ray r of the brick's list (r < total): its record, and the G offset of its sample 0
+  auto fetch_ray = [&](int r, RayRec& rr, int64_t& gofs) {
+    int ci = 0;
+    while (ci + 1 < cb.count && r >= ray_begin[ci + 1]) ++ci;
+    const int local = r - ray_begin[ci];
+    const int px = rect[ci][0] + local % rect[ci][2];
+    const int py = rect[ci][1] + local / rect[ci][2];
+    gofs = static_cast<int64_t>(pair_base + ci) * kcap * g_pixels(npix) +
+           g_offset(0, static_cast<int64_t>(py) * a.out_w + px, kcap);
+    rr = rays[(pair_base + ci) * static_cast<int>(npix) + py * a.out_w + px];
+  };
+  // each work-item's ray of the next chunk is loaded one chunk ahead, so that
+  // its latency overlaps the current chunk's samples (the first one overlaps
+  // the gradient bounds' loads)
+  RayRec next_rr{};
+  int64_t next_gofs = 0;
+  if (tid < total) fetch_ray(tid, next_rr, next_gofs);
   // the brick's fixed-point unit from the largest gradient bound of its rays' tiles
   {
     float m = 0.f;
@@ -902,23 +923,6 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   __shared__ float4 s_pd[256];     // each ray's step y, z and loss-gradient scale, offset
   __shared__ int s_wtot[4];
   __shared__ unsigned char s_ray[kItems];
-  const int fb0m1x = b0x - 1, fb0m1y = b0y - 1, fb0m1z = b0z - 1;
-  // ray r of the brick's list (r < total): its record, and the G offset of its sample 0
-  auto fetch_ray = [&](int r, RayRec& rr, int64_t& gofs) {
-    int ci = 0;
-    while (ci + 1 < cb.count && r >= ray_begin[ci + 1]) ++ci;
-    const int local = r - ray_begin[ci];
-    const int px = rect[ci][0] + local % rect[ci][2];
-    const int py = rect[ci][1] + local / rect[ci][2];
-    gofs = static_cast<int64_t>(pair_base + ci) * kcap * g_pixels(npix) +
-           g_offset(0, static_cast<int64_t>(py) * a.out_w + px, kcap);
-    rr = rays[(pair_base + ci) * static_cast<int>(npix) + py * a.out_w + px];
-  };
-  // each work-item's ray of the next chunk is loaded one chunk ahead, so that
-  // its latency overlaps the current chunk's samples
-  RayRec next_rr{};
-  int64_t next_gofs = 0;
-  if (tid < total) fetch_ray(tid, next_rr, next_gofs);
   for (int base_r = 0; base_r < total; base_r += 256) {
     const int r = base_r + tid;
     const RayRec rr = next_rr;
