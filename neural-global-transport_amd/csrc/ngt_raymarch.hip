@@ -1033,17 +1033,40 @@ NGT_DEVICE void raymarch_bwd_brick_kernel_body(const VBlock& vb, const RayArgs& 
   __syncthreads();
   const int64_t base = static_cast<int64_t>(n) * a.din.x * a.din.y * a.din.z;
   const float scale = grad_scale ? grad_scale[0] : 1.f;
-  for (int i = tid; i < NV; i += blockDim.x) {
+  // the brick's own voxels, kOwn per work-item: every value is formed first,
+  // then all are stored together (a store loop that read-added per voxel
+  // behind a select waited for each voxel's load in turn: 11 us at 128^3)
+  constexpr int kOwn = (NV + 255) / 256;
+  float vals[kOwn][CG];
+  int64_t offs[kOwn];
+  bool own[kOwn];
+#pragma unroll
+  for (int m = 0; m < kOwn; ++m) {
+    const int i = tid + 256 * m;
     const int lx = i % BX, ly = (i / BX) % BY, lz = i / (BX * BY);
     const int x = b0x + lx, y = b0y + ly, z = b0z + lz;
-    if (x > b1x || y > b1y || z > b1z) continue;
-    float* d = in_grad + (base + cell_offset(a.din, x, y, z)) * CG;
-    const int pi = ((lz + 1) * PY + ly + 1) * PX + lx + 1;
+    own[m] = i < NV && x <= b1x && y <= b1y && z <= b1z;
+    offs[m] = (base + cell_offset(a.din, min(x, b1x), min(y, b1y), min(z, b1z))) * CG;
+    const int pi = ((min(lz, BZ - 1) + 1) * PY + min(ly, BY - 1) + 1) * PX + min(lx, BX - 1) + 1;
 #pragma unroll
     for (int c = 0; c < CG; ++c) {
       const long long q = static_cast<long long>(acc[c][pi]);
-      const float v = finite ? ldexpf(static_cast<float>(q), -S) * scale : __int_as_float(0x7fc00000);
-      d[c] = accumulate ? d[c] + v : v;
+      vals[m][c] = finite ? ldexpf(static_cast<float>(q), -S) * scale : __int_as_float(0x7fc00000);
+    }
+  }
+  if (accumulate) {
+#pragma unroll
+    for (int m = 0; m < kOwn; ++m) {
+      if (!own[m]) continue;
+#pragma unroll
+      for (int c = 0; c < CG; ++c) in_grad[offs[m] + c] += vals[m][c];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < kOwn; ++m) {
+      if (!own[m]) continue;
+#pragma unroll
+      for (int c = 0; c < CG; ++c) in_grad[offs[m] + c] = vals[m][c];
     }
   }
 }
