@@ -563,24 +563,30 @@ NGT_DEVICE void scatter_linear_tile(float* tile, HeldPlane<C>& held, bool on, in
 template <int C>
 NGT_DEVICE Vec<C> gather_tiles(const float* __restrict__ tiles, int64_t slot, const BrickGrid& bg, int x, int y,
                                int z) {
-  Vec<C> s = vzero<C>();
+  // the (up to) 8 bricks whose tiles hold the cell: every load is issued
+  // before any is added, and the sum runs in the order z, y, x of the bricks
   const int bx1 = min((x + kHalo) / kBrickX, bg.nx - 1), by1 = min((y + kHalo) / kBrickY, bg.ny - 1);
   const int bz1 = min((z + kHalo) / kBrickZ, bg.nz - 1);
-  for (int bz = max(bz1 - 1, 0); bz <= bz1; ++bz) {
-    const int lz = z - (bz * kBrickZ - kHalo);
-    if (lz < 0 || lz >= kTileZ) continue;
-    for (int by = max(by1 - 1, 0); by <= by1; ++by) {
-      const int ly = y - (by * kBrickY - kHalo);
-      if (ly < 0 || ly >= kTileY) continue;
-      for (int bx = max(bx1 - 1, 0); bx <= bx1; ++bx) {
-        const int lx = x - (bx * kBrickX - kHalo);
-        if (lx < 0 || lx >= kTileX) continue;
-        const int64_t brick = (slot * bg.nz + bz) * bg.ny * bg.nx + static_cast<int64_t>(by) * bg.nx + bx;
-        const Vec<C> t = vload<C>(tiles + (brick * kTileCells + (lz * kTileY + ly) * kTileX + lx) * C);
+  Vec<C> t[8];
+  bool ok[8];
 #pragma unroll
-        for (int i = 0; i < C; ++i) s.v[i] += t.v[i];
-      }
+  for (int q = 0; q < 8; ++q) {
+    const int bz = bz1 - 1 + (q >> 2), by = by1 - 1 + ((q >> 1) & 1), bx = bx1 - 1 + (q & 1);
+    const int lz = z - (bz * kBrickZ - kHalo), ly = y - (by * kBrickY - kHalo), lx = x - (bx * kBrickX - kHalo);
+    ok[q] = bz >= 0 && by >= 0 && bx >= 0 && lz >= 0 && lz < kTileZ && ly >= 0 && ly < kTileY && lx >= 0 &&
+            lx < kTileX;
+    t[q] = vzero<C>();
+    if (ok[q]) {
+      const int64_t brick = (slot * bg.nz + bz) * bg.ny * bg.nx + static_cast<int64_t>(by) * bg.nx + bx;
+      t[q] = vload<C>(tiles + (brick * kTileCells + (lz * kTileY + ly) * kTileX + lx) * C);
     }
+  }
+  Vec<C> s = vzero<C>();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    if (!ok[q]) continue;
+#pragma unroll
+    for (int i = 0; i < C; ++i) s.v[i] += t[q].v[i];
   }
   return s;
 }
